@@ -1,0 +1,166 @@
+/*
+ * sfp.h -- C ABI of the MI355X (gfx950) grid Poisson solves that replace the
+ * amgcl calls of SimFerrofluid's Pressure and MagneticField classes.
+ *
+ * Plain C types only: ints, doubles and pointers. Every 3-D array is in the
+ * reference's z-fastest order, index = z + nz*(y + ny*x) (core/Grid.h:28);
+ * face array a has extent (n + e_a) along its own axis
+ * (core/StaggeredGrid.h:15-19). Host-pointer entry points copy to and from
+ * the device; *_device entry points take device pointers and are ordered on
+ * the context's HIP stream (sfp_stream).
+ *
+ * A context is bound to one grid (resolution, spacing, origin) and one GPU and
+ * is not reentrant: calls on one context must not overlap, matching the
+ * single-threaded use of Pressure/MagneticField in the reference
+ * (demo/Driver.cpp:144).
+ */
+#ifndef SFP_H
+#define SFP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFP_ABI_VERSION 1
+
+typedef enum sfp_status {
+    SFP_OK = 0,
+    SFP_ERR_ARGUMENT = 1,    /* null pointer, bad size or parameter */
+    SFP_ERR_INPUT = 2,       /* an open (w > 0) face on the grid boundary next to liquid:
+                                the reference indexes outside the grid there */
+    SFP_ERR_HIP = 3,         /* HIP runtime error; see sfp_last_error */
+    SFP_ERR_NO_DEVICE = 4,   /* no usable GPU */
+    SFP_ERR_STATE = 5,       /* call out of order (e.g. reproject before project) */
+    SFP_ERR_UNSUPPORTED = 6  /* e.g. learned preconditioner without a network */
+} sfp_status;
+
+typedef enum sfp_precond {
+    SFP_PRECOND_MG = 0,      /* FP32 geometric-multigrid V-cycle inside FP64 flexible CG */
+    SFP_PRECOND_LEARNED = 1, /* 3-D U-Net (BF16, MFMA) inside FP64 flexible CG, MG fallback */
+    SFP_PRECOND_JACOBI = 2   /* diagonal scaling (reference point for tests) */
+} sfp_precond;
+
+typedef struct sfp_params {
+    double tol;          /* stop when ||b - A x||_2 / ||b||_2 <= tol (FP64) */
+    int max_iters;       /* outer flexible-CG iteration cap */
+    int precond;         /* sfp_precond */
+    int mg_coarsest;     /* coarsen until every axis has <= this many cells (>= 2) */
+    int cheby_degree;    /* Chebyshev smoother degree, pre and post (1..8) */
+    double cheby_ratio;  /* smoothing interval [lambda_max/ratio, lambda_max] */
+    double coarse_scale; /* coarse operator = scale * Galerkin(aggregation) */
+    double stall_ratio;  /* learned preconditioner: fall back to MG when
+                            ||r_k|| / ||r_{k-1}|| exceeds this */
+    int verbose;         /* 1: print "{iters} iters" per solve like the reference */
+} sfp_params;
+
+typedef struct sfp_stats {
+    int iterations;      /* outer iterations */
+    int converged;       /* 1 when rel_residual <= tol */
+    int fallback;        /* learned preconditioner: iterations done after falling back to MG */
+    int precond_used;    /* sfp_precond the solve ended with */
+    double rel_residual; /* final FP64 ||b - A x|| / ||b||, recomputed from x */
+    double setup_ms;     /* assembly + hierarchy (GPU time) */
+    double solve_ms;     /* Krylov iterations (GPU time) */
+    double apply_ms;     /* velocity / field update (GPU time) */
+    long long unknowns;  /* active rows of the system */
+} sfp_stats;
+
+typedef struct sfp_context_s *sfp_context;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* Create a context for an nx*ny*nz cell grid with the given spacing and
+ * lower-corner origin (the simulation StaggeredGrid's GetOrigin()), on HIP
+ * device `device`. Replaces the allocations of Pressure::Pressure
+ * (core/Pressure.cpp:13-15) and MagneticField::MagneticField
+ * (core/MagneticField.cpp:11-27). */
+sfp_status sfp_create(sfp_context *out, int nx, int ny, int nz, double spacing,
+                      const double origin[3], int device);
+void sfp_destroy(sfp_context ctx);
+
+void sfp_default_params(sfp_params *params);
+sfp_status sfp_set_params(sfp_context ctx, const sfp_params *params);
+sfp_status sfp_get_params(sfp_context ctx, sfp_params *params);
+
+/* Text of the last error on this context (or of the last failed sfp_create
+ * when ctx is NULL). */
+const char *sfp_last_error(sfp_context ctx);
+const char *sfp_status_string(sfp_status status);
+/* The context's HIP stream, as a hipStream_t cast to void*. */
+void *sfp_stream(sfp_context ctx);
+/* Drop the magnetic solve's warm start (next solve starts from zero). */
+sfp_status sfp_reset_warm_start(sfp_context ctx);
+
+/* ---- pressure projection ---------------------------------------------- */
+
+/* Pressure::Project (core/Pressure.cpp:17-26): unknowns = cells with
+ * level_set <= 0 (SetUnKnowns :104-123), ghost-fluid matrix and divergence
+ * right-hand side (BuildProjectionMatrix :38-78), solve from zero
+ * (SolveLinearSystem :125-135), and the velocity update (ApplyProjection
+ * :137-166), done in place on `velocity`.
+ *   fraction[a], solid_velocity[a]: Collider::GetFraction() / Collider::Velocity
+ *   jump[a]: per-face value of the pressure-jump callback (SetPressureJump,
+ *            core/Pressure.h:20-22), or jump == NULL for no callback
+ *   pressure: optional N^3 output, the solution on liquid cells and 0 elsewhere
+ * Returns SFP_OK also when the solve did not reach tol (the reference accepts
+ * non-convergence); see stats->converged. */
+sfp_status sfp_pressure_project(sfp_context ctx, double *const velocity[3],
+                                const double *level_set, const double *const fraction[3],
+                                const double *const solid_velocity[3],
+                                const double *const jump[3], double vol_error,
+                                double *pressure, sfp_stats *stats);
+
+/* Pressure::Reproject (core/Pressure.cpp:28-36): same unknowns and matrix as
+ * the last sfp_pressure_project, new right-hand side without jump terms
+ * (SetRightHandSide :80-102), warm start from the last pressure. */
+sfp_status sfp_pressure_reproject(sfp_context ctx, double *const velocity[3],
+                                  const double *level_set, const double *const fraction[3],
+                                  const double *const solid_velocity[3],
+                                  const double *const jump[3], double vol_error,
+                                  double *pressure, sfp_stats *stats);
+
+/* ---- FDM magnetostatic potential --------------------------------------- */
+
+/* MagneticField::Solve (core/MagneticField.cpp:29-35): face permeability
+ * mu = 1 + (1 - H_band(phi_face)) * chi (BuildMatrix :50-60), the pinned
+ * 7-point mu-Laplacian (:62-90), right-hand side from (mu-1) H_ext
+ * (AssembleRHS :97-118), a solve warm-started from the previous call
+ * (SolveLinearSystem :120-123), and H = H_ext - grad(potential)
+ * (AssignTotalField :125-143).
+ *   potential: optional N^3 output; total_field: optional face outputs. */
+sfp_status sfp_magnetic_solve(sfp_context ctx, const double *level_set,
+                              const double *const external_field[3], double susceptibility,
+                              double band_width, double *potential,
+                              double *const total_field[3], sfp_stats *stats);
+
+/* MagneticField::CalculateVertexPressures (core/MagneticField.cpp:37-48) on
+ * the total field of the last sfp_magnetic_solve: positions and normals are
+ * nverts x 3 doubles (x,y,z per vertex); pressures gets nverts doubles. */
+sfp_status sfp_magnetic_vertex_pressures(sfp_context ctx, int nverts, const double *positions,
+                                         const double *normals, double susceptibility,
+                                         double *pressures);
+
+/* ---- device-resident variants ------------------------------------------ */
+/* Same semantics; every array pointer is a device pointer, work is enqueued
+ * on sfp_stream(ctx) and the call returns after the solve finished. */
+sfp_status sfp_pressure_project_device(sfp_context ctx, double *const velocity[3],
+                                       const double *level_set, const double *const fraction[3],
+                                       const double *const solid_velocity[3],
+                                       const double *const jump[3], double vol_error,
+                                       double *pressure, sfp_stats *stats);
+sfp_status sfp_pressure_reproject_device(sfp_context ctx, double *const velocity[3],
+                                         const double *level_set,
+                                         const double *const fraction[3],
+                                         const double *const solid_velocity[3],
+                                         const double *const jump[3], double vol_error,
+                                         double *pressure, sfp_stats *stats);
+sfp_status sfp_magnetic_solve_device(sfp_context ctx, const double *level_set,
+                                     const double *const external_field[3],
+                                     double susceptibility, double band_width, double *potential,
+                                     double *const total_field[3], sfp_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFP_H */

@@ -1,0 +1,36 @@
+/*
+ * sfp_test.h -- inspection hooks used by the parity tests. They expose the
+ * assembled operators, the multigrid hierarchy and single kernels so each
+ * can be checked against the oracle on its own. No counterpart in the
+ * reference; not needed by a drop-in caller.
+ */
+#ifndef SFP_TEST_H
+#define SFP_TEST_H
+#include "sfp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* which: 0 = pressure system (last project), 1 = magnetic system (last solve) */
+sfp_status sfp_test_num_levels(sfp_context ctx, int which, int *levels);
+sfp_status sfp_test_level_dims(sfp_context ctx, int which, int level, int dims[3]);
+/* FP64 operator of a level: diag and the 3 lower-face couplings (cells of that
+ * level each); rhs (fine level only, may be NULL). Host pointers. */
+sfp_status sfp_test_export_level(sfp_context ctx, int which, int level, double *diag,
+                                 double *cpl_x, double *cpl_y, double *cpl_z, double *rhs);
+/* FP32 operator of a level (diag is 0 on inactive cells). */
+sfp_status sfp_test_export_level32(sfp_context ctx, int which, int level, float *diag,
+                                   float *cpl_x, float *cpl_y, float *cpl_z);
+/* w = A z with the fine FP64 operator (z given in FP32, as the solver feeds it). */
+sfp_status sfp_test_apply64(sfp_context ctx, int which, const float *z, double *w);
+/* z = B r, one application of the configured preconditioner. */
+sfp_status sfp_test_precondition(sfp_context ctx, int which, const double *r, float *z);
+/* r = rhs - A x with the fine FP64 operator, plus ||r||^2 and ||rhs||^2. */
+sfp_status sfp_test_residual(sfp_context ctx, int which, const double *x, double *r,
+                             double *rr, double *bb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SFP_TEST_H */

@@ -1,0 +1,159 @@
+// Inspection hooks of include/sfp_test.h (operators, hierarchy, single kernels).
+#include "../../include/sfp_test.h"
+#include "context.hpp"
+
+#include <vector>
+
+using namespace sfp;
+
+namespace {
+
+template <class F>
+sfp_status run(sfp_context ctx, F &&f) {
+    if (!ctx) return SFP_ERR_ARGUMENT;
+    try {
+        SFP_HIP(hipSetDevice(ctx->device));
+        f();
+        return SFP_OK;
+    } catch (const ArgError &e) {
+        ctx->last_error = e.what();
+        return SFP_ERR_ARGUMENT;
+    } catch (const StateError &e) {
+        ctx->last_error = e.what();
+        return SFP_ERR_STATE;
+    } catch (const HipError &e) {
+        ctx->last_error = e.what();
+        return SFP_ERR_HIP;
+    } catch (const std::exception &e) {
+        ctx->last_error = e.what();
+        return SFP_ERR_ARGUMENT;
+    }
+}
+
+System &pick(sfp_context ctx, int which) {
+    if (which == 0) {
+        if (!ctx->pressure_ready || ctx->pressure_unknowns == 0)
+            throw StateError("no pressure system assembled");
+        return ctx->pressure;
+    }
+    if (which == 1) {
+        if (!ctx->field_ready) throw StateError("no magnetic system assembled");
+        return ctx->magnetic;
+    }
+    throw ArgError("which must be 0 (pressure) or 1 (magnetic)");
+}
+
+template <class T>
+void d2h(sfp_context ctx, T *h, const T *d, size_t n) {
+    if (!h) return;
+    SFP_HIP(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, ctx->ws.stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+sfp_status sfp_test_num_levels(sfp_context ctx, int which, int *levels) {
+    return run(ctx, [&] {
+        if (!levels) throw ArgError("null levels");
+        *levels = pick(ctx, which).num_levels();
+    });
+}
+
+sfp_status sfp_test_level_dims(sfp_context ctx, int which, int level, int dims[3]) {
+    return run(ctx, [&] {
+        System &s = pick(ctx, which);
+        if (level < 0 || level >= s.num_levels() || !dims) throw ArgError("bad level");
+        const Dims d = s.level(level).d;
+        dims[0] = d.nx;
+        dims[1] = d.ny;
+        dims[2] = d.nz;
+    });
+}
+
+sfp_status sfp_test_export_level(sfp_context ctx, int which, int level, double *diag,
+                                 double *cpl_x, double *cpl_y, double *cpl_z, double *rhs) {
+    return run(ctx, [&] {
+        System &s = pick(ctx, which);
+        if (level < 0 || level >= s.num_levels()) throw ArgError("bad level");
+        Level &L = s.level(level);
+        const size_t n = L.d.cells();
+        d2h(ctx, diag, L.diag64.p, n);
+        d2h(ctx, cpl_x, L.cpl64[0].p, n);
+        d2h(ctx, cpl_y, L.cpl64[1].p, n);
+        d2h(ctx, cpl_z, L.cpl64[2].p, n);
+        if (rhs && level == 0) d2h(ctx, rhs, s.rhs().p, n);
+        SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
+    });
+}
+
+sfp_status sfp_test_export_level32(sfp_context ctx, int which, int level, float *diag,
+                                   float *cpl_x, float *cpl_y, float *cpl_z) {
+    return run(ctx, [&] {
+        System &s = pick(ctx, which);
+        if (level < 0 || level >= s.num_levels()) throw ArgError("bad level");
+        Level &L = s.level(level);
+        const size_t n = L.d.cells();
+        d2h(ctx, diag, L.diag32.p, n);
+        d2h(ctx, cpl_x, L.cpl32[0].p, n);
+        d2h(ctx, cpl_y, L.cpl32[1].p, n);
+        d2h(ctx, cpl_z, L.cpl32[2].p, n);
+        SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
+    });
+}
+
+sfp_status sfp_test_apply64(sfp_context ctx, int which, const float *z, double *w) {
+    return run(ctx, [&] {
+        System &s = pick(ctx, which);
+        if (!z || !w) throw ArgError("null vector");
+        const Dims d = s.dims();
+        const size_t n = d.cells();
+        DevBuf<float> dz;
+        DevBuf<double> dr, dw;
+        dz.ensure(n);
+        dr.ensure(n);
+        dw.ensure(n);
+        SFP_HIP(hipMemcpyAsync(dz.p, z, n * sizeof(float), hipMemcpyHostToDevice, ctx->ws.stream));
+        SFP_HIP(hipMemsetAsync(dr.p, 0, n * sizeof(double), ctx->ws.stream));
+        launch_fcg_matvec(ctx->ws.stream, d, s.fine().op64(), dz.p, dr.p, nullptr, dw.p, 1,
+                          ctx->ws.partials.p, ctx->ws.counter.p, ctx->ws.scalars.p);
+        d2h(ctx, w, dw.p, n);
+        SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
+    });
+}
+
+sfp_status sfp_test_precondition(sfp_context ctx, int which, const double *r, float *z) {
+    return run(ctx, [&] {
+        System &s = pick(ctx, which);
+        if (!r || !z) throw ArgError("null vector");
+        const size_t n = s.dims().cells();
+        DevBuf<double> dr;
+        dr.ensure(n);
+        SFP_HIP(hipMemcpyAsync(dr.p, r, n * sizeof(double), hipMemcpyHostToDevice, ctx->ws.stream));
+        const float *dz = s.precondition(ctx->ws, ctx->cfg, dr.p);
+        d2h(ctx, z, dz, n);
+        SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
+    });
+}
+
+sfp_status sfp_test_residual(sfp_context ctx, int which, const double *x, double *r, double *rr,
+                             double *bb) {
+    return run(ctx, [&] {
+        System &s = pick(ctx, which);
+        if (!x) throw ArgError("null x");
+        const Dims d = s.dims();
+        const size_t n = d.cells();
+        DevBuf<double> dx, dr;
+        dx.ensure(n);
+        dr.ensure(n);
+        SFP_HIP(hipMemcpyAsync(dx.p, x, n * sizeof(double), hipMemcpyHostToDevice, ctx->ws.stream));
+        launch_residual64(ctx->ws.stream, d, s.fine().op64(), s.rhs().p, dx.p, dr.p,
+                          ctx->ws.partials.p, ctx->ws.counter.p, ctx->ws.scalars.p);
+        d2h(ctx, r, dr.p, n);
+        ctx->ws.read_scalars();
+        if (rr) *rr = ctx->ws.host_sc->rr;
+        if (bb) *bb = ctx->ws.host_sc->bb;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Host-side launchers of the gfx950 kernels (defined in the .hip files).
+#pragma once
+#include "sfp_internal.hpp"
+
+namespace sfp {
+
+// ---------------------------------------------------------------- multigrid (mg.hip)
+// Chebyshev step modes.
+enum SmoothMode : int {
+    SM_D_FROM_X = 1,   // previous direction equals the input iterate (step after a zero start)
+    SM_READ_D = 2,     // previous direction read from dvec
+    SM_WRITE_D = 4,    // store the new direction
+    SM_PROLONG = 8     // input iterate is x + P e (coarse correction fused in)
+};
+
+// x = c0 * D^{-1} b from a zero start; at the fine level b is first rounded
+// from the FP64 residual r64 (and stored to b).
+void launch_smooth_first(hipStream_t s, Dims d, Op32 A, const double *r64, float *b, float *x,
+                         float c0);
+// xout = xin' + d', d' = cd * d + cr * D^{-1} (b - A xin'), xin' = xin (+ P e).
+void launch_smooth(hipStream_t s, int mode, Dims d, Op32 A, const float *b, const float *xin,
+                   float *xout, float *dvec, float cd, float cr, const float *ecoarse);
+// bc = R (b - A x): sum of the residuals of the (up to 8) children.
+void launch_restrict(hipStream_t s, Dims df, Op32 A, const float *b, const float *x, float *bc);
+// Galerkin coarsening by 2x2x2 aggregation, scaled: writes the coarse FP64
+// and FP32 operators.
+void launch_coarsen(hipStream_t s, Dims df, Op64 Af, const float *diag32f, Dims dc,
+                    double scale, double *diag64c, double *const cpl64c[3], float *diag32c,
+                    float *const cpl32c[3]);
+// Dense inverse of the coarsest operator (n = cells <= kMaxCoarse) into ainv (n*n floats).
+constexpr int kMaxCoarse = 64;
+void launch_coarse_invert(hipStream_t s, Dims d, Op64 A, float *ainv);
+void launch_coarse_solve(hipStream_t s, int n, const float *ainv, const float *b, float *x);
+
+// ---------------------------------------------------------------- Krylov (krylov.hip)
+// r = b - A x (x may be null = 0); scalars->rr = r.r, scalars->bb = b.b.
+void launch_residual64(hipStream_t s, Dims d, Op64 A, const double *b, const double *x,
+                       double *r, double *partials, unsigned *counter, KrylovScalars *sc);
+// w = A z; dots z.w, z.q_prev, z.r; the last block forms beta, p.q and alpha.
+void launch_fcg_matvec(hipStream_t s, Dims d, Op64 A, const float *z, const double *r,
+                       const double *qprev, double *w, int first, double *partials,
+                       unsigned *counter, KrylovScalars *sc);
+// p = z - beta p; q = w - beta q; x += alpha p; r -= alpha q; rr = r.r.
+void launch_fcg_update(hipStream_t s, Dims d, const float *z, const double *w, double *p,
+                       double *q, double *x, double *r, int first, double *partials,
+                       unsigned *counter, KrylovScalars *sc);
+// z = D^{-1} r on active cells (Jacobi preconditioner).
+void launch_jacobi(hipStream_t s, Dims d, Op32 A, const double *r, float *z);
+
+// ---------------------------------------------------------------- assembly (assemble.hip)
+struct Faces {
+    const double *a[3];
+};
+struct MutFaces {
+    double *a[3];
+};
+// Geometry of MagneticField's own StaggeredGrid (core/MagneticField.cpp:11-16).
+struct MagGeom {
+    double face_origin[3][3];
+    double ls_origin[3];  // cell grid of the level set
+    double spacing, inv;
+};
+MagGeom mag_geometry(Dims d, double spacing, const double origin[3]);
+
+// Pressure::SetUnKnowns + BuildProjectionMatrix; liquid[c] = phi <= 0;
+// *err set to 1 when a liquid cell has an open face on the grid boundary.
+void launch_pressure_assemble(hipStream_t s, Dims d, const double *phi, Faces frac, Faces vel,
+                              Faces us, const Faces *jump, double vol_error, double *diag64,
+                              double *const cpl64[3], double *rhs, float *diag32,
+                              float *const cpl32[3], unsigned char *liquid, int *err);
+// Pressure::SetRightHandSide on the stored unknowns.
+void launch_pressure_rhs(hipStream_t s, Dims d, const unsigned char *liquid, Faces frac,
+                         Faces vel, Faces us, double vol_error, double *rhs);
+// Pressure::ApplyProjection (p on the full grid, 0 at air cells).
+void launch_pressure_apply(hipStream_t s, Dims d, const double *phi, Faces frac,
+                           const Faces *jump, const unsigned char *liquid, const double *p,
+                           MutFaces vel, int *err);
+// MagneticField::BuildMatrix coefficients, matrix and AssembleRHS.
+void launch_magnetic_mu(hipStream_t s, Dims d, const MagGeom &g, const double *phi, double chi,
+                        double band, MutFaces mu);
+void launch_magnetic_assemble(hipStream_t s, Dims d, Faces mu, Faces hext, double spacing,
+                              double *diag64, double *const cpl64[3], double *rhs,
+                              float *diag32, float *const cpl32[3]);
+// MagneticField::AssignTotalField.
+void launch_magnetic_total_field(hipStream_t s, Dims d, double inv, const double *pot,
+                                 Faces hext, MutFaces htot);
+// MagneticField::CalculateVertexPressures.
+void launch_vertex_pressures(hipStream_t s, Dims d, const MagGeom &g, Faces htot, double chi,
+                             int nverts, const double *pos, const double *nrm, double *out);
+
+}  // namespace sfp

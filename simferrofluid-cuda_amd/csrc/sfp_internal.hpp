@@ -1,0 +1,134 @@
+// Internal types of the gfx950 solver library (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sfp {
+
+// ------------------------------------------------------------------ errors
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string &m) : std::runtime_error(m) {}
+};
+struct ArgError : std::runtime_error {
+    explicit ArgError(const std::string &m) : std::runtime_error(m) {}
+};
+struct InputError : std::runtime_error {
+    explicit InputError(const std::string &m) : std::runtime_error(m) {}
+};
+struct StateError : std::runtime_error {
+    explicit StateError(const std::string &m) : std::runtime_error(m) {}
+};
+
+void hip_fail(hipError_t e, const char *what, const char *file, int line);
+#define SFP_HIP(call)                                                    \
+    do {                                                                 \
+        hipError_t _e = (call);                                          \
+        if (_e != hipSuccess) ::sfp::hip_fail(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+#define SFP_LAUNCH_CHECK() SFP_HIP(hipGetLastError())
+
+// ------------------------------------------------------------------ grid
+// Cell-grid extents; linear index = z + nz*(y + ny*x) (core/Grid.h:28).
+struct Dims {
+    int nx = 0, ny = 0, nz = 0;
+    __host__ __device__ size_t cells() const { return (size_t)nx * ny * nz; }
+    __host__ __device__ size_t idx(int i, int j, int k) const {
+        return (size_t)k + (size_t)nz * ((size_t)j + (size_t)ny * (size_t)i);
+    }
+    // extents of face grid a (core/StaggeredGrid.h:15-19)
+    __host__ __device__ Dims face(int a) const {
+        Dims f = *this;
+        if (a == 0) f.nx += 1;
+        if (a == 1) f.ny += 1;
+        if (a == 2) f.nz += 1;
+        return f;
+    }
+    __host__ __device__ Dims coarse() const { return Dims{(nx + 1) / 2, (ny + 1) / 2, (nz + 1) / 2}; }
+    __host__ __device__ int n(int a) const { return a == 0 ? nx : (a == 1 ? ny : nz); }
+};
+
+// Launch shape for a cell-parallel kernel: threads along z (unit stride),
+// then y; x planes are looped over with stride gridDim.z.
+struct Launch {
+    dim3 grid, block;
+    unsigned blocks() const { return grid.x * grid.y * grid.z; }
+};
+Launch cell_launch(Dims d, unsigned target_blocks = 8192);
+
+// ------------------------------------------------------------------ device memory
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void ensure(size_t count) {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) return;
+        SFP_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// ------------------------------------------------------------------ operators
+// A 7-point symmetric operator on a cell grid:
+//   (A x)_c = diag_c x_c - sum_a ( cpl_a[c] x_{c-e_a} + cpl_a[c+e_a] x_{c+e_a} )
+// cpl_a[c] >= 0 couples c - e_a and c (0 on the lower boundary, at inactive
+// cells and across solid faces). Inactive cells are identity rows (diag 1)
+// with zero right-hand side and zero solution.
+struct Op64 {
+    const double *diag;
+    const double *cpl[3];
+};
+struct Op32 {
+    const float *diag;  // diagonal for active cells, 0 marks an inactive cell
+    const float *cpl[3];
+};
+
+// One multigrid level: FP64 operator (setup, and the fine level's outer
+// Krylov solve) and its FP32 copy (V-cycle).
+struct Level {
+    Dims d;
+    DevBuf<double> diag64, cpl64[3];
+    DevBuf<float> diag32, cpl32[3];
+    DevBuf<float> x0, x1, dv, b;  // ping-pong iterates, Chebyshev direction, rhs
+    Op64 op64() const { return Op64{diag64.p, {cpl64[0].p, cpl64[1].p, cpl64[2].p}}; }
+    Op32 op32() const { return Op32{diag32.p, {cpl32[0].p, cpl32[1].p, cpl32[2].p}}; }
+};
+
+// Scalars of the flexible-CG recurrences, resident on the device.
+struct KrylovScalars {
+    double zw, zq, zr, pq_prev;  // dots of the current iteration
+    double alpha, beta, pq;
+    double rr, bb;               // ||r||^2, ||b||^2
+    double pad[7];
+};
+
+// Per-kernel timing collected with HIP events on the context stream.
+enum KernelId {
+    K_SMOOTH = 0,   // fine-level Chebyshev step (FP32)
+    K_MATVEC64,     // FCG matvec + dots (FP64)
+    K_UPDATE64,     // FCG vector update (FP64)
+    K_RESTRICT,     // fine residual + restriction
+    K_COUNT
+};
+
+}  // namespace sfp

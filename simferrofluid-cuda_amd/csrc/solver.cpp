@@ -1,0 +1,263 @@
+// Hierarchy setup, V-cycle orchestration and the FP64 flexible CG.
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace sfp {
+
+void hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    throw HipError(buf);
+}
+
+// ---------------------------------------------------------------- Chebyshev
+Cheby Cheby::make(int degree, double ratio) {
+    // D^{-1}A of these weakly diagonally dominant M-matrices has spectrum in
+    // (0, 2] (Gershgorin), so lambda_max = 2 is a safe bound needing no estimate.
+    Cheby c;
+    c.degree = std::max(1, std::min(degree, 8));
+    const double lmax = 2.0, lmin = lmax / ratio;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    c.cd[0] = 0.f;
+    c.cr[0] = (float)(1.0 / theta);
+    for (int s = 1; s < c.degree; s++) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        c.cd[s] = (float)(rn * rho);
+        c.cr[s] = (float)(2.0 * rn / delta);
+        rho = rn;
+    }
+    return c;
+}
+
+// ---------------------------------------------------------------- timers
+void KernelTimer::begin(hipStream_t s, double b) {
+    if (used + 2 > ev.size()) {
+        for (int t = 0; t < 64; t++) {
+            hipEvent_t e;
+            SFP_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+    }
+    if (bytes.size() < ev.size() / 2) bytes.resize(ev.size() / 2);
+    bytes[used / 2] = b;
+    SFP_HIP(hipEventRecord(ev[used], s));
+}
+void KernelTimer::end(hipStream_t s) {
+    SFP_HIP(hipEventRecord(ev[used + 1], s));
+    used += 2;
+}
+void KernelTimer::harvest() {
+    for (size_t t = 0; t + 1 < used; t += 2) {
+        float ms = 0.f;
+        SFP_HIP(hipEventElapsedTime(&ms, ev[t], ev[t + 1]));
+        total_ms += ms;
+        total_bytes += bytes[t / 2];
+        launches++;
+    }
+    used = 0;
+}
+void KernelTimer::reset() {
+    used = 0;
+    launches = 0;
+    total_ms = total_bytes = 0.0;
+}
+KernelTimer::~KernelTimer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+}
+
+void Workspace::read_scalars() {
+    SFP_HIP(hipMemcpyAsync(host_sc, scalars.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, stream));
+    SFP_HIP(hipStreamSynchronize(stream));
+}
+
+// ---------------------------------------------------------------- System
+void System::allocate(Dims d) {
+    if (d_.nx == d.nx && d_.ny == d.ny && d_.nz == d.nz && !lv_.empty()) return;
+    d_ = d;
+    lv_.clear();
+    lv_.emplace_back();
+    Level &L = lv_[0];
+    L.d = d;
+    const size_t n = d.cells();
+    L.diag64.ensure(n);
+    L.diag32.ensure(n);
+    for (int a = 0; a < 3; a++) {
+        L.cpl64[a].ensure(n);
+        L.cpl32[a].ensure(n);
+    }
+    L.x0.ensure(n);
+    L.x1.ensure(n);
+    L.dv.ensure(n);
+    L.b.ensure(n);
+    rhs_.ensure(n);
+    x_.ensure(n);
+    r_.ensure(n);
+    p_.ensure(n);
+    q_.ensure(n);
+    w_.ensure(n);
+    SFP_HIP(hipMemset(x_.p, 0, n * sizeof(double)));
+    built_coarsest_ = -1;
+}
+
+void System::build_hierarchy(Workspace &ws, const SolverConfig &cfg) {
+    const int coarsest = std::max(2, std::min(cfg.coarsest, 4));
+    if (built_coarsest_ != coarsest) {
+        lv_.resize(1);
+        while (true) {
+            Dims dl = lv_.back().d;
+            if (std::max({dl.nx, dl.ny, dl.nz}) <= coarsest) break;
+            Level L;
+            L.d = dl.coarse();
+            const size_t n = L.d.cells();
+            L.diag64.ensure(n);
+            L.diag32.ensure(n);
+            for (int a = 0; a < 3; a++) {
+                L.cpl64[a].ensure(n);
+                L.cpl32[a].ensure(n);
+            }
+            L.x0.ensure(n);
+            L.x1.ensure(n);
+            L.dv.ensure(n);
+            L.b.ensure(n);
+            lv_.push_back(std::move(L));
+        }
+        if ((int)lv_.back().d.cells() > kMaxCoarse) throw ArgError("coarsest level too large");
+        ainv_.ensure((size_t)lv_.back().d.cells() * lv_.back().d.cells());
+        built_coarsest_ = coarsest;
+    }
+    for (size_t l = 1; l < lv_.size(); l++) {
+        Level &F = lv_[l - 1];
+        Level &C = lv_[l];
+        double *c64[3] = {C.cpl64[0].p, C.cpl64[1].p, C.cpl64[2].p};
+        float *c32[3] = {C.cpl32[0].p, C.cpl32[1].p, C.cpl32[2].p};
+        launch_coarsen(ws.stream, F.d, F.op64(), F.diag32.p, C.d, cfg.coarse_scale, C.diag64.p, c64,
+                       C.diag32.p, c32);
+    }
+    launch_coarse_invert(ws.stream, lv_.back().d, lv_.back().op64(), ainv_.p);
+}
+
+float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) {
+    Level &L = lv_[l];
+    const hipStream_t st = ws.stream;
+    const bool fine = (l == 0);
+    const double cells = (double)L.d.cells();
+    if (l + 1 == (int)lv_.size()) {
+        if (r64) launch_smooth_first(st, L.d, L.op32(), r64, L.b.p, L.x1.p, 0.f);
+        launch_coarse_solve(st, (int)L.d.cells(), ainv_.p, L.b.p, L.x0.p);
+        return L.x0.p;
+    }
+    const Op32 A = L.op32();
+    float *cur = L.x0.p, *oth = L.x1.p;
+    const int deg = cb.degree;
+    launch_smooth_first(st, L.d, A, r64, L.b.p, cur, cb.cr[0]);
+    auto smooth = [&](int mode, float cd, float cr, const float *e) {
+        const bool track = fine && ws.profiling;
+        if (track) {
+            double per = 24.0;  // x, b, diag, 3 couplings
+            per += 4.0;         // x'
+            if (mode & SM_READ_D) per += 4.0;
+            if (mode & SM_WRITE_D) per += 4.0;
+            if (mode & SM_PROLONG) per += 0.5;
+            ws.timers[K_SMOOTH].begin(st, per * cells);
+        }
+        launch_smooth(st, mode, L.d, A, L.b.p, cur, oth, L.dv.p, cd, cr, e);
+        if (track) ws.timers[K_SMOOTH].end(st);
+        std::swap(cur, oth);
+    };
+    for (int s = 1; s < deg; s++)
+        smooth((s == 1 ? SM_D_FROM_X : SM_READ_D) | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s],
+               nullptr);
+    if (fine && ws.profiling) ws.timers[K_RESTRICT].begin(st, (24.0 + 0.5) * cells);
+    launch_restrict(st, L.d, A, L.b.p, cur, lv_[l + 1].b.p);
+    if (fine && ws.profiling) ws.timers[K_RESTRICT].end(st);
+    const float *e = vlevel(ws, cb, l + 1, nullptr);
+    smooth(SM_PROLONG | (deg > 1 ? SM_WRITE_D : 0), 0.f, cb.cr[0], e);
+    for (int s = 1; s < deg; s++)
+        smooth(SM_READ_D | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s], nullptr);
+    return cur;
+}
+
+float *System::precondition(Workspace &ws, const SolverConfig &cfg, const double *r64) {
+    if (cfg.precond == 2) {  // SFP_PRECOND_JACOBI
+        launch_jacobi(ws.stream, d_, lv_[0].op32(), r64, lv_[0].x0.p);
+        return lv_[0].x0.p;
+    }
+    const Cheby cb = Cheby::make(cfg.cheby_degree, cfg.cheby_ratio);
+    return vlevel(ws, cb, 0, r64);
+}
+
+long long System::count_active(Workspace &ws) {
+    launch_count_active(ws.stream, d_, lv_[0].diag32.p, ws.partials.p, ws.counter.p, ws.scalars.p);
+    ws.read_scalars();
+    return (long long)ws.host_sc->pad[0];
+}
+
+SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
+    SolveInfo info;
+    info.precond_used = cfg.precond;
+    const hipStream_t st = ws.stream;
+    const size_t n = d_.cells();
+    const Op64 A = lv_[0].op64();
+    if (!warm) SFP_HIP(hipMemsetAsync(x_.p, 0, n * sizeof(double), st));
+    launch_residual64(st, d_, A, rhs_.p, warm ? x_.p : nullptr, r_.p, ws.partials.p, ws.counter.p,
+                      ws.scalars.p);
+    ws.read_scalars();
+    const double bb = ws.host_sc->bb;
+    if (bb == 0.0) {  // zero right-hand side: x = 0
+        SFP_HIP(hipMemsetAsync(x_.p, 0, n * sizeof(double), st));
+        info.converged = true;
+        info.rel = 0.0;
+        return info;
+    }
+    double rel = std::sqrt(ws.host_sc->rr / bb);
+    bool first = true;
+    bool checked = true;  // rel is a true residual
+    const double cells = (double)n;
+    while (true) {
+        if (rel <= cfg.tol) {
+            if (checked) break;
+            // recursive residual reached tol: recompute it from x in FP64 and
+            // restart from the true residual if the two have drifted apart
+            launch_residual64(st, d_, A, rhs_.p, x_.p, r_.p, ws.partials.p, ws.counter.p,
+                              ws.scalars.p);
+            ws.read_scalars();
+            rel = std::sqrt(ws.host_sc->rr / bb);
+            checked = true;
+            first = true;
+            continue;
+        }
+        if (info.iterations >= cfg.max_iters) break;
+        const float *z = precondition(ws, cfg, r_.p);
+        if (ws.profiling) ws.timers[K_MATVEC64].begin(st, (4.0 + 32.0 + 8.0 + 8.0 + 8.0) * cells);
+        launch_fcg_matvec(st, d_, A, z, r_.p, q_.p, w_.p, first ? 1 : 0, ws.partials.p,
+                          ws.counter.p, ws.scalars.p);
+        if (ws.profiling) ws.timers[K_MATVEC64].end(st);
+        if (ws.profiling) ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 5 + 8.0 * 4) * cells);
+        launch_fcg_update(st, d_, z, w_.p, p_.p, q_.p, x_.p, r_.p, first ? 1 : 0, ws.partials.p,
+                          ws.counter.p, ws.scalars.p);
+        if (ws.profiling) ws.timers[K_UPDATE64].end(st);
+        first = false;
+        info.iterations++;
+        ws.read_scalars();
+        rel = std::sqrt(ws.host_sc->rr / bb);
+        checked = false;
+    }
+    if (!checked) {
+        launch_residual64(st, d_, A, rhs_.p, x_.p, r_.p, ws.partials.p, ws.counter.p, ws.scalars.p);
+        ws.read_scalars();
+        rel = std::sqrt(ws.host_sc->rr / bb);
+    }
+    info.rel = rel;
+    info.converged = rel <= cfg.tol;
+    if (ws.profiling)
+        for (auto &t : ws.timers) t.harvest();
+    return info;
+}
+
+}  // namespace sfp

@@ -1,0 +1,330 @@
+"""Python binding of libsfp.so (the C ABI in include/sfp.h), used by the tests,
+the smoke check and bench.py.
+
+There is no fallback: if libsfp.so is missing or no GPU is usable, the calls
+raise. Arrays follow the reference layout (z-fastest, numpy C order on shape
+(nx, ny, nz); face grid a has extent n + e_a).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libsfp.so"))
+_LIB = None
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+OK, ERR_ARGUMENT, ERR_INPUT, ERR_HIP, ERR_NO_DEVICE, ERR_STATE, ERR_UNSUPPORTED = range(7)
+PRECOND_MG, PRECOND_LEARNED, PRECOND_JACOBI = 0, 1, 2
+KERNEL_SMOOTH, KERNEL_MATVEC64, KERNEL_UPDATE64, KERNEL_RESTRICT = 0, 1, 2, 3
+KERNEL_NAMES = {0: "smooth", 1: "matvec64", 2: "update64", 3: "restrict"}
+
+# exported symbols, per header (checked by the CPU tests)
+EXPORTS = {
+    "sfp.h": ["sfp_create", "sfp_destroy", "sfp_default_params", "sfp_set_params",
+              "sfp_get_params", "sfp_last_error", "sfp_status_string", "sfp_stream",
+              "sfp_reset_warm_start", "sfp_pressure_project", "sfp_pressure_reproject",
+              "sfp_magnetic_solve", "sfp_magnetic_vertex_pressures",
+              "sfp_pressure_project_device", "sfp_pressure_reproject_device",
+              "sfp_magnetic_solve_device"],
+    "sfp_profile.h": ["sfp_profile_enable", "sfp_profile_query"],
+    "sfp_test.h": ["sfp_test_num_levels", "sfp_test_level_dims", "sfp_test_export_level",
+                   "sfp_test_export_level32", "sfp_test_apply64", "sfp_test_precondition",
+                   "sfp_test_residual"],
+}
+
+
+class Params(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("precond", C.c_int),
+                ("mg_coarsest", C.c_int), ("cheby_degree", C.c_int), ("cheby_ratio", C.c_double),
+                ("coarse_scale", C.c_double), ("stall_ratio", C.c_double), ("verbose", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("fallback", C.c_int),
+                ("precond_used", C.c_int), ("rel_residual", C.c_double), ("setup_ms", C.c_double),
+                ("solve_ms", C.c_double), ("apply_ms", C.c_double), ("unknowns", C.c_longlong)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class KernelProfile(C.Structure):
+    _fields_ = [("launches", C.c_longlong), ("total_ms", C.c_double), ("bytes", C.c_double)]
+
+
+class SfpError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"sfp status {status}: {message}")
+        self.status = status
+
+
+P3 = _dp * 3
+
+
+def lib():
+    """Load libsfp.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libsfp.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.sfp_create.restype = C.c_int
+        L.sfp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int]
+        L.sfp_destroy.restype = None
+        L.sfp_destroy.argtypes = [vp]
+        L.sfp_default_params.restype = None
+        L.sfp_default_params.argtypes = [C.POINTER(Params)]
+        L.sfp_set_params.restype = C.c_int
+        L.sfp_set_params.argtypes = [vp, C.POINTER(Params)]
+        L.sfp_get_params.restype = C.c_int
+        L.sfp_get_params.argtypes = [vp, C.POINTER(Params)]
+        L.sfp_last_error.restype = C.c_char_p
+        L.sfp_last_error.argtypes = [vp]
+        L.sfp_status_string.restype = C.c_char_p
+        L.sfp_status_string.argtypes = [C.c_int]
+        L.sfp_stream.restype = vp
+        L.sfp_stream.argtypes = [vp]
+        L.sfp_reset_warm_start.restype = C.c_int
+        L.sfp_reset_warm_start.argtypes = [vp]
+        for name in ("sfp_pressure_project", "sfp_pressure_reproject",
+                     "sfp_pressure_project_device", "sfp_pressure_reproject_device"):
+            f = getattr(L, name)
+            f.restype = C.c_int
+            f.argtypes = [vp, P3, _dp, P3, P3, vp, C.c_double, _dp, C.POINTER(Stats)]
+        for name in ("sfp_magnetic_solve", "sfp_magnetic_solve_device"):
+            f = getattr(L, name)
+            f.restype = C.c_int
+            f.argtypes = [vp, _dp, P3, C.c_double, C.c_double, _dp, vp, C.POINTER(Stats)]
+        L.sfp_magnetic_vertex_pressures.restype = C.c_int
+        L.sfp_magnetic_vertex_pressures.argtypes = [vp, C.c_int, _dp, _dp, C.c_double, _dp]
+        L.sfp_profile_enable.restype = C.c_int
+        L.sfp_profile_enable.argtypes = [vp, C.c_int]
+        L.sfp_profile_query.restype = C.c_int
+        L.sfp_profile_query.argtypes = [vp, C.c_int, C.POINTER(KernelProfile)]
+        L.sfp_test_num_levels.restype = C.c_int
+        L.sfp_test_num_levels.argtypes = [vp, C.c_int, _ip]
+        L.sfp_test_level_dims.restype = C.c_int
+        L.sfp_test_level_dims.argtypes = [vp, C.c_int, C.c_int, _ip]
+        L.sfp_test_export_level.restype = C.c_int
+        L.sfp_test_export_level.argtypes = [vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.sfp_test_export_level32.restype = C.c_int
+        L.sfp_test_export_level32.argtypes = [vp, C.c_int, C.c_int, _fp, _fp, _fp, _fp]
+        L.sfp_test_apply64.restype = C.c_int
+        L.sfp_test_apply64.argtypes = [vp, C.c_int, _fp, _dp]
+        L.sfp_test_precondition.restype = C.c_int
+        L.sfp_test_precondition.argtypes = [vp, C.c_int, _dp, _fp]
+        L.sfp_test_residual.restype = C.c_int
+        L.sfp_test_residual.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.dtype == np.float64 and a.flags.c_contiguous, "need C-contiguous float64"
+        return a.ctypes.data_as(_dp)
+    return C.cast(C.c_void_p(int(a)), _dp)  # device pointer (int)
+
+
+def _f(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_fp)
+
+
+def _p3(arrs):
+    if arrs is None:
+        return None
+    return P3(*[_d(a) for a in arrs])
+
+
+def default_params() -> Params:
+    p = Params()
+    lib().sfp_default_params(C.byref(p))
+    return p
+
+
+class Context:
+    """A solver context bound to one grid and one GPU (sfp_create)."""
+
+    def __init__(self, n, spacing, origin=(0.0, 0.0, 0.0), device=0, **params):
+        if np.isscalar(n):
+            n = (int(n),) * 3
+        self.n = tuple(int(v) for v in n)
+        self.spacing = float(spacing)
+        self.origin = tuple(float(v) for v in origin)
+        self._h = C.c_void_p()
+        org = (C.c_double * 3)(*self.origin)
+        st = lib().sfp_create(C.byref(self._h), *self.n, self.spacing,
+                              C.cast(org, _dp), int(device))
+        if st != OK:
+            raise SfpError(st, lib().sfp_last_error(None).decode())
+        if params:
+            self.set_params(**params)
+
+    @classmethod
+    def for_scene(cls, sc, device=0, **params):
+        g = sc["geo"]
+        return cls(g.n, g.spacing, g.origin, device, **params)
+
+    def close(self):
+        if self._h:
+            lib().sfp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st):
+        if st != OK:
+            raise SfpError(st, lib().sfp_last_error(self._h).decode())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def params(self) -> Params:
+        p = Params()
+        self._check(lib().sfp_get_params(self._h, C.byref(p)))
+        return p
+
+    def set_params(self, **kw):
+        p = self.params()
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise KeyError(k)
+            setattr(p, k, v)
+        self._check(lib().sfp_set_params(self._h, C.byref(p)))
+
+    def reset_warm_start(self):
+        self._check(lib().sfp_reset_warm_start(self._h))
+
+    # ------------------------------------------------------------ host API
+    def pressure_project(self, sc, reproject=False, want_pressure=True):
+        """Pressure::Project on copies of sc's velocities. Returns
+        (velocity[3], pressure N^3 or None, stats dict)."""
+        vel = [np.array(v, dtype=np.float64, copy=True, order="C") for v in sc["velocity"]]
+        phi = np.ascontiguousarray(sc["phi"], dtype=np.float64)
+        frac = [np.ascontiguousarray(a, dtype=np.float64) for a in sc["fraction"]]
+        us = [np.ascontiguousarray(a, dtype=np.float64) for a in sc["solid_velocity"]]
+        jump = sc.get("jump")
+        jump = None if jump is None else [np.ascontiguousarray(a, dtype=np.float64) for a in jump]
+        p = np.zeros(phi.shape) if want_pressure else None
+        st = Stats()
+        f = lib().sfp_pressure_reproject if reproject else lib().sfp_pressure_project
+        self._check(f(self._h, _p3(vel), _d(phi), _p3(frac), _p3(us),
+                      C.cast(_p3(jump), C.c_void_p) if jump is not None else None,
+                      float(sc.get("vol_error", 0.0)), _d(p), C.byref(st)))
+        return vel, p, st.as_dict()
+
+    def magnetic_solve(self, sc, chi=None, band=None, want_field=True):
+        """MagneticField::Solve. Returns (potential N^3, total_field[3] or None, stats)."""
+        g = sc["geo"]
+        phi = np.ascontiguousarray(sc["phi"], dtype=np.float64)
+        hext = [np.ascontiguousarray(a, dtype=np.float64) for a in sc["hext"]]
+        pot = np.zeros(phi.shape)
+        H = [np.zeros(g.face_shape(a)) for a in range(3)] if want_field else None
+        st = Stats()
+        self._check(lib().sfp_magnetic_solve(
+            self._h, _d(phi), _p3(hext), float(sc["chi"] if chi is None else chi),
+            float(sc["band"] if band is None else band), _d(pot),
+            C.cast(_p3(H), C.c_void_p) if H is not None else None, C.byref(st)))
+        return pot, H, st.as_dict()
+
+    def vertex_pressures(self, positions, normals, chi):
+        positions = np.ascontiguousarray(positions, dtype=np.float64)
+        normals = np.ascontiguousarray(normals, dtype=np.float64)
+        nv = positions.shape[0]
+        out = np.zeros(max(nv, 1))
+        self._check(lib().sfp_magnetic_vertex_pressures(self._h, nv, _d(positions), _d(normals),
+                                                        float(chi), _d(out)))
+        return out[:nv]
+
+    # ------------------------------------------------------------ device API (int pointers)
+    def pressure_project_device(self, vel, phi, frac, us, jump, vol_error, pressure=None,
+                                reproject=False):
+        st = Stats()
+        f = lib().sfp_pressure_reproject_device if reproject else lib().sfp_pressure_project_device
+        self._check(f(self._h, _p3(vel), _d(phi), _p3(frac), _p3(us),
+                      C.cast(_p3(jump), C.c_void_p) if jump is not None else None,
+                      float(vol_error), _d(pressure), C.byref(st)))
+        return st.as_dict()
+
+    def magnetic_solve_device(self, phi, hext, chi, band, potential=None, total_field=None):
+        st = Stats()
+        self._check(lib().sfp_magnetic_solve_device(
+            self._h, _d(phi), _p3(hext), float(chi), float(band), _d(potential),
+            C.cast(_p3(total_field), C.c_void_p) if total_field is not None else None,
+            C.byref(st)))
+        return st.as_dict()
+
+    # ------------------------------------------------------------ profiling
+    def profile(self, enable=True):
+        self._check(lib().sfp_profile_enable(self._h, 1 if enable else 0))
+
+    def kernel_profile(self, kernel):
+        k = KernelProfile()
+        self._check(lib().sfp_profile_query(self._h, int(kernel), C.byref(k)))
+        return {"launches": k.launches, "total_ms": k.total_ms, "bytes": k.bytes}
+
+    # ------------------------------------------------------------ test hooks
+    def num_levels(self, which):
+        v = C.c_int(0)
+        self._check(lib().sfp_test_num_levels(self._h, which, C.byref(v)))
+        return v.value
+
+    def level_dims(self, which, level):
+        d = (C.c_int * 3)()
+        self._check(lib().sfp_test_level_dims(self._h, which, level, C.cast(d, _ip)))
+        return tuple(d)
+
+    def export_level(self, which, level):
+        dims = self.level_dims(which, level)
+        arrs = [np.zeros(dims) for _ in range(5)]
+        self._check(lib().sfp_test_export_level(self._h, which, level, *[_d(a) for a in arrs]))
+        diag, cx, cy, cz, rhs = arrs
+        return dict(diag=diag, cpl=[cx, cy, cz], rhs=rhs if level == 0 else None)
+
+    def export_level32(self, which, level):
+        dims = self.level_dims(which, level)
+        arrs = [np.zeros(dims, np.float32) for _ in range(4)]
+        self._check(lib().sfp_test_export_level32(self._h, which, level, *[_f(a) for a in arrs]))
+        return dict(diag=arrs[0], cpl=arrs[1:])
+
+    def apply64(self, which, z):
+        z = np.ascontiguousarray(z, dtype=np.float32)
+        w = np.zeros(z.shape)
+        self._check(lib().sfp_test_apply64(self._h, which, _f(z), _d(w)))
+        return w
+
+    def precondition(self, which, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(r.shape, np.float32)
+        self._check(lib().sfp_test_precondition(self._h, which, _d(r), _f(z)))
+        return z
+
+    def residual(self, which, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        r = np.zeros(x.shape)
+        rr = C.c_double(0)
+        bb = C.c_double(0)
+        self._check(lib().sfp_test_residual(self._h, which, _d(x), _d(r), C.byref(rr), C.byref(bb)))
+        return r, rr.value, bb.value
