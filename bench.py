@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Benchmark: seconds per grid Poisson solve to FP64 relative residual 1e-8
+at 256^3 on one MI355X.
+
+A step is one full device-resident solve call of the library (assembly,
+multigrid setup, flexible-CG iterations and the velocity / field update; the
+inputs are already in HBM). Steps cycle through the 256^3 workload's four
+systems: the ghost-fluid pressure projection (sphere drop over a pool,
+smoothed random velocities) and the FDM magnetic potential with chi = 0.5, 2
+and 8 (cosine-perturbed interface, H_ext = (0,1,0), band 1.5 dx). Magnetic
+solves start cold (no warm start) and every pressure step gets pristine
+velocities, so each step is a full solve.
+
+Prints one JSON line (see the driver contract); `value` is the mean seconds
+per solve. The `roofline` object times the fine-level Chebyshev smoother
+kernel (the dominant kernel) with HIP events on the library's stream during
+the timed steps. The `cpu_baseline` object times the FP64 CPU oracle
+(smoothed-aggregation AMG + SPAI-0, BiCGStab for pressure, CG for magnetic,
+hierarchy rebuilt per solve) on one pressure and one magnetic solve at the
+same size on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "simferrofluid-cuda_amd"))
+
+METRIC = "s per Poisson solve to rel. residual 1e-8 at 256³ (learned FCG vs plain MG-PCG)"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--n", type=int, default=256, help="cells per axis")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--precond", choices=["mg", "jacobi"], default="mg")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--systems", default="p,m0.5,m2,m8")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU: the solver has no "
+                                   "multi-GPU path (see DESIGN.md for how it would shard)",
+                          "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import sfp
+    from sfp import scenes
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    dev = torch.device("cuda:0")
+    n = args.n
+    t0 = time.time()
+    psc = scenes.pressure_scene(n, seed=0)
+    systems = []
+    for tag in args.systems.split(","):
+        if tag == "p":
+            systems.append(("pressure", None))
+        else:
+            systems.append((f"magnetic_chi{tag[1:]}", float(tag[1:])))
+    msc = scenes.magnetic_scene(n, seed=0, chi=2.0)
+    log(f"scenes generated in {time.time() - t0:.1f}s")
+
+    def to_dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    phi_p = to_dev(psc["phi"])
+    frac = [to_dev(a) for a in psc["fraction"]]
+    us = [to_dev(a) for a in psc["solid_velocity"]]
+    vel0 = [to_dev(a) for a in psc["velocity"]]
+    phi_m = to_dev(msc["phi"])
+    hext = [to_dev(a) for a in msc["hext"]]
+    pressure_out = torch.zeros(psc["phi"].shape, dtype=torch.float64, device=dev)
+    potential_out = torch.zeros(msc["phi"].shape, dtype=torch.float64, device=dev)
+    total = args.warmup + args.steps
+    n_p = sum(1 for i in range(total) if systems[i % len(systems)][1] is None)
+    vel_ring = [[v.clone() for v in vel0] for _ in range(max(n_p, 1))]
+    torch.cuda.synchronize()
+
+    geo = psc["geo"]
+    ctx = sfp.Context(geo.n, geo.spacing, geo.origin, tol=args.tol, max_iters=200,
+                      precond=sfp.PRECOND_JACOBI if args.precond == "jacobi" else sfp.PRECOND_MG)
+    ptr = lambda t: t.data_ptr()
+    ring_i = [0]
+
+    def run_step(i):
+        name, chi = systems[i % len(systems)]
+        if chi is None:
+            v = vel_ring[ring_i[0] % len(vel_ring)]
+            ring_i[0] += 1
+            st = ctx.pressure_project_device([ptr(a) for a in v], ptr(phi_p),
+                                             [ptr(a) for a in frac], [ptr(a) for a in us], None,
+                                             0.0, ptr(pressure_out))
+        else:
+            ctx.reset_warm_start()
+            st = ctx.magnetic_solve_device(ptr(phi_m), [ptr(a) for a in hext], chi,
+                                           1.5 * geo.spacing, ptr(potential_out))
+        return name, st
+
+    for i in range(args.warmup):
+        name, st = run_step(i)
+        log(f"warmup {name}: iters={st['iterations']} rel={st['rel_residual']:.2e} "
+            f"setup={st['setup_ms']:.2f}ms solve={st['solve_ms']:.2f}ms")
+    if not args.no_profile:
+        ctx.profile(True)
+    torch.cuda.synchronize()
+    records = []
+    t_start = time.perf_counter()
+    for i in range(args.warmup, total):
+        ts = time.perf_counter()
+        name, st = run_step(i)
+        st["wall_ms"] = (time.perf_counter() - ts) * 1e3
+        st["system"] = name
+        records.append(st)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    for st in records:
+        log(f"{st['system']}: iters={st['iterations']} rel={st['rel_residual']:.2e} "
+            f"wall={st['wall_ms']:.2f}ms setup={st['setup_ms']:.2f} solve={st['solve_ms']:.2f} "
+            f"apply={st['apply_ms']:.2f}")
+    sec_per_solve = elapsed / args.steps
+    cells = geo.ncells
+
+    roof = None
+    kernels = {}
+    if not args.no_profile:
+        for kid, kname in sfp.KERNEL_NAMES.items():
+            kp = ctx.kernel_profile(kid)
+            if kp["launches"]:
+                kernels[kname] = {"launches": kp["launches"],
+                                  "avg_us": 1e3 * kp["total_ms"] / kp["launches"],
+                                  "GBps": kp["bytes"] / (kp["total_ms"] * 1e-3) / 1e9}
+        kp = ctx.kernel_profile(sfp.KERNEL_SMOOTH)
+        if kp["launches"]:
+            achieved = kp["bytes"] / (kp["total_ms"] * 1e-3) / 1e9
+            traffic = None
+            tf = os.path.join(ROOT, "profiles", f"traffic_smooth_{n}.json")
+            if os.path.exists(tf):
+                with open(tf) as f:
+                    traffic = json.load(f).get("hbm_bytes_per_launch")
+            roof = {"kernel": "k_smooth (fine-level FP32 Chebyshev step, 7-point stencil)",
+                    "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic,
+                    "avg_launch_us": round(1e3 * kp["total_ms"] / kp["launches"], 2),
+                    "algorithmic_bytes_per_launch": kp["bytes"] / kp["launches"]}
+        ctx.profile(False)
+
+    by_sys = {}
+    for st in records:
+        d = by_sys.setdefault(st["system"], {"wall_ms": [], "iterations": [], "rel_residual": []})
+        d["wall_ms"].append(round(st["wall_ms"], 3))
+        d["iterations"].append(st["iterations"])
+        d["rel_residual"].append(st["rel_residual"])
+
+    cpu = None
+    checks = {}
+    if not args.no_cpu_baseline:
+        cpu, checks = cpu_leg(psc, msc, pressure_out, potential_out, ctx, phi_p, frac, us, vel0,
+                              phi_m, hext, geo, args, log)
+
+    out = {
+        "metric": METRIC,
+        "value": round(sec_per_solve, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * sec_per_solve, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycle / f64 outer FCG",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 pressure projection + FDM magnetic potential "
+                               f"(chi 0.5/2/8) solves to FP64 rel. residual {args.tol:g}",
+                   "grid": [n, n, n], "precond": args.precond, "systems": args.systems,
+                   "learned_fcg": "not built yet", "parallelism": "single GPU"},
+        "mg_pcg_s": round(sec_per_solve, 6),
+        "learned_fcg_s": None,
+        "cells_per_s": round(cells / sec_per_solve, 1),
+        "per_system": by_sys,
+        "kernels": kernels,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "checks": checks,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_leg(psc, msc, pressure_out, potential_out, ctx, phi_p, frac, us, vel0, phi_m, hext, geo,
+            args, log):
+    """CPU oracle timing plus FP64 checks of the GPU solutions on the oracle's CSR."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    ptr = lambda t: t.data_ptr()
+    checks = {}
+    times = []
+    # pressure: GPU solution of the pristine velocities
+    v = [a.clone() for a in vel0]
+    ctx.pressure_project_device([ptr(a) for a in v], ptr(phi_p), [ptr(a) for a in frac],
+                                [ptr(a) for a in us], None, 0.0, ptr(pressure_out))
+    p_gpu = pressure_out.cpu().numpy()
+    t = time.time()
+    ps = oracle.pressure_system(psc)
+    t_asm = time.time() - t
+    x, it, rel, ts, tv = oracle.solve(ps["A"], ps["rhs"], "bicgstab", "amg", tol=args.tol, maxit=200)
+    times.append(t_asm + ts + tv)
+    xg = p_gpu.ravel()[ps["mat2grid"]]
+    checks["pressure"] = {"oracle_rel_residual_of_gpu_x": oracle.rel_residual(ps["A"], ps["rhs"], xg),
+                          "linf_gpu_minus_cpu_over_linf_cpu": float(np.abs(xg - x).max() / np.abs(x).max()),
+                          "cpu_iters": it, "cpu_rel": rel, "cpu_assembly_s": t_asm,
+                          "cpu_setup_s": ts, "cpu_solve_s": tv}
+    log(f"cpu pressure: asm {t_asm:.2f}s setup {ts:.2f}s solve {tv:.2f}s iters {it}")
+    del ps
+    # magnetic chi = 2, cold
+    ctx.reset_warm_start()
+    ctx.magnetic_solve_device(ptr(phi_m), [ptr(a) for a in hext], 2.0, 1.5 * geo.spacing,
+                              ptr(potential_out))
+    m_gpu = potential_out.cpu().numpy().ravel()
+    t = time.time()
+    ms = oracle.magnetic_system(dict(msc, chi=2.0, band=1.5 * geo.spacing))
+    t_asm = time.time() - t
+    x, it, rel, ts, tv = oracle.solve(ms["A"], ms["rhs"], "cg", "amg", tol=args.tol, maxit=200)
+    times.append(t_asm + ts + tv)
+    checks["magnetic_chi2"] = {"oracle_rel_residual_of_gpu_x": oracle.rel_residual(ms["A"], ms["rhs"], m_gpu),
+                               "linf_gpu_minus_cpu_over_linf_cpu": float(np.abs(m_gpu - x).max() / np.abs(x).max()),
+                               "cpu_iters": it, "cpu_rel": rel, "cpu_assembly_s": t_asm,
+                               "cpu_setup_s": ts, "cpu_solve_s": tv}
+    log(f"cpu magnetic: asm {t_asm:.2f}s setup {ts:.2f}s solve {tv:.2f}s iters {it}")
+    cpu = {"value": round(sum(times) / len(times), 4), "unit": "s",
+           "cores": oracle.num_threads(), "kind": "port",
+           "sample": f"one pressure (BiCGStab) and one magnetic chi=2 (CG) solve at "
+                     f"{args.n}^3 to rel. residual {args.tol:g}: serial CSR assembly + "
+                     f"SA-AMG/SPAI-0 setup + solve per call, OpenMP threads as cores"}
+    return cpu, checks
+
+
+if __name__ == "__main__":
+    main()

@@ -78,16 +78,16 @@ __device__ __forceinline__ double heaviside(double phi, double band) {
     return .5 * (1. + phi / band + sin(pi * phi / band) / pi);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pressure_assemble(
+__global__ __launch_bounds__(kBlock) void k_pressure_assemble(CellGrid cg, 
     Dims d, const double *__restrict__ phi, Faces frac, Faces vel, Faces us, Faces jump,
     int has_jump, double vol_error, double *__restrict__ diag64, MutFaces cpl64,
     double *__restrict__ rhs, float *__restrict__ diag32, float *cpl32x, float *cpl32y,
     float *cpl32z, unsigned char *__restrict__ liquid, int *err) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     if (k >= d.nz || j >= d.ny) return;
     float *cpl32[3] = {cpl32x, cpl32y, cpl32z};
-    for (int i = blockIdx.z; i < d.nx; i += gridDim.z) {
+    for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
         const size_t c = d.idx(i, j, k);
         const double pc = phi[c];
         const bool liq = pc <= 0;
@@ -141,15 +141,15 @@ __global__ __launch_bounds__(kBlock) void k_pressure_assemble(
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_pressure_rhs(Dims d,
+__global__ __launch_bounds__(kBlock) void k_pressure_rhs(CellGrid cg, Dims d,
                                                          const unsigned char *__restrict__ liquid,
                                                          Faces frac, Faces vel, Faces us,
                                                          double vol_error,
                                                          double *__restrict__ rhs) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     if (k >= d.nz || j >= d.ny) return;
-    for (int i = blockIdx.z; i < d.nx; i += gridDim.z) {
+    for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
         const size_t c = d.idx(i, j, k);
         if (!liquid[c]) {
             rhs[c] = 0.0;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_rhs(Dims d,
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_pressure_apply(Dims d, int axis,
+__global__ __launch_bounds__(kBlock) void k_pressure_apply(CellGrid cg, Dims d, int axis,
                                                            const double *__restrict__ phi,
                                                            const double *__restrict__ frac,
                                                            const double *__restrict__ jump,
@@ -179,10 +179,10 @@ __global__ __launch_bounds__(kBlock) void k_pressure_apply(Dims d, int axis,
                                                            const double *__restrict__ p,
                                                            double *__restrict__ vel, int *err) {
     const Dims fd = d.face(axis);
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     if (k >= fd.nz || j >= fd.ny) return;
-    for (int i = blockIdx.z; i < fd.nx; i += gridDim.z) {
+    for (int i = i0, ie = min(i0 + cg.planes, fd.nx); i < ie; i++) {
         const size_t fidx = fd.idx(i, j, k);
         const double weight = 1. - frac[fidx];
         if (weight == 0.) continue;
@@ -214,14 +214,14 @@ __global__ __launch_bounds__(kBlock) void k_pressure_apply(Dims d, int axis,
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_magnetic_mu(Dims d, int axis, MagGeom g,
+__global__ __launch_bounds__(kBlock) void k_magnetic_mu(CellGrid cg, Dims d, int axis, MagGeom g,
                                                         const double *__restrict__ phi, double chi,
                                                         double band, double *__restrict__ mu) {
     const Dims fd = d.face(axis);
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     if (k >= fd.nz || j >= fd.ny) return;
-    for (int i = blockIdx.z; i < fd.nx; i += gridDim.z) {
+    for (int i = i0, ie = min(i0 + cg.planes, fd.nx); i < ie; i++) {
         const int f[3] = {i, j, k};
         const size_t fidx = fd.idx(i, j, k);
         if (f[axis] <= 0 || f[axis] >= d.n(axis)) {  // IsBoundaryFace, width 0
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_mu(Dims d, int axis, MagGeo
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_magnetic_assemble(Dims d, Faces mu, Faces hext,
+__global__ __launch_bounds__(kBlock) void k_magnetic_assemble(CellGrid cg, Dims d, Faces mu, Faces hext,
                                                               double spacing,
                                                               double *__restrict__ diag64,
                                                               MutFaces cpl64,
@@ -243,11 +243,11 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_assemble(Dims d, Faces mu, 
                                                               float *__restrict__ diag32,
                                                               float *cpl32x, float *cpl32y,
                                                               float *cpl32z) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     if (k >= d.nz || j >= d.ny) return;
     float *cpl32[3] = {cpl32x, cpl32y, cpl32z};
-    for (int i = blockIdx.z; i < d.nx; i += gridDim.z) {
+    for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
         const size_t c = d.idx(i, j, k);
         double low[3] = {0.0, 0.0, 0.0};
         if (c == 0) {  // pinned cell: identity row, zero right-hand side
@@ -283,15 +283,15 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_assemble(Dims d, Faces mu, 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_magnetic_total_field(Dims d, int axis, double inv,
+__global__ __launch_bounds__(kBlock) void k_magnetic_total_field(CellGrid cg, Dims d, int axis, double inv,
                                                                  const double *__restrict__ pot,
                                                                  const double *__restrict__ hext,
                                                                  double *__restrict__ htot) {
     const Dims fd = d.face(axis);
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     if (k >= fd.nz || j >= fd.ny) return;
-    for (int i = blockIdx.z; i < fd.nx; i += gridDim.z) {
+    for (int i = i0, ie = min(i0 + cg.planes, fd.nx); i < ie; i++) {
         const size_t fidx = fd.idx(i, j, k);
         int c0[3] = {i, j, k};
         c0[axis] -= 1;
@@ -335,7 +335,7 @@ void launch_pressure_assemble(hipStream_t st, Dims d, const double *phi, Faces f
     Launch L = cell_launch(d);
     Faces j = jump ? *jump : Faces{{nullptr, nullptr, nullptr}};
     MutFaces c{{cpl64[0], cpl64[1], cpl64[2]}};
-    hipLaunchKernelGGL(k_pressure_assemble, L.grid, L.block, 0, st, d, phi, frac, vel, us, j,
+    hipLaunchKernelGGL(k_pressure_assemble, L.grid, L.block, 0, st, L.cg, d, phi, frac, vel, us, j,
                        jump ? 1 : 0, vol_error, diag64, c, rhs, diag32, cpl32[0], cpl32[1],
                        cpl32[2], liquid, err);
     SFP_LAUNCH_CHECK();
@@ -344,7 +344,7 @@ void launch_pressure_assemble(hipStream_t st, Dims d, const double *phi, Faces f
 void launch_pressure_rhs(hipStream_t st, Dims d, const unsigned char *liquid, Faces frac,
                          Faces vel, Faces us, double vol_error, double *rhs) {
     Launch L = cell_launch(d);
-    hipLaunchKernelGGL(k_pressure_rhs, L.grid, L.block, 0, st, d, liquid, frac, vel, us,
+    hipLaunchKernelGGL(k_pressure_rhs, L.grid, L.block, 0, st, L.cg, d, liquid, frac, vel, us,
                        vol_error, rhs);
     SFP_LAUNCH_CHECK();
 }
@@ -354,7 +354,7 @@ void launch_pressure_apply(hipStream_t st, Dims d, const double *phi, Faces frac
                            MutFaces vel, int *err) {
     for (int a = 0; a < 3; a++) {
         Launch L = cell_launch(d.face(a));
-        hipLaunchKernelGGL(k_pressure_apply, L.grid, L.block, 0, st, d, a, phi, frac.a[a],
+        hipLaunchKernelGGL(k_pressure_apply, L.grid, L.block, 0, st, L.cg, d, a, phi, frac.a[a],
                            jump ? jump->a[a] : nullptr, liquid, p, vel.a[a], err);
         SFP_LAUNCH_CHECK();
     }
@@ -364,7 +364,7 @@ void launch_magnetic_mu(hipStream_t st, Dims d, const MagGeom &g, const double *
                         double band, MutFaces mu) {
     for (int a = 0; a < 3; a++) {
         Launch L = cell_launch(d.face(a));
-        hipLaunchKernelGGL(k_magnetic_mu, L.grid, L.block, 0, st, d, a, g, phi, chi, band, mu.a[a]);
+        hipLaunchKernelGGL(k_magnetic_mu, L.grid, L.block, 0, st, L.cg, d, a, g, phi, chi, band, mu.a[a]);
         SFP_LAUNCH_CHECK();
     }
 }
@@ -374,7 +374,7 @@ void launch_magnetic_assemble(hipStream_t st, Dims d, Faces mu, Faces hext, doub
                               float *diag32, float *const cpl32[3]) {
     Launch L = cell_launch(d);
     MutFaces c{{cpl64[0], cpl64[1], cpl64[2]}};
-    hipLaunchKernelGGL(k_magnetic_assemble, L.grid, L.block, 0, st, d, mu, hext, spacing, diag64,
+    hipLaunchKernelGGL(k_magnetic_assemble, L.grid, L.block, 0, st, L.cg, d, mu, hext, spacing, diag64,
                        c, rhs, diag32, cpl32[0], cpl32[1], cpl32[2]);
     SFP_LAUNCH_CHECK();
 }
@@ -383,7 +383,7 @@ void launch_magnetic_total_field(hipStream_t st, Dims d, double inv, const doubl
                                  Faces hext, MutFaces htot) {
     for (int a = 0; a < 3; a++) {
         Launch L = cell_launch(d.face(a));
-        hipLaunchKernelGGL(k_magnetic_total_field, L.grid, L.block, 0, st, d, a, inv, pot,
+        hipLaunchKernelGGL(k_magnetic_total_field, L.grid, L.block, 0, st, L.cg, d, a, inv, pot,
                            hext.a[a], htot.a[a]);
         SFP_LAUNCH_CHECK();
     }

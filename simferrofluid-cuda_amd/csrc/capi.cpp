@@ -291,12 +291,9 @@ sfp_status sfp_create(sfp_context *out, int nx, int ny, int nz, double spacing,
         SFP_HIP(hipSetDevice(device));
         SFP_HIP(hipStreamCreateWithFlags(&ctx->ws.stream, hipStreamNonBlocking));
         for (auto &e : ctx->ev) SFP_HIP(hipEventCreate(&e));
-        const Launch L = cell_launch(ctx->d.face(0));
-        ctx->ws.partials.ensure(4 * (size_t)std::max(L.blocks(), cell_launch(ctx->d).blocks()));
-        ctx->ws.counter.ensure(1);
+        ctx->ws.partials.ensure(reduction_partials(ctx->d));
         ctx->ws.scalars.ensure(1);
         ctx->ws.err.ensure(1);
-        SFP_HIP(hipMemset(ctx->ws.counter.p, 0, sizeof(unsigned)));
         SFP_HIP(hipMemset(ctx->ws.scalars.p, 0, sizeof(KrylovScalars)));
         SFP_HIP(hipMemset(ctx->ws.err.p, 0, sizeof(int)));
         SFP_HIP(hipHostMalloc((void **)&ctx->ws.host_sc, sizeof(KrylovScalars), hipHostMallocDefault));

@@ -116,7 +116,7 @@ sfp_status sfp_test_apply64(sfp_context ctx, int which, const float *z, double *
         SFP_HIP(hipMemcpyAsync(dz.p, z, n * sizeof(float), hipMemcpyHostToDevice, ctx->ws.stream));
         SFP_HIP(hipMemsetAsync(dr.p, 0, n * sizeof(double), ctx->ws.stream));
         launch_fcg_matvec(ctx->ws.stream, d, s.fine().op64(), dz.p, dr.p, nullptr, dw.p, 1,
-                          ctx->ws.partials.p, ctx->ws.counter.p, ctx->ws.scalars.p);
+                          ctx->ws.partials.p, ctx->ws.scalars.p);
         d2h(ctx, w, dw.p, n);
         SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
     });
@@ -148,7 +148,7 @@ sfp_status sfp_test_residual(sfp_context ctx, int which, const double *x, double
         dr.ensure(n);
         SFP_HIP(hipMemcpyAsync(dx.p, x, n * sizeof(double), hipMemcpyHostToDevice, ctx->ws.stream));
         launch_residual64(ctx->ws.stream, d, s.fine().op64(), s.rhs().p, dx.p, dr.p,
-                          ctx->ws.partials.p, ctx->ws.counter.p, ctx->ws.scalars.p);
+                          ctx->ws.partials.p, ctx->ws.scalars.p);
         d2h(ctx, r, dr.p, n);
         ctx->ws.read_scalars();
         if (rr) *rr = ctx->ws.host_sc->rr;

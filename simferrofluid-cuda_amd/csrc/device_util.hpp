@@ -47,41 +47,45 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
     __syncthreads();
 }
 
-// Grid-wide sum of v[q]. Each block stores its partial sums; the block that
-// arrives last adds all partials in block order (deterministic for a fixed
-// launch shape) and returns true with the totals in thread 0's v[]. Every
-// thread of every block must call it. `partials` holds N * num_blocks()
-// doubles; `counter` must be 0 on entry and is 0 again on exit.
+// Block-level partial sums: block b stores its totals at partials[q*nb + b]
+// (nb = gridDim.x). A one-block finalize kernel adds them in block order, so
+// every reduction is deterministic for a given grid. (A last-block-done
+// scheme needs a device-scope fence per block, which on a multi-XCD chip
+// writes back L2 and serializes the blocks.)
 template <int N>
-__device__ __forceinline__ bool grid_sum(double (&v)[N], double *partials, unsigned *counter) {
-    __shared__ unsigned s_last;
-    const unsigned nb = num_blocks(), bid = flat_bid();
-    const int tid = flat_tid();
+__device__ __forceinline__ void block_partials(double (&v)[N], double *partials) {
     block_sum<N>(v);
-    if (tid == 0) {
+    if (flat_tid() == 0) {
 #pragma unroll
-        for (int q = 0; q < N; q++) partials[(size_t)q * nb + bid] = v[q];
-        __threadfence();
-        unsigned ticket = atomicAdd(counter, 1u);
-        s_last = (ticket == nb - 1) ? 1u : 0u;
+        for (int q = 0; q < N; q++) partials[(size_t)q * gridDim.x + blockIdx.x] = v[q];
     }
-    __syncthreads();
-    if (s_last == 0) return false;
-    __threadfence();
-    double acc[N];
+}
+
+// Inside a one-block kernel: totals of nb block partials, valid in thread 0.
+template <int N>
+__device__ __forceinline__ void sum_partials(const double *partials, unsigned nb, double (&v)[N]) {
 #pragma unroll
-    for (int q = 0; q < N; q++) acc[q] = 0.0;
-    for (unsigned b = tid; b < nb; b += kBlock) {
+    for (int q = 0; q < N; q++) v[q] = 0.0;
+    for (unsigned b = flat_tid(); b < nb; b += kBlock) {
 #pragma unroll
-        for (int q = 0; q < N; q++) acc[q] += partials[(size_t)q * nb + b];
+        for (int q = 0; q < N; q++) v[q] += partials[(size_t)q * nb + b];
     }
-    block_sum<N>(acc);
-    if (tid == 0) {
-#pragma unroll
-        for (int q = 0; q < N; q++) v[q] = acc[q];
-        atomicExch(counter, 0u);
-    }
-    return true;
+    block_sum<N>(v);
+}
+
+// Tile of a cell-parallel block: z index k (lanes), y index j (wavefronts)
+// and the first of its cg.planes consecutive x planes. Blocks are taken in
+// launch order (tiles of one plane, then the next chunk of planes); a
+// renumbering that keeps each XCD on one contiguous slab measured slower for
+// these stencils on MI355X, so none is applied.
+__device__ __forceinline__ void cell_tile(CellGrid cg, int &k, int &j, int &i0) {
+    const unsigned L = blockIdx.x;
+    const unsigned per = cg.gx * cg.gy;
+    const unsigned bz = L / per, rem = L - bz * per;
+    const unsigned by = rem / cg.gx, bx = rem - by * cg.gx;
+    k = (int)(bx * blockDim.x + threadIdx.x);
+    j = (int)(by * blockDim.y + threadIdx.y);
+    i0 = (int)bz * cg.planes;
 }
 
 // Strides of the z-fastest layout.

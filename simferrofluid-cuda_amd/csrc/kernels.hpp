@@ -7,7 +7,7 @@ namespace sfp {
 // ---------------------------------------------------------------- multigrid (mg.hip)
 // Chebyshev step modes.
 enum SmoothMode : int {
-    SM_D_FROM_X = 1,   // previous direction equals the input iterate (step after a zero start)
+    SM_FIRST = 1,      // input iterate is the zero-start first step c0 D^{-1} b, fused in
     SM_READ_D = 2,     // previous direction read from dvec
     SM_WRITE_D = 4,    // store the new direction
     SM_PROLONG = 8     // input iterate is x + P e (coarse correction fused in)
@@ -17,9 +17,11 @@ enum SmoothMode : int {
 // from the FP64 residual r64 (and stored to b).
 void launch_smooth_first(hipStream_t s, Dims d, Op32 A, const double *r64, float *b, float *x,
                          float c0);
-// xout = xin' + d', d' = cd * d + cr * D^{-1} (b - A xin'), xin' = xin (+ P e).
-void launch_smooth(hipStream_t s, int mode, Dims d, Op32 A, const float *b, const float *xin,
-                   float *xout, float *dvec, float cd, float cr, const float *ecoarse);
+// xout = x + d', d' = cd * d + cr * D^{-1} (b - A x) with x = xin, xin + P e,
+// or the fused zero-start step c0 D^{-1} b (SM_FIRST; b from r64 when given).
+void launch_smooth(hipStream_t s, int mode, Dims d, Op32 A, const double *r64, float *b,
+                   const float *xin, float *xout, float *dvec, float c0, float cd, float cr,
+                   const float *ecoarse);
 // bc = R (b - A x): sum of the residuals of the (up to 8) children.
 void launch_restrict(hipStream_t s, Dims df, Op32 A, const float *b, const float *x, float *bc);
 // Galerkin coarsening by 2x2x2 aggregation, scaled: writes the coarse FP64
@@ -33,17 +35,21 @@ void launch_coarse_invert(hipStream_t s, Dims d, Op64 A, float *ainv);
 void launch_coarse_solve(hipStream_t s, int n, const float *ainv, const float *b, float *x);
 
 // ---------------------------------------------------------------- Krylov (krylov.hip)
+// Reduction kernels cover this many x planes per block (fewer partials).
+constexpr int kReducePlanes = 0;  // automatic
+// Doubles of partial-sum scratch a reduction over grid d needs.
+size_t reduction_partials(Dims d);
 // r = b - A x (x may be null = 0); scalars->rr = r.r, scalars->bb = b.b.
 void launch_residual64(hipStream_t s, Dims d, Op64 A, const double *b, const double *x,
-                       double *r, double *partials, unsigned *counter, KrylovScalars *sc);
+                       double *r, double *partials, KrylovScalars *sc);
 // w = A z; dots z.w, z.q_prev, z.r; the last block forms beta, p.q and alpha.
 void launch_fcg_matvec(hipStream_t s, Dims d, Op64 A, const float *z, const double *r,
                        const double *qprev, double *w, int first, double *partials,
-                       unsigned *counter, KrylovScalars *sc);
+                       KrylovScalars *sc);
 // p = z - beta p; q = w - beta q; x += alpha p; r -= alpha q; rr = r.r.
 void launch_fcg_update(hipStream_t s, Dims d, const float *z, const double *w, double *p,
                        double *q, double *x, double *r, int first, double *partials,
-                       unsigned *counter, KrylovScalars *sc);
+                       KrylovScalars *sc);
 // z = D^{-1} r on active cells (Jacobi preconditioner).
 void launch_jacobi(hipStream_t s, Dims d, Op32 A, const double *r, float *z);
 

@@ -1,8 +1,8 @@
 // FP64 kernels of the outer flexible CG (Notay's FCG with one stored
 // direction): residual, matvec fused with its inner products, and the fused
-// vector update. Inner products use the deterministic grid reduction of
-// device_util.hpp; the scalar recurrences run in the last block so the host
-// only reads ||r||^2.
+// vector update. Each reduction writes per-block partial sums; a one-block
+// finalize kernel adds them in block order (deterministic) and runs the
+// scalar recurrences on the device, so the host only reads ||r||^2.
 #include "device_util.hpp"
 #include "kernels.hpp"
 #include "solver.hpp"
@@ -10,17 +10,19 @@
 namespace sfp {
 namespace {
 
-__global__ __launch_bounds__(kBlock) void k_residual64(Dims d, Op64 A, const double *__restrict__ b,
+enum FinalizeOp : int { FIN_RESIDUAL = 0, FIN_MATVEC = 1, FIN_UPDATE = 2, FIN_COUNT = 3 };
+
+__global__ __launch_bounds__(kBlock) void k_residual64(CellGrid cg, Dims d, Op64 A,
+                                                       const double *__restrict__ b,
                                                        const double *__restrict__ x,
-                                                       double *__restrict__ r, double *partials,
-                                                       unsigned *counter, KrylovScalars *sc) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const bool in = k < d.nz && j < d.ny;
+                                                       double *__restrict__ r,
+                                                       double *__restrict__ partials) {
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     const Strides s(d);
-    double v[2] = {0.0, 0.0};
-    if (in) {
-        for (int i = blockIdx.z; i < d.nx; i += gridDim.z) {
+    double v[2] = {0.0, 0.0};  // r.r, b.b
+    if (k < d.nz && j < d.ny) {
+        for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
             const size_t c = d.idx(i, j, k);
             const double bc = b[c];
             double rc = bc;
@@ -34,29 +36,25 @@ __global__ __launch_bounds__(kBlock) void k_residual64(Dims d, Op64 A, const dou
             v[1] += bc * bc;
         }
     }
-    if (grid_sum<2>(v, partials, counter) && flat_tid() == 0) {
-        sc->rr = v[0];
-        sc->bb = v[1];
-    }
+    block_partials<2>(v, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_fcg_matvec(Dims d, Op64 A, const float *__restrict__ z,
+__global__ __launch_bounds__(kBlock) void k_fcg_matvec(CellGrid cg, Dims d, Op64 A,
+                                                       const float *__restrict__ z,
                                                        const double *__restrict__ r,
                                                        const double *__restrict__ qprev,
                                                        double *__restrict__ w, int first,
-                                                       double *partials, unsigned *counter,
-                                                       KrylovScalars *sc) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const bool in = k < d.nz && j < d.ny;
+                                                       double *__restrict__ partials) {
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     const Strides s(d);
     double v[3] = {0.0, 0.0, 0.0};  // z.w, z.q_prev, z.r
-    if (in) {
-        for (int i = blockIdx.z; i < d.nx; i += gridDim.z) {
+    if (k < d.nz && j < d.ny) {
+        for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
             const size_t c = d.idx(i, j, k);
             const double zc = (double)z[c];
-            const double off =
-                offdiag_sum<double>(A.cpl, d, s, i, j, k, c, [&](size_t q) { return (double)z[q]; });
+            const double off = offdiag_sum<double>(A.cpl, d, s, i, j, k, c,
+                                                   [&](size_t q) { return (double)z[q]; });
             const double wc = A.diag[c] * zc - off;
             w[c] = wc;
             v[0] += zc * wc;
@@ -64,40 +62,24 @@ __global__ __launch_bounds__(kBlock) void k_fcg_matvec(Dims d, Op64 A, const flo
             v[2] += zc * r[c];
         }
     }
-    if (grid_sum<3>(v, partials, counter) && flat_tid() == 0) {
-        const double zw = v[0], zq = v[1], zr = v[2];
-        double beta = 0.0, pq = zw, pr = zr;
-        if (!first && sc->pq_prev != 0.0) {
-            // p = z - beta p_prev with beta = z.q_prev / p_prev.q_prev makes p
-            // A-orthogonal to p_prev; p.q and p.r follow from the dots above
-            // (p_prev.r = 0 after the previous step, A symmetric).
-            beta = zq / sc->pq_prev;
-            pq = zw - beta * zq;
-        }
-        sc->zw = zw;
-        sc->zq = zq;
-        sc->zr = zr;
-        sc->beta = beta;
-        sc->pq = pq;
-        sc->alpha = pq != 0.0 ? pr / pq : 0.0;
-    }
+    block_partials<3>(v, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_fcg_update(Dims d, const float *__restrict__ z,
+__global__ __launch_bounds__(kBlock) void k_fcg_update(CellGrid cg, Dims d,
+                                                       const float *__restrict__ z,
                                                        const double *__restrict__ w,
                                                        double *__restrict__ p,
                                                        double *__restrict__ q,
                                                        double *__restrict__ x,
                                                        double *__restrict__ r, int first,
-                                                       double *partials, unsigned *counter,
-                                                       KrylovScalars *sc) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const bool in = k < d.nz && j < d.ny;
+                                                       double *__restrict__ partials,
+                                                       const KrylovScalars *__restrict__ sc) {
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     const double alpha = sc->alpha, beta = sc->beta;
     double v[1] = {0.0};
-    if (in) {
-        for (int i = blockIdx.z; i < d.nx; i += gridDim.z) {
+    if (k < d.nz && j < d.ny) {
+        for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
             const size_t c = d.idx(i, j, k);
             const double zc = (double)z[c];
             double pc, qc;
@@ -116,74 +98,129 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(Dims d, const float *__re
             v[0] += rc * rc;
         }
     }
-    if (grid_sum<1>(v, partials, counter) && flat_tid() == 0) {
-        sc->rr = v[0];
-        sc->pq_prev = sc->pq;
-    }
+    block_partials<1>(v, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_jacobi(Dims d, Op32 A, const double *__restrict__ r,
+__global__ __launch_bounds__(kBlock) void k_jacobi(CellGrid cg, Dims d, Op32 A,
+                                                   const double *__restrict__ r,
                                                    float *__restrict__ z) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     if (k >= d.nz || j >= d.ny) return;
-    for (int i = blockIdx.z; i < d.nx; i += gridDim.z) {
+    for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
         const size_t c = d.idx(i, j, k);
         const float dg = A.diag[c];
         z[c] = dg > 0.f ? (float)(r[c] / (double)dg) : 0.f;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_count_active(Dims d, const float *__restrict__ diag32,
-                                                         double *partials, unsigned *counter,
-                                                         KrylovScalars *sc) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+__global__ __launch_bounds__(kBlock) void k_count_active(CellGrid cg, Dims d,
+                                                         const float *__restrict__ diag32,
+                                                         double *__restrict__ partials) {
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     double v[1] = {0.0};
     if (k < d.nz && j < d.ny) {
-        for (int i = blockIdx.z; i < d.nx; i += gridDim.z)
+        for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++)
             if (diag32[d.idx(i, j, k)] > 0.f) v[0] += 1.0;
     }
-    if (grid_sum<1>(v, partials, counter) && flat_tid() == 0) sc->pad[0] = v[0];
+    block_partials<1>(v, partials);
+}
+
+// One block: add the partials and advance the scalar recurrences.
+__global__ __launch_bounds__(kBlock) void k_finalize(int op, unsigned nb, int first,
+                                                     const double *__restrict__ partials,
+                                                     KrylovScalars *sc) {
+    double v[3] = {0.0, 0.0, 0.0};
+    if (op == FIN_MATVEC) {
+        sum_partials<3>(partials, nb, v);
+    } else if (op == FIN_RESIDUAL) {
+        double t[2];
+        sum_partials<2>(partials, nb, t);
+        v[0] = t[0];
+        v[1] = t[1];
+    } else {
+        double t[1];
+        sum_partials<1>(partials, nb, t);
+        v[0] = t[0];
+    }
+    if (flat_tid() != 0) return;
+    if (op == FIN_RESIDUAL) {
+        sc->rr = v[0];
+        sc->bb = v[1];
+    } else if (op == FIN_MATVEC) {
+        const double zw = v[0], zq = v[1], zr = v[2];
+        double beta = 0.0, pq = zw;
+        if (!first && sc->pq_prev != 0.0) {
+            // p = z - beta p_prev with beta = z.q_prev / p_prev.q_prev makes p
+            // A-orthogonal to p_prev; then p.q = z.w - beta z.q_prev, and
+            // p.r = z.r because p_prev.r = 0 after the previous step.
+            beta = zq / sc->pq_prev;
+            pq = zw - beta * zq;
+        }
+        sc->zw = zw;
+        sc->zq = zq;
+        sc->zr = zr;
+        sc->beta = beta;
+        sc->pq = pq;
+        sc->alpha = pq != 0.0 ? zr / pq : 0.0;
+    } else if (op == FIN_UPDATE) {
+        sc->rr = v[0];
+        sc->pq_prev = sc->pq;
+    } else {
+        sc->pad[0] = v[0];
+    }
+}
+
+void finalize(hipStream_t st, int op, unsigned nb, int first, const double *partials,
+              KrylovScalars *sc) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, st, op, nb, first, partials, sc);
+    SFP_LAUNCH_CHECK();
 }
 
 }  // namespace
 
+size_t reduction_partials(Dims d) { return 3 * (size_t)cell_launch(d, kReducePlanes).blocks(); }
+
 void launch_count_active(hipStream_t st, Dims d, const float *diag32, double *partials,
-                         unsigned *counter, KrylovScalars *sc) {
-    Launch L = cell_launch(d);
-    hipLaunchKernelGGL(k_count_active, L.grid, L.block, 0, st, d, diag32, partials, counter, sc);
+                         KrylovScalars *sc) {
+    Launch L = cell_launch(d, kReducePlanes);
+    hipLaunchKernelGGL(k_count_active, L.grid, L.block, 0, st, L.cg, d, diag32, partials);
     SFP_LAUNCH_CHECK();
+    finalize(st, FIN_COUNT, L.blocks(), 0, partials, sc);
 }
 
 void launch_residual64(hipStream_t st, Dims d, Op64 A, const double *b, const double *x,
-                       double *r, double *partials, unsigned *counter, KrylovScalars *sc) {
-    Launch L = cell_launch(d);
-    hipLaunchKernelGGL(k_residual64, L.grid, L.block, 0, st, d, A, b, x, r, partials, counter, sc);
+                       double *r, double *partials, KrylovScalars *sc) {
+    Launch L = cell_launch(d, kReducePlanes);
+    hipLaunchKernelGGL(k_residual64, L.grid, L.block, 0, st, L.cg, d, A, b, x, r, partials);
     SFP_LAUNCH_CHECK();
+    finalize(st, FIN_RESIDUAL, L.blocks(), 0, partials, sc);
 }
 
 void launch_fcg_matvec(hipStream_t st, Dims d, Op64 A, const float *z, const double *r,
                        const double *qprev, double *w, int first, double *partials,
-                       unsigned *counter, KrylovScalars *sc) {
-    Launch L = cell_launch(d);
-    hipLaunchKernelGGL(k_fcg_matvec, L.grid, L.block, 0, st, d, A, z, r, qprev, w, first,
-                       partials, counter, sc);
+                       KrylovScalars *sc) {
+    Launch L = cell_launch(d, kReducePlanes);
+    hipLaunchKernelGGL(k_fcg_matvec, L.grid, L.block, 0, st, L.cg, d, A, z, r, qprev, w, first,
+                       partials);
     SFP_LAUNCH_CHECK();
+    finalize(st, FIN_MATVEC, L.blocks(), first, partials, sc);
 }
 
 void launch_fcg_update(hipStream_t st, Dims d, const float *z, const double *w, double *p,
                        double *q, double *x, double *r, int first, double *partials,
-                       unsigned *counter, KrylovScalars *sc) {
-    Launch L = cell_launch(d);
-    hipLaunchKernelGGL(k_fcg_update, L.grid, L.block, 0, st, d, z, w, p, q, x, r, first,
-                       partials, counter, sc);
+                       KrylovScalars *sc) {
+    Launch L = cell_launch(d, kReducePlanes);
+    hipLaunchKernelGGL(k_fcg_update, L.grid, L.block, 0, st, L.cg, d, z, w, p, q, x, r, first,
+                       partials, sc);
     SFP_LAUNCH_CHECK();
+    finalize(st, FIN_UPDATE, L.blocks(), first, partials, sc);
 }
 
 void launch_jacobi(hipStream_t st, Dims d, Op32 A, const double *r, float *z) {
     Launch L = cell_launch(d);
-    hipLaunchKernelGGL(k_jacobi, L.grid, L.block, 0, st, d, A, r, z);
+    hipLaunchKernelGGL(k_jacobi, L.grid, L.block, 0, st, L.cg, d, A, r, z);
     SFP_LAUNCH_CHECK();
 }
 

@@ -4,26 +4,32 @@
 // the dense coarsest solve.
 //
 // Threads map to cells with z (the unit-stride axis of the reference layout)
-// across the 64 lanes of a wavefront, y across the 4 wavefronts of a block,
-// and x planes looped over with stride gridDim.z.
+// across the 64 lanes of a wavefront and y across the 4 wavefronts of a
+// block; each block covers one x plane of its tile (see cell_tile).
 #include "device_util.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace sfp {
 
-Launch cell_launch(Dims d, unsigned target_blocks) {
+Launch cell_launch(Dims d, int planes) {
     Launch L;
     unsigned tz = 64;
     while (tz > 1 && (int)tz / 2 >= d.nz) tz /= 2;
-    unsigned ty = kBlock / tz;
-    unsigned gx = (d.nz + tz - 1) / tz;
-    unsigned gy = (d.ny + ty - 1) / ty;
-    unsigned per_plane = gx * gy;
-    unsigned gz = target_blocks / per_plane;
-    if (gz < 1) gz = 1;
-    if (gz > (unsigned)d.nx) gz = d.nx;
+    const unsigned ty = kBlock / tz;
+    L.cg.gx = (d.nz + tz - 1) / tz;
+    L.cg.gy = (d.ny + ty - 1) / ty;
+    if (planes <= 0) {
+        // march over several x planes per block (x-neighbour reuse in
+        // registers) while keeping >= ~2048 blocks to fill 256 CUs
+        const long long tiles = (long long)L.cg.gx * L.cg.gy * d.nx;
+        planes = (int)std::min<long long>(16, std::max<long long>(1, tiles / 2048));
+    }
+    L.cg.planes = planes;
+    const unsigned gz = (d.nx + L.cg.planes - 1) / L.cg.planes;
     L.block = dim3(tz, ty, 1);
-    L.grid = dim3(gx, gy, gz);
+    L.grid = dim3(L.cg.gx * L.cg.gy * gz, 1, 1);
     return L;
 }
 
@@ -34,12 +40,12 @@ __device__ __forceinline__ float rcp_or_zero(float dg) {
 }
 
 template <bool FROM64>
-__global__ __launch_bounds__(kBlock) void k_smooth_first(Dims d, Op32 A, const double *r64,
+__global__ __launch_bounds__(kBlock) void k_smooth_first(CellGrid cg, Dims d, Op32 A, const double *r64,
                                                          float *b, float *x, float c0) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     if (k >= d.nz || j >= d.ny) return;
-    for (int i = blockIdx.z; i < d.nx; i += gridDim.z) {
+    for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
         const size_t c = d.idx(i, j, k);
         float bv;
         if constexpr (FROM64) {
@@ -52,66 +58,88 @@ __global__ __launch_bounds__(kBlock) void k_smooth_first(Dims d, Op32 A, const d
     }
 }
 
+// One Chebyshev step xout = x + d', d' = cd d + cr D^{-1} (b - A x), marching
+// over the block's x planes with the x-direction neighbours kept in
+// registers. The input iterate x is, by mode:
+//   SM_FIRST   : the zero-start first step c0 D^{-1} b, recomputed at every
+//                stencil point (b read from r64 and rounded when r64 != null,
+//                which also stores b); the previous direction equals it
+//   SM_PROLONG : xin + P e (piecewise-constant coarse correction)
+//   otherwise  : xin
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_smooth(Dims d, Op32 A, const float *__restrict__ b,
+__global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
+                                                   const double *__restrict__ r64,
+                                                   float *__restrict__ b,
                                                    const float *__restrict__ xin,
                                                    float *__restrict__ xout, float *dvec,
-                                                   float cd, float cr,
+                                                   float c0, float cd, float cr,
                                                    const float *__restrict__ e, Dims dc) {
+    constexpr bool kFirst = (MODE & SM_FIRST) != 0;
     constexpr bool kProlong = (MODE & SM_PROLONG) != 0;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
     if (k >= d.nz || j >= d.ny) return;
     const Strides s(d);
     const Strides sc(dc);
-    for (int i = blockIdx.z; i < d.nx; i += gridDim.z) {
-        const size_t c = d.idx(i, j, k);
-        const float dg = A.diag[c];
-        float xc = xin[c];
-        float xm[3], xp[3];
-        xm[0] = i > 0 ? xin[c - s.sx] : 0.f;
-        xp[0] = i + 1 < d.nx ? xin[c + s.sx] : 0.f;
-        xm[1] = j > 0 ? xin[c - s.sy] : 0.f;
-        xp[1] = j + 1 < d.ny ? xin[c + s.sy] : 0.f;
-        xm[2] = k > 0 ? xin[c - 1] : 0.f;
-        xp[2] = k + 1 < d.nz ? xin[c + 1] : 0.f;
-        if constexpr (kProlong) {
-            // piecewise-constant prolongation of the coarse correction e
-            const size_t pc = dc.idx(i >> 1, j >> 1, k >> 1);
-            const float ec = e[pc];
-            if (dg > 0.f) xc += ec;
-            if (i > 0) xm[0] += (i & 1) ? ec : e[pc - sc.sx];
-            if (i + 1 < d.nx) xp[0] += (i & 1) ? e[pc + sc.sx] : ec;
-            if (j > 0) xm[1] += (j & 1) ? ec : e[pc - sc.sy];
-            if (j + 1 < d.ny) xp[1] += (j & 1) ? e[pc + sc.sy] : ec;
-            if (k > 0) xm[2] += (k & 1) ? ec : e[pc - 1];
-            if (k + 1 < d.nz) xp[2] += (k & 1) ? e[pc + 1] : ec;
+    auto bval = [&](size_t q) -> float { return r64 ? (float)r64[q] : b[q]; };
+    // iterate value at a neighbour point q (cell coordinates qi, qj, qk)
+    auto xval = [&](size_t q, int qi, int qj, int qk) -> float {
+        if constexpr (kFirst) {
+            return c0 * rcp_or_zero(A.diag[q]) * bval(q);
+        } else if constexpr (kProlong) {
+            return xin[q] + e[dc.idx(qi >> 1, qj >> 1, qk >> 1)];
+        } else {
+            return xin[q];
         }
-        float off = 0.f;
-        off += A.cpl[0][c] * xm[0];
-        if (i + 1 < d.nx) off += A.cpl[0][c + s.sx] * xp[0];
-        off += A.cpl[1][c] * xm[1];
-        if (j + 1 < d.ny) off += A.cpl[1][c + s.sy] * xp[1];
-        off += A.cpl[2][c] * xm[2];
-        if (k + 1 < d.nz) off += A.cpl[2][c + 1] * xp[2];
-        const float r = b[c] - (dg * xc - off);
+    };
+    const int ie = min(i0 + cg.planes, d.nx);
+    size_t c = d.idx(i0, j, k);
+    float xm = i0 > 0 ? xval(c - s.sx, i0 - 1, j, k) : 0.f;
+    float xc = xval(c, i0, j, k);
+    float cxc = A.cpl[0][c];
+    for (int i = i0; i < ie; i++, c += s.sx) {
+        const bool up = i + 1 < d.nx;
+        const float xp = up ? xval(c + s.sx, i + 1, j, k) : 0.f;
+        const float cxp = up ? A.cpl[0][c + s.sx] : 0.f;
+        const float dg = A.diag[c];
+        float off = cxc * xm + cxp * xp;
+        if (j > 0) off += A.cpl[1][c] * xval(c - s.sy, i, j - 1, k);
+        if (j + 1 < d.ny) off += A.cpl[1][c + s.sy] * xval(c + s.sy, i, j + 1, k);
+        if (k > 0) off += A.cpl[2][c] * xval(c - 1, i, j, k - 1);
+        if (k + 1 < d.nz) off += A.cpl[2][c + 1] * xval(c + 1, i, j, k + 1);
+        float bc;
+        if constexpr (kFirst) {
+            bc = bval(c);
+            if (r64) b[c] = bc;
+        } else {
+            bc = b[c];
+        }
+        // centre iterate: the coarse correction only lands on active cells
+        float x0 = xc;
+        if constexpr (kProlong) {
+            if (!(dg > 0.f)) x0 = xin[c];
+        }
+        const float r = bc - (dg * x0 - off);
         float dprev = 0.f;
-        if constexpr ((MODE & SM_D_FROM_X) != 0) dprev = xin[c];
+        if constexpr (kFirst) dprev = x0;
         if constexpr ((MODE & SM_READ_D) != 0) dprev = dvec[c];
         const float dn = cd * dprev + cr * rcp_or_zero(dg) * r;
-        xout[c] = xc + dn;
+        xout[c] = x0 + dn;
         if constexpr ((MODE & SM_WRITE_D) != 0) dvec[c] = dn;
+        xm = xc;
+        xc = xp;
+        cxc = cxp;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_restrict(Dims df, Op32 A, const float *__restrict__ b,
+__global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 A, const float *__restrict__ b,
                                                      const float *__restrict__ x, Dims dc,
                                                      float *__restrict__ bc) {
-    const int K = blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    int K, J, I0;
+    cell_tile(cg, K, J, I0);
     if (K >= dc.nz || J >= dc.ny) return;
     const Strides s(df);
-    for (int I = blockIdx.z; I < dc.nx; I += gridDim.z) {
+    for (int I = I0, Ie = min(I0 + cg.planes, dc.nx); I < Ie; I++) {
         float sum = 0.f;
         for (int di = 0; di < 2; di++) {
             const int i = 2 * I + di;
@@ -155,17 +183,17 @@ __device__ __forceinline__ double layer_coupling(Dims df, const double *cpl, int
     return sum;
 }
 
-__global__ __launch_bounds__(kBlock) void k_coarsen(Dims df, Op64 Af, const float *diag32f,
+__global__ __launch_bounds__(kBlock) void k_coarsen(CellGrid cg, Dims df, Op64 Af, const float *diag32f,
                                                     Dims dc, double scale, double *diag64c,
                                                     MutFaces cpl64c, float *diag32c,
                                                     float *cpl32c0, float *cpl32c1,
                                                     float *cpl32c2) {
-    const int K = blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    int K, J, I0;
+    cell_tile(cg, K, J, I0);
     if (K >= dc.nz || J >= dc.ny) return;
     const Strides s(df);
     float *cpl32c[3] = {cpl32c0, cpl32c1, cpl32c2};
-    for (int I = blockIdx.z; I < dc.nx; I += gridDim.z) {
+    for (int I = I0, Ie = min(I0 + cg.planes, dc.nx); I < Ie; I++) {
         bool active = false;
         double extra = 0.0;
         for (int i = 2 * I; i < 2 * I + 2 && i < df.nx; i++)
@@ -286,24 +314,25 @@ void launch_smooth_first(hipStream_t st, Dims d, Op32 A, const double *r64, floa
                          float c0) {
     Launch L = cell_launch(d);
     if (r64)
-        hipLaunchKernelGGL(k_smooth_first<true>, L.grid, L.block, 0, st, d, A, r64, b, x, c0);
+        hipLaunchKernelGGL(k_smooth_first<true>, L.grid, L.block, 0, st, L.cg, d, A, r64, b, x, c0);
     else
-        hipLaunchKernelGGL(k_smooth_first<false>, L.grid, L.block, 0, st, d, A, r64, b, x, c0);
+        hipLaunchKernelGGL(k_smooth_first<false>, L.grid, L.block, 0, st, L.cg, d, A, r64, b, x, c0);
     SFP_LAUNCH_CHECK();
 }
 
-void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const float *b, const float *xin,
-                   float *xout, float *dvec, float cd, float cr, const float *ecoarse) {
+void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, float *b,
+                   const float *xin, float *xout, float *dvec, float c0, float cd, float cr,
+                   const float *ecoarse) {
     Launch L = cell_launch(d);
     Dims dc = d.coarse();
-#define SFP_SMOOTH_CASE(M)                                                                  \
-    case M:                                                                                 \
-        hipLaunchKernelGGL(k_smooth<M>, L.grid, L.block, 0, st, d, A, b, xin, xout, dvec, cd, \
-                           cr, ecoarse, dc);                                                \
+#define SFP_SMOOTH_CASE(M)                                                                    \
+    case M:                                                                                   \
+        hipLaunchKernelGGL(k_smooth<M>, L.grid, L.block, 0, st, L.cg, d, A, r64, b, xin, xout, \
+                           dvec, c0, cd, cr, ecoarse, dc);                                    \
         break;
     switch (mode) {
-        SFP_SMOOTH_CASE(SM_D_FROM_X)
-        SFP_SMOOTH_CASE(SM_D_FROM_X | SM_WRITE_D)
+        SFP_SMOOTH_CASE(SM_FIRST)
+        SFP_SMOOTH_CASE(SM_FIRST | SM_WRITE_D)
         SFP_SMOOTH_CASE(SM_READ_D)
         SFP_SMOOTH_CASE(SM_READ_D | SM_WRITE_D)
         SFP_SMOOTH_CASE(SM_PROLONG)
@@ -319,8 +348,8 @@ void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const float *b, con
 
 void launch_restrict(hipStream_t st, Dims df, Op32 A, const float *b, const float *x, float *bc) {
     Dims dc = df.coarse();
-    Launch L = cell_launch(dc);
-    hipLaunchKernelGGL(k_restrict, L.grid, L.block, 0, st, df, A, b, x, dc, bc);
+    Launch L = cell_launch(dc, 1);
+    hipLaunchKernelGGL(k_restrict, L.grid, L.block, 0, st, L.cg, df, A, b, x, dc, bc);
     SFP_LAUNCH_CHECK();
 }
 
@@ -329,7 +358,7 @@ void launch_coarsen(hipStream_t st, Dims df, Op64 Af, const float *diag32f, Dims
                     float *const cpl32c[3]) {
     Launch L = cell_launch(dc);
     MutFaces m{{cpl64c[0], cpl64c[1], cpl64c[2]}};
-    hipLaunchKernelGGL(k_coarsen, L.grid, L.block, 0, st, df, Af, diag32f, dc, scale, diag64c, m,
+    hipLaunchKernelGGL(k_coarsen, L.grid, L.block, 0, st, L.cg, df, Af, diag32f, dc, scale, diag64c, m,
                        diag32c, cpl32c[0], cpl32c[1], cpl32c[2]);
     SFP_LAUNCH_CHECK();
 }

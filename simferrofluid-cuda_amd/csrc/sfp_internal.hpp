@@ -52,13 +52,20 @@ struct Dims {
     __host__ __device__ int n(int a) const { return a == 0 ? nx : (a == 1 ? ny : nz); }
 };
 
-// Launch shape for a cell-parallel kernel: threads along z (unit stride),
-// then y; x planes are looped over with stride gridDim.z.
+// Launch shape for a cell-parallel kernel: a 1-D grid of 256-thread blocks,
+// each covering a tz x ty tile (z, the unit-stride axis, across the 64 lanes
+// of a wavefront; y across wavefronts) of `planes` consecutive x planes.
+struct CellGrid {
+    unsigned gx, gy;  // tiles per x plane along z and y
+    int planes;       // consecutive x planes per block
+};
 struct Launch {
     dim3 grid, block;
-    unsigned blocks() const { return grid.x * grid.y * grid.z; }
+    CellGrid cg;
+    unsigned blocks() const { return grid.x; }
 };
-Launch cell_launch(Dims d, unsigned target_blocks = 8192);
+// planes <= 0 picks the number of x planes per block from the grid size.
+Launch cell_launch(Dims d, int planes = 0);
 
 // ------------------------------------------------------------------ device memory
 template <class T>

@@ -155,31 +155,38 @@ float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) 
     const Op32 A = L.op32();
     float *cur = L.x0.p, *oth = L.x1.p;
     const int deg = cb.degree;
-    launch_smooth_first(st, L.d, A, r64, L.b.p, cur, cb.cr[0]);
-    auto smooth = [&](int mode, float cd, float cr, const float *e) {
+    auto smooth = [&](int mode, float cd, float cr, const double *r, const float *e) {
         const bool track = fine && ws.profiling;
         if (track) {
-            double per = 24.0;  // x, b, diag, 3 couplings
-            per += 4.0;         // x'
+            // algorithmic bytes: b (or r64 + b), diag, 3 couplings, x' + optional
+            // x, d, d', coarse e
+            double per = 4.0 + 4.0 + 12.0 + 4.0;
+            if (!(mode & SM_FIRST)) per += 4.0;
+            if (r) per += 8.0;
             if (mode & SM_READ_D) per += 4.0;
             if (mode & SM_WRITE_D) per += 4.0;
             if (mode & SM_PROLONG) per += 0.5;
             ws.timers[K_SMOOTH].begin(st, per * cells);
         }
-        launch_smooth(st, mode, L.d, A, L.b.p, cur, oth, L.dv.p, cd, cr, e);
+        launch_smooth(st, mode, L.d, A, r, L.b.p, cur, oth, L.dv.p, cb.cr[0], cd, cr, e);
         if (track) ws.timers[K_SMOOTH].end(st);
         std::swap(cur, oth);
     };
-    for (int s = 1; s < deg; s++)
-        smooth((s == 1 ? SM_D_FROM_X : SM_READ_D) | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s],
-               nullptr);
+    if (deg == 1) {
+        launch_smooth_first(st, L.d, A, r64, L.b.p, cur, cb.cr[0]);
+    } else {
+        // steps 0 and 1 in one pass; then steps 2 .. deg-1
+        smooth(SM_FIRST | (deg > 2 ? SM_WRITE_D : 0), cb.cd[1], cb.cr[1], r64, nullptr);
+        for (int s = 2; s < deg; s++)
+            smooth(SM_READ_D | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s], nullptr, nullptr);
+    }
     if (fine && ws.profiling) ws.timers[K_RESTRICT].begin(st, (24.0 + 0.5) * cells);
     launch_restrict(st, L.d, A, L.b.p, cur, lv_[l + 1].b.p);
     if (fine && ws.profiling) ws.timers[K_RESTRICT].end(st);
     const float *e = vlevel(ws, cb, l + 1, nullptr);
-    smooth(SM_PROLONG | (deg > 1 ? SM_WRITE_D : 0), 0.f, cb.cr[0], e);
+    smooth(SM_PROLONG | (deg > 1 ? SM_WRITE_D : 0), 0.f, cb.cr[0], nullptr, e);
     for (int s = 1; s < deg; s++)
-        smooth(SM_READ_D | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s], nullptr);
+        smooth(SM_READ_D | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s], nullptr, nullptr);
     return cur;
 }
 
@@ -193,7 +200,7 @@ float *System::precondition(Workspace &ws, const SolverConfig &cfg, const double
 }
 
 long long System::count_active(Workspace &ws) {
-    launch_count_active(ws.stream, d_, lv_[0].diag32.p, ws.partials.p, ws.counter.p, ws.scalars.p);
+    launch_count_active(ws.stream, d_, lv_[0].diag32.p, ws.partials.p, ws.scalars.p);
     ws.read_scalars();
     return (long long)ws.host_sc->pad[0];
 }
@@ -205,7 +212,7 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
     const size_t n = d_.cells();
     const Op64 A = lv_[0].op64();
     if (!warm) SFP_HIP(hipMemsetAsync(x_.p, 0, n * sizeof(double), st));
-    launch_residual64(st, d_, A, rhs_.p, warm ? x_.p : nullptr, r_.p, ws.partials.p, ws.counter.p,
+    launch_residual64(st, d_, A, rhs_.p, warm ? x_.p : nullptr, r_.p, ws.partials.p,
                       ws.scalars.p);
     ws.read_scalars();
     const double bb = ws.host_sc->bb;
@@ -224,7 +231,7 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
             if (checked) break;
             // recursive residual reached tol: recompute it from x in FP64 and
             // restart from the true residual if the two have drifted apart
-            launch_residual64(st, d_, A, rhs_.p, x_.p, r_.p, ws.partials.p, ws.counter.p,
+            launch_residual64(st, d_, A, rhs_.p, x_.p, r_.p, ws.partials.p,
                               ws.scalars.p);
             ws.read_scalars();
             rel = std::sqrt(ws.host_sc->rr / bb);
@@ -236,11 +243,11 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
         const float *z = precondition(ws, cfg, r_.p);
         if (ws.profiling) ws.timers[K_MATVEC64].begin(st, (4.0 + 32.0 + 8.0 + 8.0 + 8.0) * cells);
         launch_fcg_matvec(st, d_, A, z, r_.p, q_.p, w_.p, first ? 1 : 0, ws.partials.p,
-                          ws.counter.p, ws.scalars.p);
+                          ws.scalars.p);
         if (ws.profiling) ws.timers[K_MATVEC64].end(st);
         if (ws.profiling) ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 5 + 8.0 * 4) * cells);
         launch_fcg_update(st, d_, z, w_.p, p_.p, q_.p, x_.p, r_.p, first ? 1 : 0, ws.partials.p,
-                          ws.counter.p, ws.scalars.p);
+                          ws.scalars.p);
         if (ws.profiling) ws.timers[K_UPDATE64].end(st);
         first = false;
         info.iterations++;
@@ -249,7 +256,7 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
         checked = false;
     }
     if (!checked) {
-        launch_residual64(st, d_, A, rhs_.p, x_.p, r_.p, ws.partials.p, ws.counter.p, ws.scalars.p);
+        launch_residual64(st, d_, A, rhs_.p, x_.p, r_.p, ws.partials.p, ws.scalars.p);
         ws.read_scalars();
         rel = std::sqrt(ws.host_sc->rr / bb);
     }

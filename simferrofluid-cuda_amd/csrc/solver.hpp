@@ -47,7 +47,6 @@ struct KernelTimer {
 struct Workspace {
     hipStream_t stream = nullptr;
     DevBuf<double> partials;
-    DevBuf<unsigned> counter;
     DevBuf<KrylovScalars> scalars;
     DevBuf<int> err;
     KrylovScalars *host_sc = nullptr;  // pinned
@@ -94,6 +93,6 @@ class System {
 };
 
 void launch_count_active(hipStream_t s, Dims d, const float *diag32, double *partials,
-                         unsigned *counter, KrylovScalars *sc);
+                         KrylovScalars *sc);
 
 }  // namespace sfp

@@ -66,6 +66,7 @@ class SfpError(RuntimeError):
 
 
 P3 = _dp * 3
+PP = C.POINTER(_dp)  # parameter type of a `double *const a[3]` argument
 
 
 def lib():
@@ -98,11 +99,11 @@ def lib():
                      "sfp_pressure_project_device", "sfp_pressure_reproject_device"):
             f = getattr(L, name)
             f.restype = C.c_int
-            f.argtypes = [vp, P3, _dp, P3, P3, vp, C.c_double, _dp, C.POINTER(Stats)]
+            f.argtypes = [vp, PP, _dp, PP, PP, vp, C.c_double, _dp, C.POINTER(Stats)]
         for name in ("sfp_magnetic_solve", "sfp_magnetic_solve_device"):
             f = getattr(L, name)
             f.restype = C.c_int
-            f.argtypes = [vp, _dp, P3, C.c_double, C.c_double, _dp, vp, C.POINTER(Stats)]
+            f.argtypes = [vp, _dp, PP, C.c_double, C.c_double, _dp, vp, C.POINTER(Stats)]
         L.sfp_magnetic_vertex_pressures.restype = C.c_int
         L.sfp_magnetic_vertex_pressures.argtypes = [vp, C.c_int, _dp, _dp, C.c_double, _dp]
         L.sfp_profile_enable.restype = C.c_int

@@ -93,16 +93,24 @@ class Geometry:
 
 
 def _box_smooth(u: np.ndarray, passes: int) -> np.ndarray:
-    """``passes`` rounds of the 3x3x3 box mean with clamped (edge) indexing."""
+    """``passes`` rounds of the 3x3x3 box mean with clamped (edge) indexing,
+    applied as three 1-D 3-point means (the clamped box is separable)."""
     for _ in range(passes):
-        p = np.pad(u, 1, mode="edge")
-        acc = np.zeros_like(u)
-        sx, sy, sz = u.shape
-        for dx in range(3):
-            for dy in range(3):
-                for dz in range(3):
-                    acc += p[dx:dx + sx, dy:dy + sy, dz:dz + sz]
-        u = acc / 27.0
+        for a in range(3):
+            lo = [slice(None)] * 3
+            hi = [slice(None)] * 3
+            lo[a] = slice(0, -1)
+            hi[a] = slice(1, None)
+            m = u.copy()
+            m[tuple(hi)] += u[tuple(lo)]   # left neighbour
+            m[tuple(lo)] += u[tuple(hi)]   # right neighbour
+            e0 = [slice(None)] * 3
+            e1 = [slice(None)] * 3
+            e0[a] = slice(0, 1)
+            e1[a] = slice(-1, None)
+            m[tuple(e0)] += u[tuple(e0)]   # clamped left edge
+            m[tuple(e1)] += u[tuple(e1)]   # clamped right edge
+            u = m / 3.0
     return u
 
 

@@ -1,0 +1,314 @@
+"""GPU tests through the C ABI: the C++ drop-in driver, closed forms at test
+and full (256^3) sizes, configuration 0/1 against the oracle, and the edge
+cases of the reference's behaviour."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import sfp
+from sfp import scenes
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_cpp_dropin_driver():
+    exe = os.path.join(ROOT, "tests", "cpp", "dropin_test")
+    print("[case] C++ drop-in driver", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=100)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PASSED" in r.stdout
+
+
+# ------------------------------------------------------------------ closed forms
+def _planar_magnetic(n, H=1.3, chi=3.0):
+    geo = scenes.Geometry(n, 1.0 / n)
+    X, Y, Z = geo.cell_centers()
+    phi = np.ascontiguousarray(np.broadcast_to(Y - 0.45 - 0.3 / n + 0 * X + 0 * Z, geo.n)).copy()
+    return dict(geo=geo, phi=phi, chi=chi, band=1.5 / n,
+                hext=[np.full(geo.face_shape(a), (0.0, H, 0.0)[a]) for a in range(3)])
+
+
+@pytest.mark.parametrize("n", [48, 256])
+def test_k1_planar_interface_flux(n):
+    print(f"[case] K1 planar interface n={n}", flush=True)
+    sc = _planar_magnetic(n)
+    with sfp.Context.for_scene(sc, tol=1e-10) as ctx:
+        pot, Ht, st = ctx.magnetic_solve(sc)
+    # mu on interior y-faces, reconstructed from B = mu H being H_ext:
+    # check B_y is constant by comparing against mu computed from phi
+    import oracle
+    mu = oracle.magnetic_coeff(sc) if n <= 64 else None
+    H = 1.3
+    inner = (slice(None), slice(1, n), slice(None))
+    if mu is not None:
+        assert np.abs(mu[1][inner] * Ht[1][inner] - H).max() < 1e-7 * H
+    # potential constant on every y-layer; transverse field vanishes
+    P = pot.reshape(sc["geo"].n)
+    assert np.abs(P - P[:1, :, :1]).max() <= 1e-7 * np.abs(P).max()
+    for a in (0, 2):
+        s = [slice(None)] * 3
+        s[a] = slice(1, n)
+        assert np.abs(Ht[a][tuple(s)]).max() < 1e-7 * H
+    # far from the band mu is 1 (air) or 1+chi (liquid): B_y = H there
+    hy = Ht[1][0, 1:n, 0]
+    ys = (np.arange(1, n)) / n
+    air = ys > 0.45 + 3.0 / n
+    liq = ys < 0.45 - 3.0 / n
+    assert np.abs(hy[air] - H).max() < 1e-7 * H
+    assert np.abs(hy[liq] * (1 + sc["chi"]) - H).max() < 1e-7 * H
+    print(f"  iters={st['iterations']} rel={st['rel_residual']:.2e}")
+
+
+@pytest.mark.parametrize("n", [40, 256])
+def test_k3_divergence_after_projection(n):
+    print(f"[case] K3 divergence n={n}", flush=True)
+    sc = scenes.pressure_scene(n, seed=4)
+    sc["vol_error"] = 0.0021
+    tol = 1e-10 if n < 100 else 1e-8
+    with sfp.Context.for_scene(sc, tol=tol) as ctx:
+        vel, p, st = ctx.pressure_project(sc, want_pressure=False)
+    assert st["converged"] == 1
+    liq = sc["phi"] <= 0
+    out = np.zeros(sc["geo"].n)
+    for a in range(3):
+        w = 1 - sc["fraction"][a]
+        flux = w * vel[a] + (1 - w) * sc["solid_velocity"][a]
+        lo = [slice(0, n)] * 3
+        hi = [slice(0, n)] * 3
+        hi[a] = slice(1, n + 1)
+        out += flux[tuple(hi)] - flux[tuple(lo)]
+    # per-cell divergence error = per-cell residual <= ||r||_2 <= tol ||b||_2
+    import oracle
+    if n < 100:
+        bnorm = np.linalg.norm(oracle.pressure_system(sc)["rhs"])
+    else:
+        bnorm = None
+    err = np.abs(out[liq] - sc["vol_error"]).max()
+    print(f"  iters={st['iterations']} max|div - volError|={err:.2e}")
+    if bnorm is not None:
+        assert err <= 1.01 * tol * bnorm + 1e-14
+    else:
+        scale = max(np.abs(v).max() for v in sc["velocity"])
+        assert err <= 1e-6 * scale
+
+
+def test_k2_hydrostatic_column_on_gpu():
+    n = 64
+    print("[case] K2 hydrostatic n=64", flush=True)
+    geo = scenes.Geometry(n, 1.0 / n)
+    X, Y, Z = geo.cell_centers()
+    phi = np.ascontiguousarray(np.broadcast_to(Y - 0.5371 + 0 * X + 0 * Z, geo.n)).copy()
+    g_dt = 9.8e-3
+    vel = [np.zeros(geo.face_shape(a)) for a in range(3)]
+    vel[1][:] = -g_dt
+    sc = dict(geo=geo, phi=phi, velocity=vel, fraction=scenes.wall_fractions(geo),
+              solid_velocity=[np.zeros(geo.face_shape(a)) for a in range(3)], jump=None,
+              vol_error=0.0)
+    with sfp.Context.for_scene(sc, tol=1e-12) as ctx:
+        new, p, st = ctx.pressure_project(sc)
+    liq = phi <= 0
+    lo = [slice(None)] * 3
+    inner = (slice(None), slice(1, n), slice(None))
+    touched = np.zeros(geo.face_shape(1), bool)
+    touched[:, 1:n, :] = liq[:, :n - 1, :] | liq[:, 1:, :]
+    assert np.abs(new[1][touched]).max() < 1e-9 * g_dt
+    assert np.abs(new[0]).max() < 1e-9 * g_dt and np.abs(new[2]).max() < 1e-9 * g_dt
+
+
+# ------------------------------------------------------------------ configurations vs oracle
+def test_config0_pressure_64_vs_oracle(oracle):
+    print("[case] config 0: pressure 64^3 vs oracle", flush=True)
+    sc = scenes.pressure_scene(64, seed=0)
+    with sfp.Context.for_scene(sc, tol=1e-8) as ctx:
+        vel, p, st = ctx.pressure_project(sc)
+    ref = oracle.pressure_system(sc)
+    x = p.ravel()[ref["mat2grid"]]
+    rel = oracle.rel_residual(ref["A"], ref["rhs"], x)
+    xr, it, relr, _, _ = oracle.solve(ref["A"], ref["rhs"], "bicgstab", "amg", tol=1e-12)
+    linf = np.abs(x - xr).max() / np.abs(xr).max()
+    print(f"  gpu iters={st['iterations']} oracle rel={rel:.2e} Linf={linf:.2e}")
+    assert rel <= 1e-8 * 1.01
+    assert linf < 1e-6
+    ref_vel = oracle.pressure_apply(sc, ref["grid2mat"], x)
+    for a in range(3):
+        assert np.array_equal(vel[a], ref_vel[a])
+
+
+def test_config1_magnetic_128_vs_oracle(oracle):
+    print("[case] config 1: magnetic 128^3 chi=2 vs oracle", flush=True)
+    sc = scenes.magnetic_scene(128, seed=0, chi=2.0)
+    with sfp.Context.for_scene(sc, tol=1e-8) as ctx:
+        pot, H, st = ctx.magnetic_solve(sc)
+    ref = oracle.magnetic_system(sc)
+    rel = oracle.rel_residual(ref["A"], ref["rhs"], pot.ravel())
+    xr, it, relr, _, _ = oracle.solve(ref["A"], ref["rhs"], "cg", "amg", tol=1e-12)
+    linf = np.abs(pot.ravel() - xr).max() / np.abs(xr).max()
+    print(f"  gpu iters={st['iterations']} oracle rel={rel:.2e} Linf={linf:.2e}")
+    assert rel <= 1e-8 * 1.01
+    assert linf < 1e-5
+    H_ref = oracle.magnetic_total_field(sc, pot)
+    for a in range(3):
+        assert np.array_equal(H[a], H_ref[a])
+
+
+# ------------------------------------------------------------------ edge cases
+def test_no_liquid_leaves_velocity_untouched():
+    sc = scenes.random_pressure_scene(10, seed=1)
+    sc["phi"] = np.abs(sc["phi"]) + 0.1
+    with sfp.Context.for_scene(sc) as ctx:
+        vel, p, st = ctx.pressure_project(sc)
+        assert st["unknowns"] == 0 and st["iterations"] == 0
+        vel2, p2, st2 = ctx.pressure_project(sc, reproject=True)
+    for a in range(3):
+        assert np.array_equal(vel[a], sc["velocity"][a])
+        assert np.array_equal(vel2[a], sc["velocity"][a])
+    assert np.all(p == 0)
+
+
+def test_all_liquid_closed_box_consistent_singular_system():
+    # no air: a pure-Neumann system; with closed walls, zero solid velocity and
+    # no volume correction the right-hand side sums to zero (consistent)
+    sc = scenes.pressure_scene(24, seed=2)
+    sc["phi"] = -np.ones_like(sc["phi"])
+    with sfp.Context.for_scene(sc, tol=1e-8, max_iters=200) as ctx:
+        vel, p, st = ctx.pressure_project(sc)
+    print(f"  singular consistent: iters={st['iterations']} rel={st['rel_residual']:.2e}")
+    assert st["converged"] == 1
+    assert all(np.all(np.isfinite(v)) for v in vel)
+
+
+def test_inconsistent_singular_system_terminates():
+    sc = scenes.pressure_scene(16, seed=2)
+    sc["phi"] = -np.ones_like(sc["phi"])
+    sc["vol_error"] = 0.5
+    with sfp.Context.for_scene(sc, tol=1e-8, max_iters=30) as ctx:
+        vel, p, st = ctx.pressure_project(sc)
+    assert st["converged"] == 0 and st["iterations"] == 30
+    assert np.isfinite(st["rel_residual"])
+
+
+def test_isolated_and_enclosed_liquid_cells(oracle):
+    sc = scenes.random_pressure_scene(12, seed=6)
+    phi = np.abs(sc["phi"]) + 0.05
+    phi[6, 9, 6] = -0.01          # single liquid cell in air
+    phi[3, 8, 3] = -0.01          # single liquid cell with every face closed
+    sc["phi"] = phi
+    for a in range(3):
+        f = sc["fraction"][a]
+        c = [3, 8, 3]
+        f[tuple(c)] = 1.0
+        c[a] += 1
+        f[tuple(c)] = 1.0
+    with sfp.Context.for_scene(sc, tol=1e-12) as ctx:
+        vel, p, st = ctx.pressure_project(sc)
+    ref = oracle.pressure_system(sc)
+    assert ref["n"] == 2 and st["unknowns"] == 2
+    x = p.ravel()[ref["mat2grid"]]
+    assert oracle.rel_residual(ref["A"], ref["rhs"], x) <= 1e-12
+    assert ref["A"][0, 0] == 1.0  # diagCoeff == 0 -> 1 for the enclosed cell
+    ref_vel = oracle.pressure_apply(sc, ref["grid2mat"], x)
+    for a in range(3):
+        assert np.array_equal(vel[a], ref_vel[a])
+
+
+def test_open_boundary_face_next_to_liquid_is_rejected():
+    sc = scenes.pressure_scene(12, seed=1)
+    sc["fraction"][0][0, :, :] = 0.0  # open the x = 0 wall under the pool
+    with sfp.Context.for_scene(sc) as ctx:
+        with pytest.raises(sfp.SfpError) as e:
+            ctx.pressure_project(sc)
+        assert e.value.status == sfp.ERR_INPUT
+        # the context stays usable
+        sc2 = scenes.pressure_scene(12, seed=1)
+        vel, p, st = ctx.pressure_project(sc2)
+        assert st["converged"] == 1
+
+
+def test_call_order_errors():
+    sc = scenes.pressure_scene(8, seed=1)
+    with sfp.Context.for_scene(sc) as ctx:
+        with pytest.raises(sfp.SfpError) as e:
+            ctx.pressure_project(sc, reproject=True)
+        assert e.value.status == sfp.ERR_STATE
+        with pytest.raises(sfp.SfpError) as e:
+            ctx.vertex_pressures(np.zeros((1, 3)), np.ones((1, 3)), 1.0)
+        assert e.value.status == sfp.ERR_STATE
+        with pytest.raises(sfp.SfpError) as e:
+            ctx.set_params(cheby_degree=0)
+        assert e.value.status == sfp.ERR_ARGUMENT
+        with pytest.raises(sfp.SfpError) as e:
+            ctx.set_params(precond=sfp.PRECOND_LEARNED)
+        assert e.value.status == sfp.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (2, 2, 2), (3, 1, 5), (40, 7, 3), (5, 33, 2)])
+def test_tiny_and_anisotropic_magnetic_grids(oracle, shape):
+    print(f"[case] magnetic shape={shape}", flush=True)
+    geo = scenes.Geometry(shape, 1.0 / max(shape), center=(0.3, 0.5, 0.4))
+    X, Y, Z = geo.cell_centers()
+    phi = np.ascontiguousarray(np.broadcast_to(Y - 0.5 + 0.1 * np.sin(5 * X) + 0 * Z, geo.n)).copy()
+    sc = dict(geo=geo, phi=phi, chi=2.5, band=1.5 * geo.spacing,
+              hext=[np.full(geo.face_shape(a), (0.1, 1.0, 0.2)[a]) for a in range(3)])
+    with sfp.Context.for_scene(sc, tol=1e-11) as ctx:
+        pot, H, st = ctx.magnetic_solve(sc)
+    ref = oracle.magnetic_system(sc)
+    rel = oracle.rel_residual(ref["A"], ref["rhs"], pot.ravel())
+    assert rel <= 1.01e-11 or np.all(ref["rhs"] == 0)
+    H_ref = oracle.magnetic_total_field(sc, pot)
+    for a in range(3):
+        assert np.array_equal(H[a], H_ref[a])
+
+
+@pytest.mark.parametrize("shape", [(3, 4, 5), (33, 17, 9)])
+def test_small_anisotropic_pressure_grids(oracle, shape):
+    sc = scenes.random_pressure_scene(shape, seed=9)
+    with sfp.Context.for_scene(sc, tol=1e-11) as ctx:
+        vel, p, st = ctx.pressure_project(sc)
+    ref = oracle.pressure_system(sc)
+    if ref["n"]:
+        assert oracle.rel_residual(ref["A"], ref["rhs"], p.ravel()[ref["mat2grid"]]) <= 1.01e-11
+
+
+def test_jacobi_preconditioner_reaches_the_same_tolerance(oracle):
+    sc = scenes.pressure_scene(32, seed=5)
+    with sfp.Context.for_scene(sc, tol=1e-8, precond=sfp.PRECOND_JACOBI, max_iters=2000) as ctx:
+        vel, p, stj = ctx.pressure_project(sc)
+    with sfp.Context.for_scene(sc, tol=1e-8) as ctx:
+        vel2, p2, stm = ctx.pressure_project(sc)
+    ref = oracle.pressure_system(sc)
+    for pp in (p, p2):
+        assert oracle.rel_residual(ref["A"], ref["rhs"], pp.ravel()[ref["mat2grid"]]) <= 1.01e-8
+    print(f"  jacobi iters={stj['iterations']} mg iters={stm['iterations']}")
+    assert stm["iterations"] * 5 < stj["iterations"]
+
+
+def test_solves_are_deterministic():
+    sc = scenes.magnetic_scene(48, seed=3, chi=8.0)
+    with sfp.Context.for_scene(sc) as ctx:
+        pot1, H1, st1 = ctx.magnetic_solve(sc)
+        ctx.reset_warm_start()
+        pot2, H2, st2 = ctx.magnetic_solve(sc)
+    assert st1["iterations"] == st2["iterations"]
+    assert np.array_equal(pot1, pot2)
+
+
+def test_v_cycle_is_a_symmetric_positive_preconditioner():
+    # FCG tolerates nonsymmetric preconditioners; the V-cycle here (same
+    # Chebyshev polynomial before and after, R = P^T) is symmetric up to FP32
+    sc = scenes.random_pressure_scene(14, seed=2)
+    with sfp.Context.for_scene(sc) as ctx:
+        ctx.pressure_project(sc)
+        rng = np.random.default_rng(0)
+        liq = (sc["phi"] <= 0).astype(float)
+        u = rng.standard_normal(sc["geo"].n) * liq
+        v = rng.standard_normal(sc["geo"].n) * liq
+        Bu = ctx.precondition(0, u).astype(np.float64)
+        Bv = ctx.precondition(0, v).astype(np.float64)
+    lhs, rhs = np.vdot(v, Bu), np.vdot(u, Bv)
+    assert abs(lhs - rhs) <= 1e-4 * (abs(lhs) + abs(rhs))
+    assert np.vdot(u, Bu) > 0 and np.vdot(v, Bv) > 0

@@ -1,0 +1,129 @@
+"""CPU checks of the C ABI library and the host-side helpers (no GPU needed):
+libsfp.so loads and exports every function include/*.h declares, the ctypes
+structures match the C layouts, and the scene generators are deterministic."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import sfp
+from sfp import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def _declared(header):
+    text = open(os.path.join(INCLUDE, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sfp_\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header", sorted(sfp.EXPORTS))
+def test_library_exports_every_declared_function(header):
+    lib = sfp.lib()
+    names = _declared(header)
+    assert names, header
+    assert sorted(sfp.EXPORTS[header]) == names
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_headers_cover_all_exports():
+    out = subprocess.run(["nm", "-D", "--defined-only", sfp.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = sorted(set(re.findall(r" T (sfp_\w+)", out)))
+    declared = sorted(set(sum((_declared(h) for h in os.listdir(INCLUDE) if h.endswith(".h")), [])))
+    assert exported == declared
+
+
+def test_struct_layouts_match_c():
+    src = r'''
+#include <stddef.h>
+#include <stdio.h>
+#include "sfp.h"
+#include "sfp_profile.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(sfp_params), offsetof(sfp_params, max_iters),
+         offsetof(sfp_params, cheby_ratio), offsetof(sfp_params, stall_ratio), offsetof(sfp_params, verbose),
+         sizeof(sfp_stats), offsetof(sfp_stats, rel_residual), offsetof(sfp_stats, unknowns),
+         sizeof(sfp_kernel_profile), offsetof(sfp_kernel_profile, bytes));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "probe.c")
+        exe = os.path.join(d, "probe")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-std=c99", "-Wall", "-I", INCLUDE, c, "-o", exe], check=True)
+        vals = [int(v) for v in subprocess.run([exe], capture_output=True, text=True,
+                                               check=True).stdout.split()]
+    P, S, K = sfp.Params, sfp.Stats, sfp.KernelProfile
+    assert vals == [C.sizeof(P), P.max_iters.offset, P.cheby_ratio.offset, P.stall_ratio.offset,
+                    P.verbose.offset, C.sizeof(S), S.rel_residual.offset, S.unknowns.offset,
+                    C.sizeof(K), K.bytes.offset]
+
+
+def test_default_params_and_status_strings():
+    p = sfp.default_params()
+    assert p.tol == 1e-8 and p.max_iters == 100 and p.precond == sfp.PRECOND_MG
+    assert p.mg_coarsest == 4 and p.cheby_degree == 3
+    L = sfp.lib()
+    for st in range(7):
+        assert L.sfp_status_string(st)
+    assert L.sfp_status_string(99) == b"unknown status"
+
+
+def test_create_rejects_bad_arguments_before_touching_a_device():
+    L = sfp.lib()
+    h = C.c_void_p()
+    org = (C.c_double * 3)(0, 0, 0)
+    assert L.sfp_create(C.byref(h), 0, 4, 4, 0.1, C.cast(org, sfp._dp), 0) == sfp.ERR_ARGUMENT
+    assert L.sfp_create(C.byref(h), 4, 4, 4, -1.0, C.cast(org, sfp._dp), 0) == sfp.ERR_ARGUMENT
+    assert L.sfp_create(C.byref(h), 2048, 2048, 1024, 0.1, C.cast(org, sfp._dp), 0) == sfp.ERR_ARGUMENT
+    assert b"int" in L.sfp_last_error(None)
+    assert L.sfp_set_params(None, None) == sfp.ERR_ARGUMENT
+    assert L.sfp_pressure_project(None, None, None, None, None, None, 0.0, None, None) == sfp.ERR_ARGUMENT
+
+
+def test_create_without_gpu_fails_loudly():
+    import torch
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(sfp.SfpError) as e:
+        sfp.Context(8, 1 / 8)
+    assert e.value.status == sfp.ERR_NO_DEVICE
+
+
+def test_splitmix64_known_answers():
+    # SplitMix64 (Steele, Lea, Flood 2014) from state 0 and 1
+    got = scenes.splitmix64(np.array([0, 1], dtype=np.uint64))
+    assert int(got[0]) == 0xE220A8397B1DCDAF
+    assert int(got[1]) == 0x910A2DEC89025CC1
+
+
+def test_scenes_are_deterministic_and_counter_based():
+    a = scenes.uniform(3, 5, 100)
+    b = scenes.uniform(3, 5, 60, start=40)
+    assert np.array_equal(a[40:], b)
+    assert not np.array_equal(a, scenes.uniform(4, 5, 100))
+    assert np.all((a >= 0) & (a < 1))
+    s1 = scenes.pressure_scene(12, seed=1)
+    s2 = scenes.pressure_scene(12, seed=1)
+    for k in range(3):
+        assert np.array_equal(s1["velocity"][k], s2["velocity"][k])
+    liquid = (s1["phi"] <= 0).mean()
+    assert 0.25 < liquid < 0.40
+
+
+def test_geometry_matches_staggered_grid_construction():
+    g = scenes.Geometry((20, 18, 22), 1.0 / 22, center=(0.41, 0.52, 0.47))
+    for a, (c, r) in enumerate(zip((0.41, 0.52, 0.47), (20, 18, 22))):
+        assert g.origin[a] == c - (float(r) * (1.0 / 22)) / 2
+    X, Y, Z = g.cell_centers()
+    assert X[0, 0, 0] == g.origin[0] + (1.0 / 22) / 2
+    assert g.face_shape(1) == (20, 19, 22)
