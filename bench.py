@@ -49,14 +49,128 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
     ap.add_argument("--systems", default="p,m0.5,m2,m8")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (counter calibration)")
+    ap.add_argument("--summarize-profiles", metavar="DIR",
+                    help="turn the rocprofv3 CSVs under DIR into the profiles/ summaries")
     return ap.parse_args()
+
+
+SMOOTH_KERNEL = "k_smooth<6>"  # SM_READ_D | SM_WRITE_D
+
+
+def summarize_profiles(d, n=256):
+    """Writes profiles/k_smooth_256cube_trace.md, profiles/traffic_smooth_256.json and
+    profiles/traffic_smooth_256cube.md from the CSVs of the measurement commands."""
+    import csv
+    import statistics
+
+    def rows(path):
+        with open(path) as f:
+            return list(csv.DictReader(f))
+
+    out_dir = os.path.join(ROOT, "profiles")
+    os.makedirs(out_dir, exist_ok=True)
+    trace = rows(os.path.join(d, "prof_trace", "trace_kernel_trace.csv"))
+    sm = [r for r in trace if SMOOTH_KERNEL in r["Kernel_Name"]]
+    grid = max(int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) for r in sm)
+    fine = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3 for r in sm
+            if int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) == grid]
+    stats = rows(os.path.join(d, "prof_trace", "trace_kernel_stats.csv"))
+    bench = json.load(open(os.path.join(d, "bench_full.json")))
+    cells = n ** 3
+    alg = 36.0 * cells
+    avg = statistics.mean(fine)
+    lines = [f"# k_smooth<READ_D|WRITE_D> at {n}^3: rocprofv3 kernel trace", "",
+             "Command (GPU box, repo root):", "",
+             "    rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_trace -o trace -- \\",
+             "        python3 bench.py --no-cpu-baseline --no-profile --steps 4 --warmup 0 --systems p,m2", "",
+             f"Fine-level dispatches of `{SMOOTH_KERNEL}` (grid {grid} work-items): {len(fine)}",
+             f"average duration {avg:.2f} us (min {min(fine):.2f}, max {max(fine):.2f});",
+             f"algorithmic bytes per launch 36 B x {cells} cells = {alg / 1e6:.1f} MB "
+             f"-> {alg / (avg * 1e-6) / 1e9:.0f} GB/s = {alg / (avg * 1e-6) / 8e12:.3f} of 8 TB/s.",
+             f"bench.py's HIP-event average over its timed steps: "
+             f"{bench['roofline']['avg_launch_us']:.2f} us.", "",
+             "Top kernels by total time (all levels; from trace_kernel_stats.csv):", "",
+             "| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
+    for r in stats[:14]:
+        name = r["Name"].replace("sfp::(anonymous namespace)::", "").split("(")[0]
+        lines.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                     f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    open(os.path.join(out_dir, f"k_smooth_{n}cube_trace.md"), "w").write("\n".join(lines) + "\n")
+
+    def counter(sub, name):
+        rs = rows(os.path.join(d, sub, [f for f in os.listdir(os.path.join(d, sub))
+                                        if f.endswith("counter_collection.csv")][0]))
+        return rs
+
+    fetch = [float(r["Counter_Value"]) for r in counter("prof_fetch", "FETCH_SIZE")
+             if SMOOTH_KERNEL in r["Kernel_Name"] and int(r["Grid_Size"]) == grid * 1]
+    write = [float(r["Counter_Value"]) for r in counter("prof_write", "WRITE_SIZE")
+             if SMOOTH_KERNEL in r["Kernel_Name"] and int(r["Grid_Size"]) == grid * 1]
+    cf = [float(r["Counter_Value"]) for r in counter("prof_copy_fetch", "FETCH_SIZE")
+          if "copyBuffer" in r["Kernel_Name"]]
+    cw = [float(r["Counter_Value"]) for r in counter("prof_copy_write", "WRITE_SIZE")
+          if "copyBuffer" in r["Kernel_Name"]]
+    copy_kib = (1 << 30) / 1024.0
+    fs, ws = copy_kib / statistics.mean(cf), copy_kib / statistics.mean(cw)
+    rd = statistics.mean(fetch) * 1024 * fs
+    wr = statistics.mean(write) * 1024 * ws
+    res = {"kernel": SMOOTH_KERNEL, "cells": cells, "grid_work_items": grid,
+           "fetch_kib_avg": statistics.mean(fetch), "write_kib_avg": statistics.mean(write),
+           "fetch_scale": fs, "write_scale": ws, "hbm_bytes_read": rd, "hbm_bytes_written": wr,
+           "hbm_bytes_per_launch": rd + wr, "algorithmic_bytes_per_launch": alg,
+           "traffic_over_algorithmic": (rd + wr) / alg}
+    json.dump(res, open(os.path.join(out_dir, f"traffic_smooth_{n}.json"), "w"), indent=1)
+    md = [f"# k_smooth<READ_D|WRITE_D> at {n}^3: HBM traffic from FETCH_SIZE / WRITE_SIZE", "",
+          "Commands (GPU box, repo root), one counter per pass:", "",
+          "    rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/prof_fetch -o fetch -- \\",
+          "        python3 bench.py --no-cpu-baseline --no-profile --steps 2 --warmup 0 --systems p,m2",
+          "    rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/prof_write -o write -- \\",
+          "        python3 bench.py --no-cpu-baseline --no-profile --steps 2 --warmup 0 --systems p,m2",
+          "    rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/prof_copy_fetch -o copyf -- python3 bench.py --copy-probe",
+          "    rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/prof_copy_write -o copyw -- python3 bench.py --copy-probe",
+          "    python3 bench.py --summarize-profiles gpurun_out", "",
+          "Calibration: a plain 1 GiB device copy (`__amd_rocclr_copyBuffer`) reads "
+          f"{statistics.mean(cf):.0f} KiB by FETCH_SIZE and writes {statistics.mean(cw):.0f} KiB by "
+          f"WRITE_SIZE for {copy_kib:.0f} KiB each way: scale factors {fs:.4f} (fetch) and {ws:.4f} (write).", "",
+          f"| quantity | per launch |", "|---|---:|",
+          f"| FETCH_SIZE (KiB, mean of {len(fetch)} fine-level dispatches) | {statistics.mean(fetch):.0f} |",
+          f"| WRITE_SIZE (KiB, mean of {len(write)}) | {statistics.mean(write):.0f} |",
+          f"| bytes read (scaled) | {rd / 1e6:.1f} MB |",
+          f"| bytes written (scaled) | {wr / 1e6:.1f} MB |",
+          f"| algorithmic bytes (36 B/cell) | {alg / 1e6:.1f} MB |",
+          f"| traffic / algorithmic | {(rd + wr) / alg:.3f} |"]
+    open(os.path.join(out_dir, f"traffic_smooth_{n}cube.md"), "w").write("\n".join(md) + "\n")
+    print(json.dumps(res))
+
+
+def copy_probe():
+    """A plain device-to-device copy of 2^27 doubles (1 GiB read, 1 GiB
+    written), run 5 times: the known-size reference for FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                      "launches": 5}), flush=True)
 
 
 def main():
     args = parse()
+    if args.summarize_profiles:
+        summarize_profiles(args.summarize_profiles)
+        return
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU: the solver has no "
-                                   "multi-GPU path (see DESIGN.md for how it would shard)",
+        print(json.dumps({"error": "the solver runs on exactly one GPU; a multi-GPU "
+                                   "(x-slab) decomposition is not built",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
 
@@ -158,7 +272,7 @@ def main():
             if os.path.exists(tf):
                 with open(tf) as f:
                     traffic = json.load(f).get("hbm_bytes_per_launch")
-            roof = {"kernel": "k_smooth (fine-level FP32 Chebyshev step, 7-point stencil)",
+            roof = {"kernel": "k_smooth<READ_D|WRITE_D> (fine-level FP32 Chebyshev step, 7-point stencil, 36 B/cell)",
                     "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic,

@@ -131,7 +131,7 @@ struct KrylovScalars {
 
 // Per-kernel timing collected with HIP events on the context stream.
 enum KernelId {
-    K_SMOOTH = 0,   // fine-level Chebyshev step (FP32)
+    K_SMOOTH = 0,   // fine-level Chebyshev step reading and writing the direction (FP32)
     K_MATVEC64,     // FCG matvec + dots (FP64)
     K_UPDATE64,     // FCG vector update (FP64)
     K_RESTRICT,     // fine residual + restriction

@@ -156,18 +156,10 @@ float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) 
     float *cur = L.x0.p, *oth = L.x1.p;
     const int deg = cb.degree;
     auto smooth = [&](int mode, float cd, float cr, const double *r, const float *e) {
-        const bool track = fine && ws.profiling;
-        if (track) {
-            // algorithmic bytes: b (or r64 + b), diag, 3 couplings, x' + optional
-            // x, d, d', coarse e
-            double per = 4.0 + 4.0 + 12.0 + 4.0;
-            if (!(mode & SM_FIRST)) per += 4.0;
-            if (r) per += 8.0;
-            if (mode & SM_READ_D) per += 4.0;
-            if (mode & SM_WRITE_D) per += 4.0;
-            if (mode & SM_PROLONG) per += 0.5;
-            ws.timers[K_SMOOTH].begin(st, per * cells);
-        }
+        // timed: the fine-level step that reads and writes the direction,
+        // 36 algorithmic bytes per cell (x, b, d, diag, 3 couplings, x', d')
+        const bool track = fine && ws.profiling && mode == (SM_READ_D | SM_WRITE_D);
+        if (track) ws.timers[K_SMOOTH].begin(st, 36.0 * cells);
         launch_smooth(st, mode, L.d, A, r, L.b.p, cur, oth, L.dv.p, cb.cr[0], cd, cr, e);
         if (track) ws.timers[K_SMOOTH].end(st);
         std::swap(cur, oth);
