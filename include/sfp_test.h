@@ -29,6 +29,11 @@ sfp_status sfp_test_precondition(sfp_context ctx, int which, const double *r, fl
 /* r = rhs - A x with the fine FP64 operator, plus ||r||^2 and ||rhs||^2. */
 sfp_status sfp_test_residual(sfp_context ctx, int which, const double *x, double *r,
                              double *rr, double *bb);
+/* Solve A x = rhs for the assembled system with the current parameters from
+ * x = 0 (host arrays); the system's own right-hand side and solution (the warm
+ * start) are restored afterwards. stats may be NULL. */
+sfp_status sfp_test_solve(sfp_context ctx, int which, const double *rhs, double *x,
+                          sfp_stats *stats);
 
 #ifdef __cplusplus
 }

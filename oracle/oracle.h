@@ -93,6 +93,17 @@ int orc_bicgstab(int n, const int *rowptr, const int *col, const double *val, co
                  double *x, double tol, int maxit, int precond, double *rel, double *setup_s,
                  double *solve_s);
 
+/* ---- U-Net of the learned preconditioner (unet.cpp) ----
+ * Volumes are channels-last [batch][x][y][z][c]; input has 4 channels,
+ * output 1; params as laid out in unet.cpp. Returns the masked MSE loss
+ * against target (0 when target is NULL). */
+long long orc_unet_param_count(void);
+double orc_unet_forward(int batch, int nx, int ny, int nz, const double *params, const double *input,
+                        const double *target, const double *mask, double *output);
+/* Forward + backward of the masked MSE loss; grad has param_count entries. */
+double orc_unet_gradients(int batch, int nx, int ny, int nz, const double *params, const double *input,
+                          const double *target, const double *mask, double *grad);
+
 /* Number of OpenMP threads the solvers use. */
 int orc_num_threads(void);
 

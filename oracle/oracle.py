@@ -59,6 +59,12 @@ def lib():
             f.restype = C.c_int
             f.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int,
                           _dp, _dp, _dp]
+        L.orc_unet_param_count.restype = C.c_longlong
+        L.orc_unet_param_count.argtypes = []
+        for name in ("orc_unet_forward", "orc_unet_gradients"):
+            f = getattr(L, name)
+            f.restype = C.c_double
+            f.argtypes = [C.c_int] * 4 + [_dp, _dp, C.c_void_p, C.c_void_p, _dp]
         L.orc_num_threads.restype = C.c_int
         _LIB = L
     return _LIB
@@ -255,3 +261,35 @@ def solve(A, b, method="cg", precond="amg", tol=1e-12, maxit=1000, x0=None):
     it = f(n, _i(ptr), _i(col), _d(val), _d(b), _d(x), tol, maxit, PRECOND[precond],
            C.byref(rel), C.byref(ts), C.byref(tv))
     return x, it, rel.value, ts.value, tv.value
+
+
+# ---------------------------------------------------------------- U-Net
+def unet_param_count():
+    return int(lib().orc_unet_param_count())
+
+
+def _opt(a):
+    return None if a is None else C.cast(_d(np.ascontiguousarray(a, dtype=np.float64)), C.c_void_p)
+
+
+def unet_forward(params, inp, target=None, mask=None):
+    """inp: [B, X, Y, Z, 4]; returns (output [B, X, Y, Z], loss)."""
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    B, X, Y, Z, _ = inp.shape
+    out = np.zeros((B, X, Y, Z))
+    t = None if target is None else np.ascontiguousarray(target, dtype=np.float64)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float64)
+    loss = lib().orc_unet_forward(B, X, Y, Z, _d(np.ascontiguousarray(params, np.float64)), _d(inp),
+                                  _opt(t), _opt(m), _d(out))
+    return out, loss
+
+
+def unet_gradients(params, inp, target, mask=None):
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    B, X, Y, Z, _ = inp.shape
+    g = np.zeros(unet_param_count())
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float64)
+    loss = lib().orc_unet_gradients(B, X, Y, Z, _d(np.ascontiguousarray(params, np.float64)), _d(inp),
+                                    C.cast(_d(t), C.c_void_p), _opt(m), _d(g))
+    return loss, g

@@ -135,6 +135,8 @@ void pressure_dev(sfp_context ctx, double *const vel[3], const double *phi,
         launch_pressure_rhs(s, d, ctx->liquid.p, F, V, U, vol_error, sys.rhs().p);
     }
     SFP_HIP(hipEventRecord(ctx->ev[1], s));
+    ctx->ws.phi = phi;
+    ctx->ws.spacing = ctx->spacing;
     SolveInfo info = sys.solve(ctx->ws, ctx->cfg, reproject);
     SFP_HIP(hipEventRecord(ctx->ev[2], s));
     MutFaces Vm{{vel[0], vel[1], vel[2]}};
@@ -175,6 +177,8 @@ void magnetic_dev(sfp_context ctx, const double *phi, const double *const hext[3
                              c32);
     sys.build_hierarchy(ctx->ws, ctx->cfg);
     SFP_HIP(hipEventRecord(ctx->ev[1], s));
+    ctx->ws.phi = phi;
+    ctx->ws.spacing = ctx->spacing;
     SolveInfo info = sys.solve(ctx->ws, ctx->cfg, ctx->magnetic_warm);
     ctx->magnetic_warm = true;
     SFP_HIP(hipEventRecord(ctx->ev[2], s));
@@ -329,8 +333,8 @@ sfp_status sfp_set_params(sfp_context ctx, const sfp_params *p) {
         ctx->last_error = "parameter out of range";
         return SFP_ERR_ARGUMENT;
     }
-    if (p->precond == SFP_PRECOND_LEARNED) {
-        ctx->last_error = "learned preconditioner needs a loaded network";
+    if (p->precond == SFP_PRECOND_LEARNED && !ctx->net) {
+        ctx->last_error = "learned preconditioner needs a network (sfp_set_network)";
         return SFP_ERR_UNSUPPORTED;
     }
     SolverConfig &c = ctx->cfg;

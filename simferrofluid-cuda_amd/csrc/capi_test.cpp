@@ -156,4 +156,34 @@ sfp_status sfp_test_residual(sfp_context ctx, int which, const double *x, double
     });
 }
 
+sfp_status sfp_test_solve(sfp_context ctx, int which, const double *rhs, double *x,
+                          sfp_stats *stats) {
+    return run(ctx, [&] {
+        System &s = pick(ctx, which);
+        if (!rhs || !x) throw ArgError("null vector");
+        const size_t n = s.dims().cells();
+        const hipStream_t st = ctx->ws.stream;
+        DevBuf<double> keep_b, keep_x;
+        keep_b.ensure(n);
+        keep_x.ensure(n);
+        SFP_HIP(hipMemcpyAsync(keep_b.p, s.rhs().p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        SFP_HIP(hipMemcpyAsync(keep_x.p, s.x().p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        SFP_HIP(hipMemcpyAsync(s.rhs().p, rhs, n * sizeof(double), hipMemcpyHostToDevice, st));
+        const SolveInfo info = s.solve(ctx->ws, ctx->cfg, false);
+        d2h(ctx, x, s.x().p, n);
+        SFP_HIP(hipMemcpyAsync(s.rhs().p, keep_b.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        SFP_HIP(hipMemcpyAsync(s.x().p, keep_x.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        SFP_HIP(hipStreamSynchronize(st));
+        if (stats) {
+            *stats = sfp_stats{};
+            stats->iterations = info.iterations;
+            stats->converged = info.converged ? 1 : 0;
+            stats->fallback = info.fallback;
+            stats->precond_used = info.precond_used;
+            stats->rel_residual = info.rel;
+            stats->unknowns = (long long)n;
+        }
+    });
+}
+
 }  // extern "C"

@@ -4,6 +4,7 @@
 #include "../../include/sfp_profile.h"
 #include "solver.hpp"
 
+#include <memory>
 #include <string>
 
 struct sfp_context_s {
@@ -28,5 +29,6 @@ struct sfp_context_s {
     sfp::DevBuf<double> s_face[5][3];   // velocity, fraction, solid velocity, jump, field
     sfp::DevBuf<double> s_vert[3];      // positions, normals, pressures
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::unique_ptr<sfp::UNet> net;  // learned preconditioner weights, when given
     std::string last_error;
 };

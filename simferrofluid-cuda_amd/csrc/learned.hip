@@ -1,0 +1,132 @@
+// Glue between the FP64 Krylov solver and the U-Net: operator features,
+// residual scaling and the masked network output.
+#include "device_util.hpp"
+#include "kernels.hpp"
+#include "learned.hpp"
+
+namespace sfp {
+namespace {
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    const unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// Channels 1..3 of the network input (and zeros in 0, 4..7):
+// log(diag/6), (diag - sum of the six couplings)/diag, clamp(phi/dx,-4,4)/4.
+__global__ __launch_bounds__(kBlock) void k_features(CellGrid cg, Dims d, Op64 A, const float *diag32,
+                                                     const double *phi, double inv_dx,
+                                                     uint16_t *__restrict__ x8,
+                                                     float *__restrict__ f3) {
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
+    if (k >= d.nz || j >= d.ny) return;
+    const Strides s(d);
+    for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
+        const size_t c = d.idx(i, j, k);
+        float f[3] = {0.f, 0.f, 0.f};
+        if (diag32[c] > 0.f) {
+            const double dg = A.diag[c];
+            double off = A.cpl[0][c] + A.cpl[1][c] + A.cpl[2][c];
+            if (i + 1 < d.nx) off += A.cpl[0][c + s.sx];
+            if (j + 1 < d.ny) off += A.cpl[1][c + s.sy];
+            if (k + 1 < d.nz) off += A.cpl[2][c + 1];
+            f[0] = (float)log(dg / 6.0);
+            f[1] = (float)((dg - off) / dg);
+            double p = phi ? phi[c] * inv_dx : 0.0;
+            p = p < -4.0 ? -4.0 : (p > 4.0 ? 4.0 : p);
+            f[2] = (float)(p * 0.25);
+        }
+        if (x8) {
+            uint16_t *o = x8 + c * 8;
+            o[0] = 0;
+            o[1] = to_bf16(f[0]);
+            o[2] = to_bf16(f[1]);
+            o[3] = to_bf16(f[2]);
+            o[4] = o[5] = o[6] = o[7] = 0;
+        }
+        if (f3) {
+            f3[c * 3] = f[0];
+            f3[c * 3 + 1] = f[1];
+            f3[c * 3 + 2] = f[2];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rms_partials(CellGrid cg, Dims d, const double *r,
+                                                         const float *diag32, double *partials) {
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
+    double v[2] = {0.0, 0.0};
+    if (k < d.nz && j < d.ny) {
+        for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
+            const size_t c = d.idx(i, j, k);
+            if (diag32[c] > 0.f) {
+                v[0] += r[c] * r[c];
+                v[1] += 1.0;
+            }
+        }
+    }
+    block_partials<2>(v, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_rms_finalize(unsigned nb, const double *partials,
+                                                         KrylovScalars *sc) {
+    double v[2];
+    sum_partials<2>(partials, nb, v);
+    if (flat_tid() == 0) sc->pad[1] = (v[0] > 0.0 && v[1] > 0.0) ? 1.0 / sqrt(v[0] / v[1]) : 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_residual(CellGrid cg, Dims d, const double *r,
+                                                          const float *diag32, const KrylovScalars *sc,
+                                                          uint16_t *__restrict__ x8) {
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
+    if (k >= d.nz || j >= d.ny) return;
+    const double s = sc->pad[1];
+    for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
+        const size_t c = d.idx(i, j, k);
+        x8[c * 8] = diag32[c] > 0.f ? to_bf16((float)(r[c] * s)) : (uint16_t)0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mask_output(CellGrid cg, Dims d, const float *zin,
+                                                        const float *diag32, float *__restrict__ z) {
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
+    if (k >= d.nz || j >= d.ny) return;
+    for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
+        const size_t c = d.idx(i, j, k);
+        z[c] = diag32[c] > 0.f ? zin[c] : 0.f;
+    }
+}
+
+}  // namespace
+
+void launch_learned_features(hipStream_t st, Dims d, Op64 A, const float *diag32, const double *phi,
+                             double spacing, uint16_t *x8, float *f3) {
+    Launch L = cell_launch(d);
+    hipLaunchKernelGGL(k_features, L.grid, L.block, 0, st, L.cg, d, A, diag32, phi, 1.0 / spacing, x8, f3);
+    SFP_LAUNCH_CHECK();
+}
+
+void launch_pack_residual(hipStream_t st, Dims d, const double *r, const float *diag32,
+                          double *partials, KrylovScalars *sc, uint16_t *x8) {
+    Launch L = cell_launch(d, kReducePlanes);
+    hipLaunchKernelGGL(k_rms_partials, L.grid, L.block, 0, st, L.cg, d, r, diag32, partials);
+    SFP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_rms_finalize, dim3(1), dim3(kBlock), 0, st, L.blocks(), partials, sc);
+    SFP_LAUNCH_CHECK();
+    Launch P = cell_launch(d);
+    hipLaunchKernelGGL(k_pack_residual, P.grid, P.block, 0, st, P.cg, d, r, diag32, sc, x8);
+    SFP_LAUNCH_CHECK();
+}
+
+void launch_mask_output(hipStream_t st, Dims d, const float *zin, const float *diag32, float *z) {
+    Launch L = cell_launch(d);
+    hipLaunchKernelGGL(k_mask_output, L.grid, L.block, 0, st, L.cg, d, zin, diag32, z);
+    SFP_LAUNCH_CHECK();
+}
+
+}  // namespace sfp

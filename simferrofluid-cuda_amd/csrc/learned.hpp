@@ -1,0 +1,19 @@
+// Launchers connecting the Krylov solver and the U-Net (learned.hip).
+#pragma once
+#include "sfp_internal.hpp"
+
+#include <cstdint>
+
+namespace sfp {
+
+struct KrylovScalars;
+// network input channels 1..3 into x8 ([N][8] BF16, others zeroed) and/or f3 ([N][3] FP32)
+void launch_learned_features(hipStream_t st, Dims d, Op64 A, const float *diag32, const double *phi,
+                             double spacing, uint16_t *x8, float *f3);
+// x8[c][0] = r / RMS(r over active cells)
+void launch_pack_residual(hipStream_t st, Dims d, const double *r, const float *diag32,
+                          double *partials, KrylovScalars *sc, uint16_t *x8);
+// z = zin on active cells, 0 elsewhere
+void launch_mask_output(hipStream_t st, Dims d, const float *zin, const float *diag32, float *z);
+
+}  // namespace sfp

@@ -1,6 +1,8 @@
 // Hierarchy setup, V-cycle orchestration and the FP64 flexible CG.
 #include "solver.hpp"
 
+#include "learned.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -191,6 +193,25 @@ float *System::precondition(Workspace &ws, const SolverConfig &cfg, const double
     return vlevel(ws, cb, 0, r64);
 }
 
+bool System::learned_ok(const Workspace &ws) const {
+    return ws.net && d_.nx % 4 == 0 && d_.ny % 4 == 0 && d_.nz % 4 == 0;
+}
+
+void System::prepare_learned(Workspace &ws) {
+    ws.net->ensure(UNetShape{1, d_.nx, d_.ny, d_.nz}, false);
+    launch_learned_features(ws.stream, d_, lv_[0].op64(), lv_[0].diag32.p, ws.phi, ws.spacing,
+                            ws.net->input_buffer(), nullptr);
+}
+
+float *System::precondition_learned(Workspace &ws, const double *r64) {
+    UNet &net = *ws.net;
+    launch_pack_residual(ws.stream, d_, r64, lv_[0].diag32.p, ws.partials.p, ws.scalars.p,
+                         net.input_buffer());
+    net.forward(ws.stream, net.input_buffer(), net.output_buffer());
+    launch_mask_output(ws.stream, d_, net.output_buffer(), lv_[0].diag32.p, lv_[0].x0.p);
+    return lv_[0].x0.p;
+}
+
 long long System::count_active(Workspace &ws) {
     launch_count_active(ws.stream, d_, lv_[0].diag32.p, ws.partials.p, ws.scalars.p);
     ws.read_scalars();
@@ -218,6 +239,11 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
     bool first = true;
     bool checked = true;  // rel is a true residual
     const double cells = (double)n;
+    // learned preconditioner until an iteration reduces the residual by less
+    // than stall_ratio; then the multigrid V-cycle
+    bool learned = cfg.precond == 1 && learned_ok(ws);
+    if (learned) prepare_learned(ws);
+    info.precond_used = learned ? 1 : (cfg.precond == 1 ? 0 : cfg.precond);
     while (true) {
         if (rel <= cfg.tol) {
             if (checked) break;
@@ -232,7 +258,7 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
             continue;
         }
         if (info.iterations >= cfg.max_iters) break;
-        const float *z = precondition(ws, cfg, r_.p);
+        const float *z = learned ? precondition_learned(ws, r_.p) : precondition(ws, cfg, r_.p);
         if (ws.profiling) ws.timers[K_MATVEC64].begin(st, (4.0 + 32.0 + 8.0 + 8.0 + 8.0) * cells);
         launch_fcg_matvec(st, d_, A, z, r_.p, q_.p, w_.p, first ? 1 : 0, ws.partials.p,
                           ws.scalars.p);
@@ -243,9 +269,12 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
         if (ws.profiling) ws.timers[K_UPDATE64].end(st);
         first = false;
         info.iterations++;
+        if (!learned && cfg.precond == 1 && info.precond_used == 1) info.fallback++;
         ws.read_scalars();
+        const double prev = rel;
         rel = std::sqrt(ws.host_sc->rr / bb);
         checked = false;
+        if (learned && rel > cfg.stall_ratio * prev) learned = false;  // fall back to multigrid
     }
     if (!checked) {
         launch_residual64(st, d_, A, rhs_.p, x_.p, r_.p, ws.partials.p, ws.scalars.p);
@@ -254,6 +283,7 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
     }
     info.rel = rel;
     info.converged = rel <= cfg.tol;
+    if (info.fallback > 0) info.precond_used = 0;
     if (ws.profiling)
         for (auto &t : ws.timers) t.harvest();
     return info;

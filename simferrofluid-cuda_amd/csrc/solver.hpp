@@ -4,6 +4,7 @@
 #pragma once
 #include "kernels.hpp"
 #include "sfp_internal.hpp"
+#include "unet.hpp"
 
 #include <array>
 #include <vector>
@@ -53,6 +54,11 @@ struct Workspace {
     int *host_err = nullptr;           // pinned
     bool profiling = false;
     std::array<KernelTimer, K_COUNT> timers;
+    // learned preconditioner: the context's network and the level set and
+    // spacing of the current solve (for the operator features)
+    UNet *net = nullptr;
+    const double *phi = nullptr;
+    double spacing = 1.0;
     void read_scalars();               // synchronous copy of *scalars to host_sc
 };
 
@@ -78,6 +84,11 @@ class System {
     void build_hierarchy(Workspace &ws, const SolverConfig &cfg);
     // One preconditioner application z = B r (FP32 result, fine grid).
     float *precondition(Workspace &ws, const SolverConfig &cfg, const double *r64);
+    // The learned preconditioner can run on this grid (each axis a multiple of 4).
+    bool learned_ok(const Workspace &ws) const;
+    // Network input features of the current operator (into the network input).
+    void prepare_learned(Workspace &ws);
+    float *precondition_learned(Workspace &ws, const double *r64);
     // Flexible CG on A x = rhs; x holds the start when warm.
     SolveInfo solve(Workspace &ws, const SolverConfig &cfg, bool warm);
     // Number of active rows (sets ws.scalars->pad[0]); synchronous.

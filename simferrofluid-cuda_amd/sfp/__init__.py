@@ -36,7 +36,11 @@ EXPORTS = {
     "sfp_profile.h": ["sfp_profile_enable", "sfp_profile_query"],
     "sfp_test.h": ["sfp_test_num_levels", "sfp_test_level_dims", "sfp_test_export_level",
                    "sfp_test_export_level32", "sfp_test_apply64", "sfp_test_precondition",
-                   "sfp_test_residual"],
+                   "sfp_test_residual", "sfp_test_solve"],
+    "sfp_learned.h": ["sfp_unet_param_count", "sfp_set_network", "sfp_learned_features",
+                      "sfp_unet_create", "sfp_unet_destroy", "sfp_unet_set_params",
+                      "sfp_unet_get_params", "sfp_unet_init_params", "sfp_unet_forward",
+                      "sfp_unet_gradients", "sfp_unet_train_step_device", "sfp_unet_last_error"],
 }
 
 
@@ -124,6 +128,32 @@ def lib():
         L.sfp_test_precondition.argtypes = [vp, C.c_int, _dp, _fp]
         L.sfp_test_residual.restype = C.c_int
         L.sfp_test_residual.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp]
+        L.sfp_test_solve.restype = C.c_int
+        L.sfp_test_solve.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(Stats)]
+        L.sfp_unet_param_count.restype = C.c_longlong
+        L.sfp_unet_param_count.argtypes = []
+        L.sfp_set_network.restype = C.c_int
+        L.sfp_set_network.argtypes = [vp, _fp, C.c_longlong]
+        L.sfp_learned_features.restype = C.c_int
+        L.sfp_learned_features.argtypes = [vp, C.c_int, _dp, _fp]
+        L.sfp_unet_create.restype = C.c_int
+        L.sfp_unet_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_int]
+        L.sfp_unet_destroy.restype = None
+        L.sfp_unet_destroy.argtypes = [vp]
+        for name in ("sfp_unet_set_params", "sfp_unet_get_params"):
+            getattr(L, name).restype = C.c_int
+            getattr(L, name).argtypes = [vp, _fp, C.c_longlong]
+        L.sfp_unet_init_params.restype = C.c_int
+        L.sfp_unet_init_params.argtypes = [vp, C.c_ulonglong]
+        L.sfp_unet_forward.restype = C.c_int
+        L.sfp_unet_forward.argtypes = [vp, _fp, _fp]
+        L.sfp_unet_gradients.restype = C.c_int
+        L.sfp_unet_gradients.argtypes = [vp, _fp, _fp, _fp, _dp, _fp]
+        L.sfp_unet_train_step_device.restype = C.c_int
+        L.sfp_unet_train_step_device.argtypes = [vp, _fp, _fp, _fp, C.c_float, _dp]
+        L.sfp_unet_last_error.restype = C.c_char_p
+        L.sfp_unet_last_error.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -138,8 +168,12 @@ def _d(a):
 
 
 def _f(a):
-    assert a.dtype == np.float32 and a.flags.c_contiguous
-    return a.ctypes.data_as(_fp)
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.dtype == np.float32 and a.flags.c_contiguous, "need C-contiguous float32"
+        return a.ctypes.data_as(_fp)
+    return C.cast(C.c_void_p(int(a)), _fp)  # device pointer (int)
 
 
 def _p3(arrs):
@@ -277,6 +311,20 @@ class Context:
             C.byref(st)))
         return st.as_dict()
 
+    # ------------------------------------------------------------ learned preconditioner
+    def set_network(self, params):
+        """Load U-Net weights (float32, unet_param_count() values); afterwards
+        precond=PRECOND_LEARNED may be selected."""
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(lib().sfp_set_network(self._h, _f(params), params.size))
+
+    def learned_features(self, which, phi=None):
+        """Network input channels 1..3 of the last assembled system, [nx,ny,nz,3]."""
+        out = np.zeros(self.n + (3,), np.float32)
+        ph = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+        self._check(lib().sfp_learned_features(self._h, int(which), _d(ph), _f(out)))
+        return out
+
     # ------------------------------------------------------------ profiling
     def profile(self, enable=True):
         self._check(lib().sfp_profile_enable(self._h, 1 if enable else 0))
@@ -329,3 +377,85 @@ class Context:
         bb = C.c_double(0)
         self._check(lib().sfp_test_residual(self._h, which, _d(x), _d(r), C.byref(rr), C.byref(bb)))
         return r, rr.value, bb.value
+
+    def solve_rhs(self, which, rhs):
+        """x = A^-1 rhs for the assembled system (from x = 0); returns (x, stats)."""
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros(rhs.shape)
+        st = Stats()
+        self._check(lib().sfp_test_solve(self._h, int(which), _d(rhs), _d(x), C.byref(st)))
+        return x, st.as_dict()
+
+
+def unet_param_count() -> int:
+    return int(lib().sfp_unet_param_count())
+
+
+class UNet:
+    """The stand-alone U-Net (sfp_unet_*): forward, gradients and Adam steps on
+    the GPU for volumes of shape (batch, nx, ny, nz), each axis a multiple of 4."""
+
+    def __init__(self, batch, nx, ny, nz, training=True, device=0):
+        self.shape = (int(batch), int(nx), int(ny), int(nz))
+        self._h = C.c_void_p()
+        st = lib().sfp_unet_create(C.byref(self._h), int(device), *self.shape, 1 if training else 0)
+        if st != OK:
+            raise SfpError(st, lib().sfp_unet_last_error(None).decode())
+
+    def close(self):
+        if self._h:
+            lib().sfp_unet_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st):
+        if st != OK:
+            raise SfpError(st, lib().sfp_unet_last_error(self._h).decode())
+
+    def set_params(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(lib().sfp_unet_set_params(self._h, _f(params), params.size))
+
+    def get_params(self):
+        p = np.zeros(unet_param_count(), np.float32)
+        self._check(lib().sfp_unet_get_params(self._h, _f(p), p.size))
+        return p
+
+    def init_params(self, seed):
+        self._check(lib().sfp_unet_init_params(self._h, int(seed)))
+
+    def forward(self, inp):
+        inp = np.ascontiguousarray(inp, dtype=np.float32)
+        assert inp.shape == self.shape + (4,)
+        out = np.zeros(self.shape, np.float32)
+        self._check(lib().sfp_unet_forward(self._h, _f(inp), _f(out)))
+        return out
+
+    def gradients(self, inp, target, mask=None):
+        inp = np.ascontiguousarray(inp, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        mask = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        g = np.zeros(unet_param_count(), np.float32)
+        loss = C.c_double(0)
+        self._check(lib().sfp_unet_gradients(self._h, _f(inp), _f(target), _f(mask), C.byref(loss),
+                                             _f(g)))
+        return loss.value, g
+
+    def train_step_device(self, inp, target, mask, lr):
+        """One Adam step on device arrays (int pointers); returns the loss before it."""
+        loss = C.c_double(0)
+        self._check(lib().sfp_unet_train_step_device(self._h, _f(inp), _f(target),
+                                                     None if mask is None else _f(mask),
+                                                     float(lr), C.byref(loss)))
+        return loss.value
