@@ -12,7 +12,9 @@ solves start cold (no warm start) and every pressure step gets pristine
 velocities, so each step is a full solve.
 
 Prints one JSON line (see the driver contract); `value` is the mean seconds
-per solve. The `roofline` object times the fine-level Chebyshev smoother
+per solve of plain MG-PCG (the library's default). `learned_fcg_s` is the
+same solves with the learned preconditioner (the package's trained U-Net
+weights) over the same step count, and `learned_over_mg_time` their ratio. The `roofline` object times the fine-level Chebyshev smoother
 kernel (the dominant kernel) with HIP events on the library's stream during
 the timed steps. The `cpu_baseline` object times the FP64 CPU oracle
 (smoothed-aggregation AMG + SPAI-0, BiCGStab for pressure, CG for magnetic,
@@ -48,6 +50,8 @@ def parse():
     ap.add_argument("--precond", choices=["mg", "jacobi"], default="mg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-preconditioner leg (same systems, same step count)")
     ap.add_argument("--systems", default="p,m0.5,m2,m8")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (counter calibration)")
@@ -217,7 +221,14 @@ def main():
     ptr = lambda t: t.data_ptr()
     ring_i = [0]
 
-    def run_step(i):
+    def refill_ring():
+        for ring in vel_ring:
+            for a, b in zip(ring, vel0):
+                a.copy_(b)
+        ring_i[0] = 0
+        torch.cuda.synchronize()
+
+    def run_step(i, ctx=ctx):
         name, chi = systems[i % len(systems)]
         if chi is None:
             v = vel_ring[ring_i[0] % len(vel_ring)]
@@ -287,6 +298,10 @@ def main():
         d["iterations"].append(st["iterations"])
         d["rel_residual"].append(st["rel_residual"])
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_leg(args, systems, total, run_step, refill_ring, geo, by_sys, log)
+
     cpu = None
     checks = {}
     if not args.no_cpu_baseline:
@@ -309,9 +324,12 @@ def main():
         "config": {"workload": f"{n}^3 pressure projection + FDM magnetic potential "
                                f"(chi 0.5/2/8) solves to FP64 rel. residual {args.tol:g}",
                    "grid": [n, n, n], "precond": args.precond, "systems": args.systems,
-                   "learned_fcg": "not built yet", "parallelism": "single GPU"},
+                   "learned_fcg": None if learned is None else learned["weights"],
+                   "parallelism": "single GPU"},
         "mg_pcg_s": round(sec_per_solve, 6),
-        "learned_fcg_s": None,
+        "learned_fcg_s": None if learned is None else learned["s_per_solve"],
+        "learned_over_mg_time": None if learned is None else round(learned["s_per_solve"] / sec_per_solve, 3),
+        "learned": learned,
         "cells_per_s": round(cells / sec_per_solve, 1),
         "per_system": by_sys,
         "kernels": kernels,
@@ -321,6 +339,50 @@ def main():
     }
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def learned_leg(args, systems, total, run_step, refill_ring, geo, mg_by_sys, log):
+    """The same systems and step count with the learned preconditioner (the
+    package's trained weights) in a second context."""
+    import torch
+
+    import sfp
+    from sfp import learn
+
+    params, meta = learn.load_weights()
+    lctx = sfp.Context(geo.n, geo.spacing, geo.origin, tol=args.tol, max_iters=200)
+    lctx.set_network(params)
+    lctx.set_params(precond=sfp.PRECOND_LEARNED)
+    refill_ring()
+    for i in range(args.warmup):
+        name, st = run_step(i, lctx)
+        log(f"learned warmup {name}: iters={st['iterations']} fallback={st['fallback']} "
+            f"rel={st['rel_residual']:.2e} solve={st['solve_ms']:.2f}ms")
+    torch.cuda.synchronize()
+    records = []
+    t_start = time.perf_counter()
+    for i in range(args.warmup, total):
+        name, st = run_step(i, lctx)
+        st["system"] = name
+        records.append(st)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    lctx.close()
+    by_sys = {}
+    for st in records:
+        log(f"learned {st['system']}: iters={st['iterations']} fallback={st['fallback']} "
+            f"rel={st['rel_residual']:.2e} solve={st['solve_ms']:.2f}ms")
+        d = by_sys.setdefault(st["system"], {"solve_ms": [], "iterations": [], "fallback": [],
+                                             "rel_residual": [], "mg_iterations": None})
+        d["solve_ms"].append(round(st["solve_ms"], 3))
+        d["iterations"].append(st["iterations"])
+        d["fallback"].append(st["fallback"])
+        d["rel_residual"].append(st["rel_residual"])
+        d["mg_iterations"] = mg_by_sys.get(st["system"], {}).get("iterations")
+    return {"s_per_solve": round(elapsed / args.steps, 6), "per_system": by_sys,
+            "converged_all": all(st["converged"] for st in records),
+            "weights": {"sha256": meta["sha256"], "params": meta["count"],
+                        "trained_at": meta.get("n"), "steps": meta.get("steps")}}
 
 
 def cpu_leg(psc, msc, pressure_out, potential_out, ctx, phi_p, frac, us, vel0, phi_m, hext, geo,

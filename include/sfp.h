@@ -36,8 +36,11 @@ typedef enum sfp_status {
 
 typedef enum sfp_precond {
     SFP_PRECOND_MG = 0,      /* FP32 geometric-multigrid V-cycle inside FP64 flexible CG */
-    SFP_PRECOND_LEARNED = 1, /* 3-D U-Net (BF16, MFMA) inside FP64 flexible CG, MG fallback */
-    SFP_PRECOND_JACOBI = 2   /* diagonal scaling (reference point for tests) */
+    SFP_PRECOND_LEARNED = 1, /* V-cycle plus a 3-D U-Net (BF16, MFMA) correction of the residual
+                                it leaves, inside FP64 flexible CG, MG fallback */
+    SFP_PRECOND_JACOBI = 2,  /* diagonal scaling (reference point for tests) */
+    SFP_PRECOND_SMOOTH = 3   /* the learned cycle without its network (zero correction, so the
+                                V-cycle plus extra post-smoothing): the ablation baseline */
 } sfp_precond;
 
 typedef struct sfp_params {

@@ -1,15 +1,21 @@
 /*
- * sfp_learned.h -- the learned preconditioner: a 3-level 3-D U-Net (BF16
- * activations and weights, FP32 accumulation on MFMA) that maps the residual
- * and three operator features to a search direction inside the FP64 flexible
- * CG, with a fall-back to the multigrid preconditioner when an iteration
- * reduces the residual by less than sfp_params.stall_ratio.
+ * sfp_learned.h -- the learned preconditioner. Inside the FP64 flexible CG
+ * each application runs the multigrid V-cycle, restricts the residual the
+ * V-cycle leaves to the first coarse level, and there applies a 3-level 3-D
+ * U-Net (BF16 activations and weights, FP32 accumulation on MFMA) that maps
+ * that residual and three operator features to an approximation of the
+ * level's exact correction; the correction is scaled to be optimal in the
+ * level's energy norm, prolonged and post-smoothed on the fine grid. The solve
+ * falls back to the plain V-cycle when an iteration reduces the residual by
+ * less than sfp_params.stall_ratio.
  * No counterpart in the reference (it solves with amgcl only).
  *
  * Volumes are channels-last float arrays [batch][x][y][z][c]; every axis must
- * be a multiple of 4. Network input channels: 0 residual (scaled to unit RMS
- * over the active cells), 1 log(diag/6), 2 (diag - sum of couplings)/diag,
- * 3 clamp(phi/dx, -4, 4)/4; inactive cells are all zero.
+ * be a multiple of 4 (for the solver: the first coarse level, (n+1)/2 per axis).
+ * Network input channels: 0 residual (scaled to unit RMS over the active
+ * cells), 1 log(diag/6), 2 (diag - sum of the six couplings)/diag,
+ * 3 clamp(phi/h, -4, 4)/4 with h the level's cell size and phi the mean of
+ * the fine level set over the children; inactive cells are all zero.
  */
 #ifndef SFP_LEARNED_H
 #define SFP_LEARNED_H
@@ -26,9 +32,10 @@ long long sfp_unet_param_count(void);
  * SFP_PRECOND_LEARNED. */
 sfp_status sfp_set_network(sfp_context ctx, const float *params, long long count);
 
-/* Operator features of the last solve of `which` (0 pressure, 1 magnetic), as
- * fed to the network: out has 3 floats per cell (channels 1..3); phi is the
- * level set (host, N^3). */
+/* Operator features of the last solve of `which` (0 pressure, 1 magnetic) on
+ * the first coarse level, as fed to the network: out has 3 floats per coarse
+ * cell (channels 1..3; dims from sfp_test_level_dims(ctx, which, 1, dims));
+ * phi is the fine level set (host, N^3) or NULL for zeros in channel 3. */
 sfp_status sfp_learned_features(sfp_context ctx, int which, const double *phi, float *out);
 
 /* ---- stand-alone network (training and tests) ---- */

@@ -29,11 +29,17 @@ sfp_status sfp_test_precondition(sfp_context ctx, int which, const double *r, fl
 /* r = rhs - A x with the fine FP64 operator, plus ||r||^2 and ||rhs||^2. */
 sfp_status sfp_test_residual(sfp_context ctx, int which, const double *x, double *r,
                              double *rr, double *bb);
-/* Solve A x = rhs for the assembled system with the current parameters from
- * x = 0 (host arrays); the system's own right-hand side and solution (the warm
- * start) are restored afterwards. stats may be NULL. */
-sfp_status sfp_test_solve(sfp_context ctx, int which, const double *rhs, double *x,
+/* Solve A_l x = rhs with level `level`'s FP64 operator (0 = the assembled
+ * system) by flexible CG with the V-cycle from that level down and the current
+ * parameters, from x = 0 (host arrays of that level's cells). The system's
+ * own right-hand side and solution (the warm start) are restored afterwards.
+ * stats may be NULL. */
+sfp_status sfp_test_solve(sfp_context ctx, int which, int level, const double *rhs, double *x,
                           sfp_stats *stats);
+/* rc = the residual one V-cycle leaves for right-hand side r, restricted to
+ * the first coarse level: the learned correction's input (host arrays: r
+ * fine, rc coarse). */
+sfp_status sfp_test_restrict_smoothed(sfp_context ctx, int which, const double *r, float *rc);
 
 #ifdef __cplusplus
 }

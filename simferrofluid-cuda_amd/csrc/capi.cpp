@@ -327,7 +327,7 @@ void sfp_destroy(sfp_context ctx) {
 
 sfp_status sfp_set_params(sfp_context ctx, const sfp_params *p) {
     if (!ctx || !p) return SFP_ERR_ARGUMENT;
-    if (!(p->tol > 0) || p->max_iters < 0 || p->precond < 0 || p->precond > 2 ||
+    if (!(p->tol > 0) || p->max_iters < 0 || p->precond < 0 || p->precond > 3 ||
         p->mg_coarsest < 2 || p->mg_coarsest > 4 || p->cheby_degree < 1 || p->cheby_degree > 8 ||
         !(p->cheby_ratio > 1) || !(p->coarse_scale > 0) || !(p->stall_ratio > 0)) {
         ctx->last_error = "parameter out of range";

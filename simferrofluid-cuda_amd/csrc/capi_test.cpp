@@ -156,21 +156,30 @@ sfp_status sfp_test_residual(sfp_context ctx, int which, const double *x, double
     });
 }
 
-sfp_status sfp_test_solve(sfp_context ctx, int which, const double *rhs, double *x,
+sfp_status sfp_test_solve(sfp_context ctx, int which, int level, const double *rhs, double *x,
                           sfp_stats *stats) {
     return run(ctx, [&] {
         System &s = pick(ctx, which);
         if (!rhs || !x) throw ArgError("null vector");
-        const size_t n = s.dims().cells();
+        if (level < 0 || level >= s.num_levels()) throw ArgError("bad level");
+        const size_t n = s.dims().cells(), nl = s.level(level).d.cells();
         const hipStream_t st = ctx->ws.stream;
         DevBuf<double> keep_b, keep_x;
         keep_b.ensure(n);
         keep_x.ensure(n);
         SFP_HIP(hipMemcpyAsync(keep_b.p, s.rhs().p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         SFP_HIP(hipMemcpyAsync(keep_x.p, s.x().p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-        SFP_HIP(hipMemcpyAsync(s.rhs().p, rhs, n * sizeof(double), hipMemcpyHostToDevice, st));
-        const SolveInfo info = s.solve(ctx->ws, ctx->cfg, false);
-        d2h(ctx, x, s.x().p, n);
+        SFP_HIP(hipMemcpyAsync(s.rhs().p, rhs, nl * sizeof(double), hipMemcpyHostToDevice, st));
+        SolveInfo info;
+        try {
+            info = level == 0 ? s.solve(ctx->ws, ctx->cfg, false) : s.solve_level(ctx->ws, ctx->cfg, level);
+        } catch (...) {
+            (void)hipMemcpyAsync(s.rhs().p, keep_b.p, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+            (void)hipMemcpyAsync(s.x().p, keep_x.p, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+            (void)hipStreamSynchronize(st);
+            throw;
+        }
+        d2h(ctx, x, s.x().p, nl);
         SFP_HIP(hipMemcpyAsync(s.rhs().p, keep_b.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         SFP_HIP(hipMemcpyAsync(s.x().p, keep_x.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         SFP_HIP(hipStreamSynchronize(st));
@@ -181,8 +190,23 @@ sfp_status sfp_test_solve(sfp_context ctx, int which, const double *rhs, double 
             stats->fallback = info.fallback;
             stats->precond_used = info.precond_used;
             stats->rel_residual = info.rel;
-            stats->unknowns = (long long)n;
+            stats->unknowns = (long long)nl;
         }
+    });
+}
+
+sfp_status sfp_test_restrict_smoothed(sfp_context ctx, int which, const double *r, float *rc) {
+    return run(ctx, [&] {
+        System &s = pick(ctx, which);
+        if (!r || !rc) throw ArgError("null vector");
+        if (s.num_levels() < 2) throw StateError("the hierarchy has a single level");
+        const size_t n = s.dims().cells();
+        DevBuf<double> dr;
+        dr.ensure(n);
+        SFP_HIP(hipMemcpyAsync(dr.p, r, n * sizeof(double), hipMemcpyHostToDevice, ctx->ws.stream));
+        const float *b = s.restrict_smoothed(ctx->ws, ctx->cfg, dr.p);
+        d2h(ctx, rc, b, s.level(1).d.cells());
+        SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
     });
 }
 

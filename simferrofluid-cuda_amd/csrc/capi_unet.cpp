@@ -82,18 +82,30 @@ sfp_status sfp_learned_features(sfp_context ctx, int which, const double *phi, f
         if (which == 0 && ctx->pressure_ready && ctx->pressure_unknowns > 0) s = &ctx->pressure;
         if (which == 1 && ctx->field_ready) s = &ctx->magnetic;
         if (!s) throw StateError("no such system assembled");
+        if (s->num_levels() < 2) throw StateError("the hierarchy has a single level");
         const Dims d = s->dims();
+        const size_t nc = s->level(1).d.cells();
         DevBuf<double> dphi;
         DevBuf<float> f3;
-        f3.ensure(3 * d.cells());
+        f3.ensure(3 * nc);
         if (phi) {
             dphi.ensure(d.cells());
-            SFP_HIP(hipMemcpyAsync(dphi.p, phi, d.cells() * sizeof(double), hipMemcpyHostToDevice, ctx->ws.stream));
+            SFP_HIP(hipMemcpyAsync(dphi.p, phi, d.cells() * sizeof(double), hipMemcpyHostToDevice,
+                                   ctx->ws.stream));
         }
-        launch_learned_features(ctx->ws.stream, d, s->fine().op64(), s->fine().diag32.p, dphi.p, ctx->spacing,
-                                nullptr, f3.p);
-        SFP_HIP(hipMemcpyAsync(out, f3.p, 3 * d.cells() * sizeof(float), hipMemcpyDeviceToHost, ctx->ws.stream));
-        SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
+        Workspace &ws = ctx->ws;
+        const double *keep_phi = ws.phi;
+        ws.phi = dphi.p;
+        ws.spacing = ctx->spacing;
+        try {
+            s->learned_features(ws, f3.p);
+        } catch (...) {
+            ws.phi = keep_phi;
+            throw;
+        }
+        ws.phi = keep_phi;
+        SFP_HIP(hipMemcpyAsync(out, f3.p, 3 * nc * sizeof(float), hipMemcpyDeviceToHost, ws.stream));
+        SFP_HIP(hipStreamSynchronize(ws.stream));
         return SFP_OK;
     } catch (const StateError &e) {
         ctx->last_error = e.what();

@@ -54,7 +54,8 @@ __global__ __launch_bounds__(kBlock) void k_features(CellGrid cg, Dims d, Op64 A
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_rms_partials(CellGrid cg, Dims d, const double *r,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_rms_partials(CellGrid cg, Dims d, const T *r,
                                                          const float *diag32, double *partials) {
     int k, j, i0;
     cell_tile(cg, k, j, i0);
@@ -63,7 +64,8 @@ __global__ __launch_bounds__(kBlock) void k_rms_partials(CellGrid cg, Dims d, co
         for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
             const size_t c = d.idx(i, j, k);
             if (diag32[c] > 0.f) {
-                v[0] += r[c] * r[c];
+                const double rv = (double)r[c];
+                v[0] += rv * rv;
                 v[1] += 1.0;
             }
         }
@@ -78,7 +80,8 @@ __global__ __launch_bounds__(kBlock) void k_rms_finalize(unsigned nb, const doub
     if (flat_tid() == 0) sc->pad[1] = (v[0] > 0.0 && v[1] > 0.0) ? 1.0 / sqrt(v[0] / v[1]) : 0.0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_pack_residual(CellGrid cg, Dims d, const double *r,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pack_residual(CellGrid cg, Dims d, const T *r,
                                                           const float *diag32, const KrylovScalars *sc,
                                                           uint16_t *__restrict__ x8) {
     int k, j, i0;
@@ -87,7 +90,72 @@ __global__ __launch_bounds__(kBlock) void k_pack_residual(CellGrid cg, Dims d, c
     const double s = sc->pad[1];
     for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
         const size_t c = d.idx(i, j, k);
-        x8[c * 8] = diag32[c] > 0.f ? to_bf16((float)(r[c] * s)) : (uint16_t)0;
+        x8[c * 8] = diag32[c] > 0.f ? to_bf16((float)((double)r[c] * s)) : (uint16_t)0;
+    }
+}
+
+// Mean of the fine level set over the (up to 8) children of each coarse cell.
+__global__ __launch_bounds__(kBlock) void k_restrict_phi(CellGrid cg, Dims dc, Dims df, const double *phi,
+                                                         double *__restrict__ out) {
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
+    if (k >= dc.nz || j >= dc.ny) return;
+    for (int i = i0, ie = min(i0 + cg.planes, dc.nx); i < ie; i++) {
+        double s = 0.0;
+        int n = 0;
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+                for (int e = 0; e < 2; e++) {
+                    const int fi = 2 * i + a, fj = 2 * j + b, fk = 2 * k + e;
+                    if (fi < df.nx && fj < df.ny && fk < df.nz) {
+                        s += phi[df.idx(fi, fj, fk)];
+                        n++;
+                    }
+                }
+        out[dc.idx(i, j, k)] = s / n;
+    }
+}
+
+// Partials of (z.b, z.Az) over the active cells, z taken as 0 on inactive ones.
+__global__ __launch_bounds__(kBlock) void k_corr_dots(CellGrid cg, Dims d, Op32 A, const float *z,
+                                                      const float *b, double *partials) {
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
+    double v[2] = {0.0, 0.0};
+    if (k < d.nz && j < d.ny) {
+        const Strides s(d);
+        auto zv = [&](size_t c) { return A.diag[c] > 0.f ? z[c] : 0.f; };
+        for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
+            const size_t c = d.idx(i, j, k);
+            const float dg = A.diag[c];
+            if (dg > 0.f) {
+                const float zc = z[c];
+                const float az = dg * zc - offdiag_sum<float>(A.cpl, d, s, i, j, k, c, zv);
+                v[0] += (double)zc * b[c];
+                v[1] += (double)zc * az;
+            }
+        }
+    }
+    block_partials<2>(v, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_corr_finalize(unsigned nb, const double *partials,
+                                                          KrylovScalars *sc) {
+    double v[2];
+    sum_partials<2>(partials, nb, v);
+    if (flat_tid() == 0) sc->pad[2] = v[1] > 0.0 ? v[0] / v[1] : 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_corr_scale(CellGrid cg, Dims d, const float *z,
+                                                       const float *diag32, const KrylovScalars *sc,
+                                                       float *__restrict__ e) {
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
+    if (k >= d.nz || j >= d.ny) return;
+    const float a = (float)sc->pad[2];
+    for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
+        const size_t c = d.idx(i, j, k);
+        e[c] = diag32[c] > 0.f ? a * z[c] : 0.f;
     }
 }
 
@@ -111,15 +179,44 @@ void launch_learned_features(hipStream_t st, Dims d, Op64 A, const float *diag32
     SFP_LAUNCH_CHECK();
 }
 
-void launch_pack_residual(hipStream_t st, Dims d, const double *r, const float *diag32,
-                          double *partials, KrylovScalars *sc, uint16_t *x8) {
+template <class T>
+void pack_residual(hipStream_t st, Dims d, const T *r, const float *diag32, double *partials,
+                   KrylovScalars *sc, uint16_t *x8) {
     Launch L = cell_launch(d, kReducePlanes);
-    hipLaunchKernelGGL(k_rms_partials, L.grid, L.block, 0, st, L.cg, d, r, diag32, partials);
+    hipLaunchKernelGGL(k_rms_partials<T>, L.grid, L.block, 0, st, L.cg, d, r, diag32, partials);
     SFP_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_rms_finalize, dim3(1), dim3(kBlock), 0, st, L.blocks(), partials, sc);
     SFP_LAUNCH_CHECK();
     Launch P = cell_launch(d);
-    hipLaunchKernelGGL(k_pack_residual, P.grid, P.block, 0, st, P.cg, d, r, diag32, sc, x8);
+    hipLaunchKernelGGL(k_pack_residual<T>, P.grid, P.block, 0, st, P.cg, d, r, diag32, sc, x8);
+    SFP_LAUNCH_CHECK();
+}
+
+void launch_pack_residual(hipStream_t st, Dims d, const double *r, const float *diag32,
+                          double *partials, KrylovScalars *sc, uint16_t *x8) {
+    pack_residual(st, d, r, diag32, partials, sc, x8);
+}
+
+void launch_pack_residual32(hipStream_t st, Dims d, const float *r, const float *diag32,
+                            double *partials, KrylovScalars *sc, uint16_t *x8) {
+    pack_residual(st, d, r, diag32, partials, sc, x8);
+}
+
+void launch_restrict_phi(hipStream_t st, Dims df, const double *phi, Dims dc, double *out) {
+    Launch L = cell_launch(dc);
+    hipLaunchKernelGGL(k_restrict_phi, L.grid, L.block, 0, st, L.cg, dc, df, phi, out);
+    SFP_LAUNCH_CHECK();
+}
+
+void launch_scaled_correction(hipStream_t st, Dims d, Op32 A, const float *z, const float *b,
+                              double *partials, KrylovScalars *sc, float *e) {
+    Launch L = cell_launch(d, kReducePlanes);
+    hipLaunchKernelGGL(k_corr_dots, L.grid, L.block, 0, st, L.cg, d, A, z, b, partials);
+    SFP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_corr_finalize, dim3(1), dim3(kBlock), 0, st, L.blocks(), partials, sc);
+    SFP_LAUNCH_CHECK();
+    Launch P = cell_launch(d);
+    hipLaunchKernelGGL(k_corr_scale, P.grid, P.block, 0, st, P.cg, d, z, A.diag, sc, e);
     SFP_LAUNCH_CHECK();
 }
 

@@ -178,9 +178,27 @@ float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) 
     launch_restrict(st, L.d, A, L.b.p, cur, lv_[l + 1].b.p);
     if (fine && ws.profiling) ws.timers[K_RESTRICT].end(st);
     const float *e = vlevel(ws, cb, l + 1, nullptr);
-    smooth(SM_PROLONG | (deg > 1 ? SM_WRITE_D : 0), 0.f, cb.cr[0], nullptr, e);
-    for (int s = 1; s < deg; s++)
-        smooth(SM_READ_D | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s], nullptr, nullptr);
+    return post_smooth(ws, cb, l, cur, e);
+}
+
+float *System::post_smooth(Workspace &ws, const Cheby &cb, int l, float *cur, const float *e) {
+    Level &L = lv_[l];
+    const hipStream_t st = ws.stream;
+    const Op32 A = L.op32();
+    float *oth = cur == L.x0.p ? L.x1.p : L.x0.p;
+    const int deg = cb.degree;
+    const bool track = l == 0 && ws.profiling;
+    for (int s = 0; s < deg; s++) {
+        // step 0 adds the prolonged correction; the fine-level steps that read
+        // and write the direction (36 algorithmic bytes per cell) are timed
+        const int mode = (s == 0 ? SM_PROLONG : SM_READ_D) | (s < deg - 1 ? SM_WRITE_D : 0);
+        const bool t = track && mode == (SM_READ_D | SM_WRITE_D);
+        if (t) ws.timers[K_SMOOTH].begin(st, 36.0 * (double)L.d.cells());
+        launch_smooth(st, mode, L.d, A, nullptr, L.b.p, cur, oth, L.dv.p, cb.cr[0], s == 0 ? 0.f : cb.cd[s],
+                      s == 0 ? cb.cr[0] : cb.cr[s], s == 0 ? e : nullptr);
+        if (t) ws.timers[K_SMOOTH].end(st);
+        std::swap(cur, oth);
+    }
     return cur;
 }
 
@@ -189,27 +207,71 @@ float *System::precondition(Workspace &ws, const SolverConfig &cfg, const double
         launch_jacobi(ws.stream, d_, lv_[0].op32(), r64, lv_[0].x0.p);
         return lv_[0].x0.p;
     }
+    if (cfg.precond == 3 && lv_.size() > 1) return learned_cycle(ws, cfg, r64, kCycleSmooth);
+    return vlevel(ws, Cheby::make(cfg.cheby_degree, cfg.cheby_ratio), 0, r64);
+}
+
+float *System::learned_cycle(Workspace &ws, const SolverConfig &cfg, const double *r64, Cycle c) {
     const Cheby cb = Cheby::make(cfg.cheby_degree, cfg.cheby_ratio);
-    return vlevel(ws, cb, 0, r64);
+    Level &L = lv_[0], &C = lv_[1];
+    const hipStream_t st = ws.stream;
+    float *z = vlevel(ws, cb, 0, r64);
+    // residual the V-cycle leaves, restricted to the first coarse level
+    launch_restrict(st, L.d, L.op32(), L.b.p, z, C.b.p);
+    if (c == kCycleRestrictOnly) return C.b.p;
+    if (c == kCycleLearned) {
+        network_correction(ws, C.x0.p);
+    } else {  // the same cycle without the network: a zero correction
+        SFP_HIP(hipMemsetAsync(C.x0.p, 0, C.d.cells() * sizeof(float), st));
+    }
+    return post_smooth(ws, cb, 0, z, C.x0.p);
 }
 
 bool System::learned_ok(const Workspace &ws) const {
-    return ws.net && d_.nx % 4 == 0 && d_.ny % 4 == 0 && d_.nz % 4 == 0;
+    if (!ws.net || lv_.size() < 2) return false;
+    const Dims c = lv_[1].d;
+    return c.nx % 4 == 0 && c.ny % 4 == 0 && c.nz % 4 == 0;
+}
+
+void System::coarse_phi(Workspace &ws) {
+    if (!ws.phi) return;
+    phic_.ensure(lv_[1].d.cells());
+    launch_restrict_phi(ws.stream, d_, ws.phi, lv_[1].d, phic_.p);
 }
 
 void System::prepare_learned(Workspace &ws) {
-    ws.net->ensure(UNetShape{1, d_.nx, d_.ny, d_.nz}, false);
-    launch_learned_features(ws.stream, d_, lv_[0].op64(), lv_[0].diag32.p, ws.phi, ws.spacing,
-                            ws.net->input_buffer(), nullptr);
+    const Level &C = lv_[1];
+    ws.net->ensure(UNetShape{1, C.d.nx, C.d.ny, C.d.nz}, false);
+    coarse_phi(ws);
+    launch_learned_features(ws.stream, C.d, C.op64(), C.diag32.p, ws.phi ? phic_.p : nullptr,
+                            2.0 * ws.spacing, ws.net->input_buffer(), nullptr);
 }
 
-float *System::precondition_learned(Workspace &ws, const double *r64) {
+void System::learned_features(Workspace &ws, float *f3) {
+    if (lv_.size() < 2) throw StateError("the hierarchy has a single level");
+    const Level &C = lv_[1];
+    coarse_phi(ws);
+    launch_learned_features(ws.stream, C.d, C.op64(), C.diag32.p, ws.phi ? phic_.p : nullptr,
+                            2.0 * ws.spacing, nullptr, f3);
+}
+
+void System::network_correction(Workspace &ws, float *x) {
+    Level &C = lv_[1];
+    const hipStream_t st = ws.stream;
     UNet &net = *ws.net;
-    launch_pack_residual(ws.stream, d_, r64, lv_[0].diag32.p, ws.partials.p, ws.scalars.p,
-                         net.input_buffer());
-    net.forward(ws.stream, net.input_buffer(), net.output_buffer());
-    launch_mask_output(ws.stream, d_, net.output_buffer(), lv_[0].diag32.p, lv_[0].x0.p);
-    return lv_[0].x0.p;
+    launch_pack_residual32(st, C.d, C.b.p, C.diag32.p, ws.partials.p, ws.scalars.p, net.input_buffer());
+    net.forward(st, net.input_buffer(), net.output_buffer());
+    launch_scaled_correction(st, C.d, C.op32(), net.output_buffer(), C.b.p, ws.partials.p,
+                             ws.scalars.p, x);
+}
+
+float *System::precondition_learned(Workspace &ws, const SolverConfig &cfg, const double *r64) {
+    return learned_cycle(ws, cfg, r64, kCycleLearned);
+}
+
+const float *System::restrict_smoothed(Workspace &ws, const SolverConfig &cfg, const double *r64) {
+    if (lv_.size() < 2) throw StateError("the hierarchy has a single level");
+    return learned_cycle(ws, cfg, r64, kCycleRestrictOnly);
 }
 
 long long System::count_active(Workspace &ws) {
@@ -218,14 +280,17 @@ long long System::count_active(Workspace &ws) {
     return (long long)ws.host_sc->pad[0];
 }
 
-SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
+SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool warm) {
     SolveInfo info;
     info.precond_used = cfg.precond;
     const hipStream_t st = ws.stream;
-    const size_t n = d_.cells();
-    const Op64 A = lv_[0].op64();
+    const Dims d = lv_[level].d;
+    const size_t n = d.cells();
+    const Op64 A = lv_[level].op64();
+    const bool fine = level == 0;
+    const bool profiling = fine && ws.profiling;
     if (!warm) SFP_HIP(hipMemsetAsync(x_.p, 0, n * sizeof(double), st));
-    launch_residual64(st, d_, A, rhs_.p, warm ? x_.p : nullptr, r_.p, ws.partials.p,
+    launch_residual64(st, d, A, rhs_.p, warm ? x_.p : nullptr, r_.p, ws.partials.p,
                       ws.scalars.p);
     ws.read_scalars();
     const double bb = ws.host_sc->bb;
@@ -241,7 +306,7 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
     const double cells = (double)n;
     // learned preconditioner until an iteration reduces the residual by less
     // than stall_ratio; then the multigrid V-cycle
-    bool learned = cfg.precond == 1 && learned_ok(ws);
+    bool learned = fine && cfg.precond == 1 && learned_ok(ws);
     if (learned) prepare_learned(ws);
     info.precond_used = learned ? 1 : (cfg.precond == 1 ? 0 : cfg.precond);
     while (true) {
@@ -249,7 +314,7 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
             if (checked) break;
             // recursive residual reached tol: recompute it from x in FP64 and
             // restart from the true residual if the two have drifted apart
-            launch_residual64(st, d_, A, rhs_.p, x_.p, r_.p, ws.partials.p,
+            launch_residual64(st, d, A, rhs_.p, x_.p, r_.p, ws.partials.p,
                               ws.scalars.p);
             ws.read_scalars();
             rel = std::sqrt(ws.host_sc->rr / bb);
@@ -258,15 +323,21 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
             continue;
         }
         if (info.iterations >= cfg.max_iters) break;
-        const float *z = learned ? precondition_learned(ws, r_.p) : precondition(ws, cfg, r_.p);
-        if (ws.profiling) ws.timers[K_MATVEC64].begin(st, (4.0 + 32.0 + 8.0 + 8.0 + 8.0) * cells);
-        launch_fcg_matvec(st, d_, A, z, r_.p, q_.p, w_.p, first ? 1 : 0, ws.partials.p,
+        const float *z;
+        if (learned)
+            z = precondition_learned(ws, cfg, r_.p);
+        else if (fine)
+            z = precondition(ws, cfg, r_.p);
+        else
+            z = vlevel(ws, Cheby::make(cfg.cheby_degree, cfg.cheby_ratio), level, r_.p);
+        if (profiling) ws.timers[K_MATVEC64].begin(st, (4.0 + 32.0 + 8.0 + 8.0 + 8.0) * cells);
+        launch_fcg_matvec(st, d, A, z, r_.p, q_.p, w_.p, first ? 1 : 0, ws.partials.p,
                           ws.scalars.p);
-        if (ws.profiling) ws.timers[K_MATVEC64].end(st);
-        if (ws.profiling) ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 5 + 8.0 * 4) * cells);
-        launch_fcg_update(st, d_, z, w_.p, p_.p, q_.p, x_.p, r_.p, first ? 1 : 0, ws.partials.p,
+        if (profiling) ws.timers[K_MATVEC64].end(st);
+        if (profiling) ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 5 + 8.0 * 4) * cells);
+        launch_fcg_update(st, d, z, w_.p, p_.p, q_.p, x_.p, r_.p, first ? 1 : 0, ws.partials.p,
                           ws.scalars.p);
-        if (ws.profiling) ws.timers[K_UPDATE64].end(st);
+        if (profiling) ws.timers[K_UPDATE64].end(st);
         first = false;
         info.iterations++;
         if (!learned && cfg.precond == 1 && info.precond_used == 1) info.fallback++;
@@ -277,14 +348,14 @@ SolveInfo System::solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
         if (learned && rel > cfg.stall_ratio * prev) learned = false;  // fall back to multigrid
     }
     if (!checked) {
-        launch_residual64(st, d_, A, rhs_.p, x_.p, r_.p, ws.partials.p, ws.scalars.p);
+        launch_residual64(st, d, A, rhs_.p, x_.p, r_.p, ws.partials.p, ws.scalars.p);
         ws.read_scalars();
         rel = std::sqrt(ws.host_sc->rr / bb);
     }
     info.rel = rel;
     info.converged = rel <= cfg.tol;
     if (info.fallback > 0) info.precond_used = 0;
-    if (ws.profiling)
+    if (profiling)
         for (auto &t : ws.timers) t.harvest();
     return info;
 }

@@ -14,7 +14,7 @@ namespace sfp {
 struct SolverConfig {
     double tol = 1e-8;
     int max_iters = 100;
-    int precond = 0;  // sfp_precond
+    int precond = 0;  // sfp_precond: 0 MG, 1 learned, 2 Jacobi, 3 smoothing only
     int coarsest = 4;
     int cheby_degree = 3;
     double cheby_ratio = 4.0;
@@ -82,20 +82,46 @@ class System {
 
     // Coarsen the fine operator into the hierarchy and invert the coarsest level.
     void build_hierarchy(Workspace &ws, const SolverConfig &cfg);
-    // One preconditioner application z = B r (FP32 result, fine grid).
+    // One preconditioner application z = B r (FP32 result, fine grid): the
+    // V-cycle, Jacobi, or (cfg.precond 3) the learned cycle with a zero
+    // correction in place of the network's.
     float *precondition(Workspace &ws, const SolverConfig &cfg, const double *r64);
-    // The learned preconditioner can run on this grid (each axis a multiple of 4).
+    // The learned preconditioner can run on this hierarchy: a network is
+    // loaded and the first coarse level has every axis a multiple of 4.
     bool learned_ok(const Workspace &ws) const;
-    // Network input features of the current operator (into the network input).
+    // Network input features of the first coarse level (into the network input).
     void prepare_learned(Workspace &ws);
-    float *precondition_learned(Workspace &ws, const double *r64);
+    // The learned cycle: the V-cycle, then the residual it leaves restricted
+    // to the first coarse level, the network's correction there (scaled to
+    // be optimal in that level's energy norm), prolongation and fine
+    // post-smoothing.
+    float *precondition_learned(Workspace &ws, const SolverConfig &cfg, const double *r64);
+    // The network's input of the learned cycle: the first coarse level's
+    // right-hand side after the V-cycle of r64 (FP32).
+    const float *restrict_smoothed(Workspace &ws, const SolverConfig &cfg, const double *r64);
+    // Features of the first coarse level into f3 ([cells][3] FP32).
+    void learned_features(Workspace &ws, float *f3);
     // Flexible CG on A x = rhs; x holds the start when warm.
-    SolveInfo solve(Workspace &ws, const SolverConfig &cfg, bool warm);
+    SolveInfo solve(Workspace &ws, const SolverConfig &cfg, bool warm) {
+        return fcg(ws, cfg, 0, warm);
+    }
+    // Flexible CG with level `level`'s FP64 operator and the V-cycle from that
+    // level down, on the first cells of rhs/x (from x = 0).
+    SolveInfo solve_level(Workspace &ws, const SolverConfig &cfg, int level) {
+        return fcg(ws, cfg, level, false);
+    }
     // Number of active rows (sets ws.scalars->pad[0]); synchronous.
     long long count_active(Workspace &ws);
 
   private:
+    enum Cycle { kCycleLearned, kCycleSmooth, kCycleRestrictOnly };
+    float *learned_cycle(Workspace &ws, const SolverConfig &cfg, const double *r64, Cycle c);
+    float *post_smooth(Workspace &ws, const Cheby &cb, int l, float *cur, const float *e);
+    SolveInfo fcg(Workspace &ws, const SolverConfig &cfg, int level, bool warm);
     float *vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64);
+    void network_correction(Workspace &ws, float *x);
+    void coarse_phi(Workspace &ws);
+    DevBuf<double> phic_;  // level set restricted to the first coarse level
     Dims d_{};
     std::vector<Level> lv_;
     DevBuf<double> rhs_, x_, r_, p_, q_, w_;

@@ -21,7 +21,7 @@ _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
 
 OK, ERR_ARGUMENT, ERR_INPUT, ERR_HIP, ERR_NO_DEVICE, ERR_STATE, ERR_UNSUPPORTED = range(7)
-PRECOND_MG, PRECOND_LEARNED, PRECOND_JACOBI = 0, 1, 2
+PRECOND_MG, PRECOND_LEARNED, PRECOND_JACOBI, PRECOND_SMOOTH = 0, 1, 2, 3
 KERNEL_SMOOTH, KERNEL_MATVEC64, KERNEL_UPDATE64, KERNEL_RESTRICT = 0, 1, 2, 3
 KERNEL_NAMES = {0: "smooth", 1: "matvec64", 2: "update64", 3: "restrict"}
 
@@ -36,7 +36,8 @@ EXPORTS = {
     "sfp_profile.h": ["sfp_profile_enable", "sfp_profile_query"],
     "sfp_test.h": ["sfp_test_num_levels", "sfp_test_level_dims", "sfp_test_export_level",
                    "sfp_test_export_level32", "sfp_test_apply64", "sfp_test_precondition",
-                   "sfp_test_residual", "sfp_test_solve"],
+                   "sfp_test_residual", "sfp_test_solve",
+                   "sfp_test_restrict_smoothed"],
     "sfp_learned.h": ["sfp_unet_param_count", "sfp_set_network", "sfp_learned_features",
                       "sfp_unet_create", "sfp_unet_destroy", "sfp_unet_set_params",
                       "sfp_unet_get_params", "sfp_unet_init_params", "sfp_unet_forward",
@@ -129,7 +130,9 @@ def lib():
         L.sfp_test_residual.restype = C.c_int
         L.sfp_test_residual.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp]
         L.sfp_test_solve.restype = C.c_int
-        L.sfp_test_solve.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(Stats)]
+        L.sfp_test_solve.argtypes = [vp, C.c_int, C.c_int, _dp, _dp, C.POINTER(Stats)]
+        L.sfp_test_restrict_smoothed.restype = C.c_int
+        L.sfp_test_restrict_smoothed.argtypes = [vp, C.c_int, _dp, _fp]
         L.sfp_unet_param_count.restype = C.c_longlong
         L.sfp_unet_param_count.argtypes = []
         L.sfp_set_network.restype = C.c_int
@@ -319,8 +322,9 @@ class Context:
         self._check(lib().sfp_set_network(self._h, _f(params), params.size))
 
     def learned_features(self, which, phi=None):
-        """Network input channels 1..3 of the last assembled system, [nx,ny,nz,3]."""
-        out = np.zeros(self.n + (3,), np.float32)
+        """Network input channels 1..3 on the first coarse level of the last
+        assembled system, shape level_dims(which, 1) + (3,)."""
+        out = np.zeros(self.level_dims(which, 1) + (3,), np.float32)
         ph = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
         self._check(lib().sfp_learned_features(self._h, int(which), _d(ph), _f(out)))
         return out
@@ -378,13 +382,23 @@ class Context:
         self._check(lib().sfp_test_residual(self._h, which, _d(x), _d(r), C.byref(rr), C.byref(bb)))
         return r, rr.value, bb.value
 
-    def solve_rhs(self, which, rhs):
-        """x = A^-1 rhs for the assembled system (from x = 0); returns (x, stats)."""
+    def solve_rhs(self, which, rhs, level=0):
+        """x = A_level^-1 rhs (level 0: the assembled system; from x = 0);
+        returns (x, stats)."""
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
         x = np.zeros(rhs.shape)
         st = Stats()
-        self._check(lib().sfp_test_solve(self._h, int(which), _d(rhs), _d(x), C.byref(st)))
+        self._check(lib().sfp_test_solve(self._h, int(which), int(level), _d(rhs), _d(x),
+                                         C.byref(st)))
         return x, st.as_dict()
+
+    def restrict_smoothed(self, which, r):
+        """The residual one V-cycle leaves for right-hand side r, restricted
+        to the first coarse level (FP32, coarse dims)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        rc = np.zeros(self.level_dims(which, 1), np.float32)
+        self._check(lib().sfp_test_restrict_smoothed(self._h, int(which), _d(r), _f(rc)))
+        return rc
 
 
 def unet_param_count() -> int:

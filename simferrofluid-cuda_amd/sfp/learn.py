@@ -1,26 +1,31 @@
 """Training of the U-Net preconditioner (include/sfp_learned.h).
 
-The network maps a residual r (scaled to unit RMS over the active cells) and
-three operator features to a search direction z for the FP64 flexible CG.
-FCG is invariant to the scale of z, so the network is trained on directions:
-the target of a sample is e = A^-1 r scaled to unit RMS, with a masked MSE
-over the active cells.
+The learned preconditioner runs the multigrid V-cycle and then corrects the
+residual it leaves: restricted to the first coarse level, that residual r_c
+(scaled to unit RMS over the active coarse cells) and three features of the
+coarse operator go through the network, whose output approximates the exact
+coarse correction e_c = A_c^-1 r_c; it is prolonged and post-smoothed on the
+fine grid. The cycle rescales the network output to be optimal in A_c's
+energy norm, so the network is trained on directions: the target of a sample
+is e_c scaled to unit RMS, with a masked MSE over the active cells.
 
 Samples come from systems the library assembles itself (pressure projections
-of drops over pools and magnetostatic solves of wavy interfaces):
+of drops over pools and magnetostatic solves of wavy interfaces). Fine
+residuals are
 
-* r = the system's right-hand side;
-* r = residuals of unpreconditioned CG after a few iterations (what is left
-  after the easy, high-frequency part is gone);
-* r = white and box-smoothed noise on the active cells;
+* the system's right-hand side;
+* residuals of unpreconditioned CG after a few iterations;
+* white and box-smoothed noise on the active cells;
 
-and e is computed by the library's own multigrid-preconditioned solve
-(sfp_test_solve). Training runs on the GPU through sfp_unet_train_step_device
-(the library's forward, backward and Adam kernels); PyTorch only holds the
-device arrays and flips them for augmentation.
+each goes through one V-cycle of the library's own kernels and the residual it
+leaves is restricted (sfp_test_restrict_smoothed); e_c is the library's
+multigrid-preconditioned solve on the coarse level (sfp_test_solve). Training runs on
+the GPU through sfp_unet_train_step_device (the library's forward, backward
+and Adam kernels); PyTorch only holds the device arrays, stacks batches and
+flips them for augmentation.
 
-Usage: python -m sfp.learn --out weights/unet_precond.f32 [--n 128 --steps 1500]
-(from the package directory), or train() from Python.
+Usage (from the package directory):
+    python -m sfp.learn [--n 128 --steps 6000 --batch 8 --out weights/unet_precond.f32]
 """
 from __future__ import annotations
 
@@ -98,10 +103,11 @@ def _rms(v, active):
 
 # ------------------------------------------------------------------ harvest
 def harvest_system(ctx: Context, which: int, phi, rng, cg_keep=(2, 6, 16, 40), noise=2):
-    """(input [N,N,N,4], target [N,N,N], mask [N,N,N]) samples of one system."""
+    """(input [n,n,n,4], target [n,n,n], mask [n,n,n]) samples on the first
+    coarse level of one assembled system."""
     op = ctx.export_level(which, 0)
-    d32 = ctx.export_level32(which, 0)["diag"]
-    active = (d32 > 0).astype(np.float64)
+    active = (ctx.export_level32(which, 0)["diag"] > 0).astype(np.float64)
+    active_c = ctx.export_level32(which, 1)["diag"] > 0
     feats = ctx.learned_features(which, phi)
     diag, cpl, b = op["diag"], op["cpl"], op["rhs"]
     res = [b * active]
@@ -113,24 +119,28 @@ def harvest_system(ctx: Context, which: int, phi, rng, cg_keep=(2, 6, 16, 40), n
         res.append(w)
     out = []
     for r in res:
-        sr = _rms(r, active)
+        if _rms(r, active) == 0.0:
+            continue
+        rc = ctx.restrict_smoothed(which, r).astype(np.float64) * active_c
+        sr = _rms(rc, active_c)
         if sr == 0.0:
             continue
-        e, st = ctx.solve_rhs(which, r)
+        ec, st = ctx.solve_rhs(which, rc, level=1)
         if not st["converged"]:
             continue
-        se = _rms(e, active)
+        se = _rms(ec, active_c)
         if se == 0.0:
             continue
-        inp = np.empty(b.shape + (4,), np.float32)
-        inp[..., 0] = (r * active / sr).astype(np.float32)
+        inp = np.empty(rc.shape + (4,), np.float32)
+        inp[..., 0] = (rc / sr).astype(np.float32)
         inp[..., 1:] = feats
-        out.append((inp, (e * active / se).astype(np.float32), active.astype(np.float32)))
+        out.append((inp, (ec * active_c / se).astype(np.float32), active_c.astype(np.float32)))
     return out
 
 
-def harvest(n=128, pressure_seeds=(1, 2, 3, 4), magnetic=((5, 0.5), (6, 2.0), (7, 8.0)), seed=0,
-            log=print):
+def harvest(n=128, pressure_seeds=(100, 101, 102, 103, 104, 105, 106, 107),
+            magnetic=((108, 0.5), (109, 1.0), (110, 2.0), (111, 4.0), (112, 8.0), (113, 2.0)),
+            seed=0, log=print):
     samples = []
     rng = np.random.default_rng(seed)
     with Context(n, 1.0 / n, scenes.Geometry(n, 1.0 / n).origin, tol=1e-7, max_iters=200,
@@ -151,8 +161,9 @@ def harvest(n=128, pressure_seeds=(1, 2, 3, 4), magnetic=((5, 0.5), (6, 2.0), (7
 
 
 # ------------------------------------------------------------------ training
-def train(samples, steps=1500, lr=1e-3, seed=0, init=None, log=print, log_every=100):
-    """Adam on the masked MSE with random axis flips; returns (params, losses)."""
+def train(samples, steps=6000, lr=1e-3, seed=0, batch=8, init=None, log=print, log_every=200):
+    """Adam on the masked MSE over batches of randomly flipped samples;
+    returns (params, losses)."""
     import torch  # device arrays only
 
     shape = samples[0][0].shape[:3]
@@ -163,20 +174,25 @@ def train(samples, steps=1500, lr=1e-3, seed=0, init=None, log=print, log_every=
     torch.cuda.synchronize()
     rng = np.random.default_rng(seed)
     losses = []
-    with UNet(1, *shape, training=True) as net:
+    with UNet(batch, *shape, training=True) as net:
         if init is None:
             net.init_params(seed + 1)
         else:
             net.set_params(init)
         t0 = time.time()
         for step in range(steps):
-            i = int(rng.integers(len(samples)))
-            axes = [a for a in range(3) if rng.uniform() < 0.5]
-            x, t, m = inp[i], tgt[i], msk[i]
-            if axes:
-                x = torch.flip(x, axes).contiguous()
-                t = torch.flip(t, axes).contiguous()
-                m = torch.flip(m, axes).contiguous()
+            xs, ts, ms = [], [], []
+            for i in rng.integers(len(samples), size=batch):
+                axes = [a for a in range(3) if rng.uniform() < 0.5]
+                x, t, m = inp[i], tgt[i], msk[i]
+                if axes:
+                    x, t, m = torch.flip(x, axes), torch.flip(t, axes), torch.flip(m, axes)
+                xs.append(x)
+                ts.append(t)
+                ms.append(m)
+            x = torch.stack(xs).contiguous()
+            t = torch.stack(ts).contiguous()
+            m = torch.stack(ms).contiguous()
             torch.cuda.synchronize()
             frac = step / max(steps - 1, 1)
             rate = lr * (0.05 + 0.95 * 0.5 * (1.0 + np.cos(np.pi * frac)))  # cosine decay
@@ -215,7 +231,8 @@ def load_weights(path=WEIGHTS):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=WEIGHTS)
@@ -227,8 +244,8 @@ def main(argv=None):
     samples = harvest(a.n, seed=a.seed, log=log)
     th = time.time() - t0
     log(f"{len(samples)} samples in {th:.1f}s")
-    params, losses = train(samples, a.steps, a.lr, a.seed, log=log)
-    save_weights(params, a.out, dict(n=a.n, steps=a.steps, lr=a.lr, seed=a.seed,
+    params, losses = train(samples, a.steps, a.lr, a.seed, a.batch, log=log)
+    save_weights(params, a.out, dict(n=a.n, steps=a.steps, lr=a.lr, seed=a.seed, batch=a.batch,
                                      samples=len(samples), harvest_s=round(th, 1),
                                      train_s=round(time.time() - t0 - th, 1),
                                      loss_first100=float(np.mean(losses[:100])),

@@ -145,27 +145,70 @@ def test_unet_errors():
 
 
 # ------------------------------------------------------------------ inside the solver
-def test_features_match_operator():
+def _coarse_phi(phi):
+    nx, ny, nz = phi.shape
+    out = np.zeros(((nx + 1) // 2, (ny + 1) // 2, (nz + 1) // 2))
+    cnt = np.zeros_like(out)
+    for a in range(2):
+        for b in range(2):
+            for c in range(2):
+                ch = phi[a::2, b::2, c::2]
+                out[:ch.shape[0], :ch.shape[1], :ch.shape[2]] += ch
+                cnt[:ch.shape[0], :ch.shape[1], :ch.shape[2]] += 1
+    return out / cnt
+
+
+def test_features_match_coarse_operator():
     sc = scenes.pressure_scene(32, 3)
-    print("[case] learned features 32^3 pressure", flush=True)
+    print("[case] learned features 32^3 pressure (coarse level 16^3)", flush=True)
     with sfp.Context.for_scene(sc) as ctx:
         ctx.pressure_project(sc, want_pressure=False)
         f = ctx.learned_features(0, sc["phi"])
-        op = ctx.export_level(0, 0)
-        d32 = ctx.export_level32(0, 0)["diag"]
+        op = ctx.export_level(0, 1)
+        d32 = ctx.export_level32(0, 1)["diag"]
+    assert f.shape == (16, 16, 16, 3)
     act = d32 > 0
     diag, cpl = op["diag"], op["cpl"]
     off = cpl[0] + cpl[1] + cpl[2]
     off[:-1] += cpl[0][1:]
     off[:, :-1] += cpl[1][:, 1:]
     off[:, :, :-1] += cpl[2][:, :, 1:]
-    dx = sc["geo"].spacing
+    h = 2 * sc["geo"].spacing
     want = np.zeros(f.shape)
-    want[..., 0] = np.where(act, np.log(np.where(act, diag, 6.0) / 6.0), 0.0)
-    want[..., 1] = np.where(act, (diag - off) / np.where(act, diag, 1.0), 0.0)
-    want[..., 2] = np.where(act, np.clip(sc["phi"] / dx, -4, 4) / 4, 0.0)
+    safe = np.where(act, diag, 6.0)
+    want[..., 0] = np.where(act, np.log(safe / 6.0), 0.0)
+    want[..., 1] = np.where(act, (diag - off) / safe, 0.0)
+    want[..., 2] = np.where(act, np.clip(_coarse_phi(sc["phi"]) / h, -4, 4) / 4, 0.0)
     np.testing.assert_allclose(f, want, rtol=1e-6, atol=1e-6)
-    assert act.sum() > 1000 and np.all(f[~act] == 0)
+    assert act.sum() > 100 and np.all(f[~act] == 0)
+
+
+def _apply(op, x):
+    from sfp.learn import apply_full
+    return apply_full(op["diag"], op["cpl"], x)
+
+
+def test_coarse_level_hooks():
+    """restrict_smoothed gives the restricted residual a V-cycle leaves and
+    solve_rhs(level=1) solves the coarse operator to the tolerance."""
+    sc = scenes.magnetic_scene(32, 4, 2.0)
+    print("[case] coarse-level hooks 32^3 magnetic", flush=True)
+    with sfp.Context.for_scene(sc, tol=1e-10) as ctx:
+        ctx.magnetic_solve(sc, want_field=False)
+        op0 = ctx.export_level(1, 0)
+        op1 = ctx.export_level(1, 1)
+        act1 = ctx.export_level32(1, 1)["diag"] > 0
+        r = np.random.default_rng(5).normal(size=op0["diag"].shape)
+        r *= ctx.export_level32(1, 0)["diag"] > 0
+        rc = ctx.restrict_smoothed(1, r).astype(np.float64)
+        ec, st = ctx.solve_rhs(1, rc, level=1)
+        # the system's own solve is untouched by the hook
+        x_before, _, _ = ctx.residual(1, np.zeros(op0["diag"].shape))
+    assert st["converged"] and st["unknowns"] == act1.size
+    res = (rc - _apply(op1, ec)) * act1
+    assert np.linalg.norm(res) <= 1e-9 * np.linalg.norm(rc * act1)
+    assert np.linalg.norm(rc) > 0
+    np.testing.assert_array_equal(x_before, op0["rhs"])
 
 
 def _solve(sc, precond, params=None, kind="pressure", **kw):
@@ -191,20 +234,26 @@ def test_learned_requires_network():
         assert e.value.status == sfp.ERR_ARGUMENT
 
 
-def test_learned_random_network_falls_back_and_converges():
-    """An untrained network is a poor preconditioner: FCG falls back to the
-    multigrid V-cycle and still meets the tolerance with the same answer."""
+def test_learned_random_network_converges_and_falls_back():
+    """With an untrained network the learned cycle still converges (its
+    correction is energy-optimally scaled, so it cannot undo the V-cycle) to
+    the MG-PCG answer; with a stall ratio no iteration can meet, the solve
+    falls back to the plain V-cycle after the first iteration and converges."""
     sc = scenes.pressure_scene(64, 0)
     print("[case] learned (random weights) 64^3 pressure", flush=True)
     p = U.random_params(31).astype(np.float32)
-    vel_l, pres_l, st_l = _solve(sc, sfp.PRECOND_LEARNED, p)
+    vel_l, pres_l, st_l = _solve(sc, sfp.PRECOND_LEARNED, p, stall_ratio=10.0)
     vel_m, pres_m, st_m = _solve(sc, sfp.PRECOND_MG)
-    print(st_l, st_m)
+    vel_f, pres_f, st_f = _solve(sc, sfp.PRECOND_LEARNED, p, stall_ratio=1e-6)
+    print(st_l, st_m, st_f)
     assert st_l["converged"] and st_l["rel_residual"] <= 1e-8
-    assert st_l["fallback"] > 0 and st_l["precond_used"] == sfp.PRECOND_MG
-    assert rel_l2(pres_l, pres_m) <= 1e-6
-    for a in range(3):
-        assert np.abs(vel_l[a] - vel_m[a]).max() <= 1e-6 * np.abs(vel_m[a]).max()
+    assert st_l["fallback"] == 0 and st_l["precond_used"] == sfp.PRECOND_LEARNED
+    assert st_f["converged"] and st_f["rel_residual"] <= 1e-8
+    assert st_f["fallback"] == st_f["iterations"] - 1 and st_f["precond_used"] == sfp.PRECOND_MG
+    for pres, vel in ((pres_l, vel_l), (pres_f, vel_f)):
+        assert rel_l2(pres, pres_m) <= 1e-6
+        for a in range(3):
+            assert np.abs(vel[a] - vel_m[a]).max() <= 1e-6 * np.abs(vel_m[a]).max()
 
 
 def test_learned_grid_not_multiple_of_4_uses_multigrid():
@@ -223,19 +272,36 @@ def _trained():
 
 @pytest.mark.parametrize("kind", ["pressure", "magnetic"])
 def test_ablation_trained_network(kind):
-    """What the trained network gains over FCG without it (Jacobi scaling
-    only), at a fixed iteration budget, on a scene it was not trained on;
-    and the learned solve with fall-back meets the tolerance."""
+    """What the trained network gains over the same cycle without it (a zero
+    correction: the V-cycle plus the same post-smoothing) and over plain
+    MG-PCG, at a fixed budget of 5 iterations, on 128^3 scenes it was not
+    trained on; and the learned solve with fall-back meets the tolerance and
+    agrees with plain MG-PCG. Measured with the committed weights: the
+    pressure residual after 5 iterations is several times lower than both
+    baselines; on the magnetic system the network gains nothing (its error is
+    dominated by global modes the coarse levels already handle), so there the
+    test only bounds the loss."""
     params = _trained()
-    sc = scenes.pressure_scene(64, 11) if kind == "pressure" else scenes.magnetic_scene(64, 12, 2.0)
-    print(f"[case] ablation {kind} 64^3", flush=True)
-    budget = 12
-    _, _, st_l = _solve(sc, sfp.PRECOND_LEARNED, params, kind, max_iters=budget, tol=1e-14,
-                        stall_ratio=10.0)
-    _, _, st_j = _solve(sc, sfp.PRECOND_JACOBI, None, kind, max_iters=budget, tol=1e-14)
-    print(f"after {budget} its: learned {st_l['rel_residual']:.3e}  jacobi {st_j['rel_residual']:.3e}")
+    sc = scenes.pressure_scene(128, 11) if kind == "pressure" else scenes.magnetic_scene(128, 12, 2.0)
+    print(f"[case] ablation {kind} 128^3", flush=True)
+    budget = 5
+    kw = dict(max_iters=budget, tol=1e-14)
+    _, _, st_l = _solve(sc, sfp.PRECOND_LEARNED, params, kind, stall_ratio=10.0, **kw)
+    _, _, st_s = _solve(sc, sfp.PRECOND_SMOOTH, None, kind, **kw)
+    _, _, st_m = _solve(sc, sfp.PRECOND_MG, None, kind, **kw)
+    _, _, st_j = _solve(sc, sfp.PRECOND_JACOBI, None, kind, **kw)
+    print(f"after {budget} its: learned {st_l['rel_residual']:.3e}  without network "
+          f"{st_s['rel_residual']:.3e}  MG {st_m['rel_residual']:.3e}  "
+          f"Jacobi {st_j['rel_residual']:.3e}")
     assert st_l["fallback"] == 0 and st_l["precond_used"] == sfp.PRECOND_LEARNED
-    assert st_l["rel_residual"] < 0.5 * st_j["rel_residual"]
-    _, _, st = _solve(sc, sfp.PRECOND_LEARNED, params, kind)
+    assert st_l["rel_residual"] < 1e-3 * st_j["rel_residual"]
+    if kind == "pressure":
+        assert st_l["rel_residual"] < 0.5 * st_s["rel_residual"]
+        assert st_l["rel_residual"] < 0.5 * st_m["rel_residual"]
+    else:
+        assert st_l["rel_residual"] < 3.0 * st_s["rel_residual"]
+    vel_l, pres_l, st = _solve(sc, sfp.PRECOND_LEARNED, params, kind)
+    _, pres_m, _ = _solve(sc, sfp.PRECOND_MG, None, kind)
     print("learned with fall-back:", st)
     assert st["converged"] and st["rel_residual"] <= 1e-8
+    assert rel_l2(pres_l, pres_m) <= 1e-6

@@ -127,3 +127,27 @@ def test_geometry_matches_staggered_grid_construction():
     X, Y, Z = g.cell_centers()
     assert X[0, 0, 0] == g.origin[0] + (1.0 / 22) / 2
     assert g.face_shape(1) == (20, 19, 22)
+
+
+def test_trained_weights_load_and_checksum():
+    from sfp import learn
+    params, meta = learn.load_weights()
+    assert params.dtype == np.float32 and params.size == meta["count"] == 321745
+    assert np.all(np.isfinite(params))
+    assert meta["n"] == 128 and meta["samples"] > 0
+
+
+def test_apply_full_matches_csr():
+    from sfp import learn
+    from helpers import csr_from_full_grid
+    rng = np.random.default_rng(3)
+    shape = (5, 4, 6)
+    diag = rng.uniform(4, 6, shape)
+    cpl = [rng.uniform(0, 1, shape) for _ in range(3)]
+    for a in range(3):
+        sl = [slice(None)] * 3
+        sl[a] = 0
+        cpl[a][tuple(sl)] = 0.0
+    x = rng.normal(size=shape)
+    A = csr_from_full_grid(diag, cpl)
+    np.testing.assert_allclose(learn.apply_full(diag, cpl, x).ravel(), A @ x.ravel(), rtol=1e-14, atol=1e-14)
