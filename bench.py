@@ -88,7 +88,7 @@ def summarize_profiles(d, n=256):
     lines = [f"# k_smooth<READ_D|WRITE_D> at {n}^3: rocprofv3 kernel trace", "",
              "Command (GPU box, repo root):", "",
              "    rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_trace -o trace -- \\",
-             "        python3 bench.py --no-cpu-baseline --no-profile --steps 4 --warmup 0 --systems p,m2", "",
+             "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 4 --warmup 0 --systems p,m2", "",
              f"Fine-level dispatches of `{SMOOTH_KERNEL}` (grid {grid} work-items): {len(fine)}",
              f"average duration {avg:.2f} us (min {min(fine):.2f}, max {max(fine):.2f});",
              f"algorithmic bytes per launch 36 B x {cells} cells = {alg / 1e6:.1f} MB "
@@ -129,9 +129,9 @@ def summarize_profiles(d, n=256):
     md = [f"# k_smooth<READ_D|WRITE_D> at {n}^3: HBM traffic from FETCH_SIZE / WRITE_SIZE", "",
           "Commands (GPU box, repo root), one counter per pass:", "",
           "    rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/prof_fetch -o fetch -- \\",
-          "        python3 bench.py --no-cpu-baseline --no-profile --steps 2 --warmup 0 --systems p,m2",
+          "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 2 --warmup 0 --systems p,m2",
           "    rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/prof_write -o write -- \\",
-          "        python3 bench.py --no-cpu-baseline --no-profile --steps 2 --warmup 0 --systems p,m2",
+          "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 2 --warmup 0 --systems p,m2",
           "    rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/prof_copy_fetch -o copyf -- python3 bench.py --copy-probe",
           "    rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/prof_copy_write -o copyw -- python3 bench.py --copy-probe",
           "    python3 bench.py --summarize-profiles gpurun_out", "",
@@ -146,6 +146,25 @@ def summarize_profiles(d, n=256):
           f"| algorithmic bytes (36 B/cell) | {alg / 1e6:.1f} MB |",
           f"| traffic / algorithmic | {(rd + wr) / alg:.3f} |"]
     open(os.path.join(out_dir, f"traffic_smooth_{n}cube.md"), "w").write("\n".join(md) + "\n")
+    ld = os.path.join(d, "prof_learned", "learned_kernel_stats.csv")
+    if os.path.exists(ld):
+        st = rows(ld)
+        total = sum(float(r["TotalDurationNs"]) for r in st)
+        unet = [r for r in st if "k_conv" in r["Name"] or "k_pool" in r["Name"] or "k_up2" in r["Name"]
+                or "k_head" in r["Name"] or "k_pack" in r["Name"] or "k_corr" in r["Name"]]
+        lm = [f"# Learned leg at {n}^3 (network on the {n // 2}^3 first coarse level): rocprofv3 kernel trace",
+              "", "Command (GPU box, repo root):", "",
+              "    rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_learned -o learned -- \\",
+              "        python3 bench.py --no-cpu-baseline --no-profile --steps 2 --warmup 1 --systems p", "",
+              "The run holds both legs (plain MG-PCG, then the learned preconditioner) of 3 pressure",
+              "solves each. Network kernels: "
+              f"{sum(float(r['TotalDurationNs']) for r in unet) / 1e6:.3f} ms of {total / 1e6:.3f} ms.", "",
+              "| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
+        for r in st[:24]:
+            name = r["Name"].replace("sfp::(anonymous namespace)::", "").split("(")[0]
+            lm.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                      f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+        open(os.path.join(out_dir, f"learned_{n}cube_trace.md"), "w").write("\n".join(lm) + "\n")
     print(json.dumps(res))
 
 

@@ -12,6 +12,10 @@ namespace {
 
 enum FinalizeOp : int { FIN_RESIDUAL = 0, FIN_MATVEC = 1, FIN_UPDATE = 2, FIN_COUNT = 3 };
 
+// The stencil kernels below march over the block's x planes with the
+// x-neighbours in registers and take the z-neighbours (value and z coupling)
+// from the adjacent lanes of the tile row; only the row's end lanes load
+// them. The off-diagonal sum keeps the order x-, x+, y-, y+, z-, z+.
 __global__ __launch_bounds__(kBlock) void k_residual64(CellGrid cg, Dims d, Op64 A,
                                                        const double *__restrict__ b,
                                                        const double *__restrict__ x,
@@ -20,21 +24,56 @@ __global__ __launch_bounds__(kBlock) void k_residual64(CellGrid cg, Dims d, Op64
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     const Strides s(d);
+    const int tz = blockDim.x, lane = threadIdx.x;
+    const bool on = k < d.nz && j < d.ny && i0 < d.nx;
+    const bool zlo = k > 0, zhi = k + 1 < d.nz;
+    const int ie = min(i0 + cg.planes, d.nx);
     double v[2] = {0.0, 0.0};  // r.r, b.b
-    if (k < d.nz && j < d.ny) {
-        for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
-            const size_t c = d.idx(i, j, k);
-            const double bc = b[c];
-            double rc = bc;
-            if (x) {
-                const double off =
-                    offdiag_sum<double>(A.cpl, d, s, i, j, k, c, [&](size_t q) { return x[q]; });
-                rc = bc - (A.diag[c] * x[c] - off);
+    size_t c = on ? d.idx(i0, j, k) : 0;
+    if (!x) {
+        if (on)
+            for (int i = i0; i < ie; i++, c += s.sx) {
+                const double bc = b[c];
+                r[c] = bc;
+                v[0] += bc * bc;
+                v[1] += bc * bc;
             }
+        block_partials<2>(v, partials);
+        return;
+    }
+    double xm = (on && i0 > 0) ? x[c - s.sx] : 0.0;
+    double xc = on ? x[c] : 0.0;
+    double cxc = on ? A.cpl[0][c] : 0.0;
+    for (int i = i0; i < ie; i++, c += s.sx) {
+        const bool up = i + 1 < d.nx;
+        const double xp = (on && up) ? x[c + s.sx] : 0.0;
+        const double cxp = (on && up) ? A.cpl[0][c + s.sx] : 0.0;
+        const double czm = on ? A.cpl[2][c] : 0.0;
+        double xzm = __shfl_up(xc, 1, tz);
+        double xzp = __shfl_down(xc, 1, tz);
+        double czp = __shfl_down(czm, 1, tz);
+        if (lane == 0) xzm = (on && zlo) ? x[c - 1] : 0.0;
+        if (lane == tz - 1) {
+            xzp = (on && zhi) ? x[c + 1] : 0.0;
+            czp = (on && zhi) ? A.cpl[2][c + 1] : 0.0;
+        }
+        if (on) {
+            double off = 0.0;
+            if (i > 0) off += cxc * xm;
+            if (up) off += cxp * xp;
+            if (j > 0) off += A.cpl[1][c] * x[c - s.sy];
+            if (j + 1 < d.ny) off += A.cpl[1][c + s.sy] * x[c + s.sy];
+            if (zlo) off += czm * xzm;
+            if (zhi) off += czp * xzp;
+            const double bc = b[c];
+            const double rc = bc - (A.diag[c] * xc - off);
             r[c] = rc;
             v[0] += rc * rc;
             v[1] += bc * bc;
         }
+        xm = xc;
+        xc = xp;
+        cxc = cxp;
     }
     block_partials<2>(v, partials);
 }
@@ -48,19 +87,46 @@ __global__ __launch_bounds__(kBlock) void k_fcg_matvec(CellGrid cg, Dims d, Op64
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     const Strides s(d);
+    const int tz = blockDim.x, lane = threadIdx.x;
+    const bool on = k < d.nz && j < d.ny && i0 < d.nx;
+    const bool zlo = k > 0, zhi = k + 1 < d.nz;
+    const int ie = min(i0 + cg.planes, d.nx);
     double v[3] = {0.0, 0.0, 0.0};  // z.w, z.q_prev, z.r
-    if (k < d.nz && j < d.ny) {
-        for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
-            const size_t c = d.idx(i, j, k);
-            const double zc = (double)z[c];
-            const double off = offdiag_sum<double>(A.cpl, d, s, i, j, k, c,
-                                                   [&](size_t q) { return (double)z[q]; });
-            const double wc = A.diag[c] * zc - off;
-            w[c] = wc;
-            v[0] += zc * wc;
-            if (!first) v[1] += zc * qprev[c];
-            v[2] += zc * r[c];
+    size_t c = on ? d.idx(i0, j, k) : 0;
+    float zm = (on && i0 > 0) ? z[c - s.sx] : 0.f;
+    float zc = on ? z[c] : 0.f;
+    double cxc = on ? A.cpl[0][c] : 0.0;
+    for (int i = i0; i < ie; i++, c += s.sx) {
+        const bool up = i + 1 < d.nx;
+        const float zp = (on && up) ? z[c + s.sx] : 0.f;
+        const double cxp = (on && up) ? A.cpl[0][c + s.sx] : 0.0;
+        const double czm = on ? A.cpl[2][c] : 0.0;
+        float zzm = __shfl_up(zc, 1, tz);
+        float zzp = __shfl_down(zc, 1, tz);
+        double czp = __shfl_down(czm, 1, tz);
+        if (lane == 0) zzm = (on && zlo) ? z[c - 1] : 0.f;
+        if (lane == tz - 1) {
+            zzp = (on && zhi) ? z[c + 1] : 0.f;
+            czp = (on && zhi) ? A.cpl[2][c + 1] : 0.0;
         }
+        if (on) {
+            double off = 0.0;
+            if (i > 0) off += cxc * (double)zm;
+            if (up) off += cxp * (double)zp;
+            if (j > 0) off += A.cpl[1][c] * (double)z[c - s.sy];
+            if (j + 1 < d.ny) off += A.cpl[1][c + s.sy] * (double)z[c + s.sy];
+            if (zlo) off += czm * (double)zzm;
+            if (zhi) off += czp * (double)zzp;
+            const double zd = (double)zc;
+            const double wc = A.diag[c] * zd - off;
+            w[c] = wc;
+            v[0] += zd * wc;
+            if (!first) v[1] += zd * qprev[c];
+            v[2] += zd * r[c];
+        }
+        zm = zc;
+        zc = zp;
+        cxc = cxp;
     }
     block_partials<3>(v, partials);
 }

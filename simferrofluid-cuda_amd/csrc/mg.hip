@@ -13,7 +13,7 @@
 
 namespace sfp {
 
-Launch cell_launch(Dims d, int planes) {
+Launch cell_launch(Dims d, int planes, int max_planes) {
     Launch L;
     unsigned tz = 64;
     while (tz > 1 && (int)tz / 2 >= d.nz) tz /= 2;
@@ -24,7 +24,7 @@ Launch cell_launch(Dims d, int planes) {
         // march over several x planes per block (x-neighbour reuse in
         // registers) while keeping >= ~2048 blocks to fill 256 CUs
         const long long tiles = (long long)L.cg.gx * L.cg.gy * d.nx;
-        planes = (int)std::min<long long>(16, std::max<long long>(1, tiles / 2048));
+        planes = (int)std::min<long long>(max_planes, std::max<long long>(1, tiles / 2048));
     }
     L.cg.planes = planes;
     const unsigned gz = (d.nx + L.cg.planes - 1) / L.cg.planes;
@@ -60,12 +60,16 @@ __global__ __launch_bounds__(kBlock) void k_smooth_first(CellGrid cg, Dims d, Op
 
 // One Chebyshev step xout = x + d', d' = cd d + cr D^{-1} (b - A x), marching
 // over the block's x planes with the x-direction neighbours kept in
-// registers. The input iterate x is, by mode:
+// registers and the z-direction neighbours (iterate and z coupling) taken
+// from the adjacent lanes of the tile row (only the row's end lanes load
+// them). The input iterate x is, by mode:
 //   SM_FIRST   : the zero-start first step c0 D^{-1} b, recomputed at every
 //                stencil point (b read from r64 and rounded when r64 != null,
 //                which also stores b); the previous direction equals it
 //   SM_PROLONG : xin + P e (piecewise-constant coarse correction)
 //   otherwise  : xin
+// Every lane of a row takes part in the shuffles, so lanes past the grid
+// edge stay in the loop with their loads and stores masked.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
                                                    const double *__restrict__ r64,
@@ -78,9 +82,11 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
     constexpr bool kProlong = (MODE & SM_PROLONG) != 0;
     int k, j, i0;
     cell_tile(cg, k, j, i0);
-    if (k >= d.nz || j >= d.ny) return;
+    const int tz = blockDim.x;
+    const int lane = threadIdx.x;  // position in the tile row
+    const bool on = k < d.nz && j < d.ny;
+    if (i0 >= d.nx) return;  // uniform over the block
     const Strides s(d);
-    const Strides sc(dc);
     auto bval = [&](size_t q) -> float { return r64 ? (float)r64[q] : b[q]; };
     // iterate value at a neighbour point q (cell coordinates qi, qj, qk)
     auto xval = [&](size_t q, int qi, int qj, int qk) -> float {
@@ -93,78 +99,131 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
         }
     };
     const int ie = min(i0 + cg.planes, d.nx);
-    size_t c = d.idx(i0, j, k);
-    float xm = i0 > 0 ? xval(c - s.sx, i0 - 1, j, k) : 0.f;
-    float xc = xval(c, i0, j, k);
-    float cxc = A.cpl[0][c];
+    const bool zlo = k > 0, zhi = k + 1 < d.nz;
+    size_t c = on ? d.idx(i0, j, k) : 0;
+    float xm = (on && i0 > 0) ? xval(c - s.sx, i0 - 1, j, k) : 0.f;
+    float xc = on ? xval(c, i0, j, k) : 0.f;
+    float cxc = on ? A.cpl[0][c] : 0.f;
+    const bool ylo = on && j > 0, yhi = on && j + 1 < d.ny;
     for (int i = i0; i < ie; i++, c += s.sx) {
+        // every load of the plane is issued before the lane exchange
         const bool up = i + 1 < d.nx;
-        const float xp = up ? xval(c + s.sx, i + 1, j, k) : 0.f;
-        const float cxp = up ? A.cpl[0][c + s.sx] : 0.f;
-        const float dg = A.diag[c];
-        float off = cxc * xm + cxp * xp;
-        if (j > 0) off += A.cpl[1][c] * xval(c - s.sy, i, j - 1, k);
-        if (j + 1 < d.ny) off += A.cpl[1][c + s.sy] * xval(c + s.sy, i, j + 1, k);
-        if (k > 0) off += A.cpl[2][c] * xval(c - 1, i, j, k - 1);
-        if (k + 1 < d.nz) off += A.cpl[2][c + 1] * xval(c + 1, i, j, k + 1);
-        float bc;
+        const float xp = (on && up) ? xval(c + s.sx, i + 1, j, k) : 0.f;
+        const float cxp = (on && up) ? A.cpl[0][c + s.sx] : 0.f;
+        const float czm = on ? A.cpl[2][c] : 0.f;
+        const float cym = ylo ? A.cpl[1][c] : 0.f;
+        const float xym = ylo ? xval(c - s.sy, i, j - 1, k) : 0.f;
+        const float cyp = yhi ? A.cpl[1][c + s.sy] : 0.f;
+        const float xyp = yhi ? xval(c + s.sy, i, j + 1, k) : 0.f;
+        const float dg = on ? A.diag[c] : 0.f;
+        float bc = 0.f;
         if constexpr (kFirst) {
-            bc = bval(c);
-            if (r64) b[c] = bc;
+            bc = on ? bval(c) : 0.f;
         } else {
-            bc = b[c];
+            bc = on ? b[c] : 0.f;
         }
-        // centre iterate: the coarse correction only lands on active cells
-        float x0 = xc;
-        if constexpr (kProlong) {
-            if (!(dg > 0.f)) x0 = xin[c];
-        }
-        const float r = bc - (dg * x0 - off);
         float dprev = 0.f;
-        if constexpr (kFirst) dprev = x0;
-        if constexpr ((MODE & SM_READ_D) != 0) dprev = dvec[c];
-        const float dn = cd * dprev + cr * rcp_or_zero(dg) * r;
-        xout[c] = x0 + dn;
-        if constexpr ((MODE & SM_WRITE_D) != 0) dvec[c] = dn;
+        if constexpr ((MODE & SM_READ_D) != 0) dprev = on ? dvec[c] : 0.f;
+        float xin_c = 0.f;
+        if constexpr (kProlong) xin_c = on ? xin[c] : 0.f;
+        float xzm = __shfl_up(xc, 1, tz);
+        float xzp = __shfl_down(xc, 1, tz);
+        float czp = __shfl_down(czm, 1, tz);
+        if (lane == 0) xzm = (on && zlo) ? xval(c - 1, i, j, k - 1) : 0.f;
+        if (lane == tz - 1) {
+            xzp = (on && zhi) ? xval(c + 1, i, j, k + 1) : 0.f;
+            czp = (on && zhi) ? A.cpl[2][c + 1] : 0.f;
+        }
+        if (on) {
+            float off = cxc * xm + cxp * xp;
+            if (j > 0) off += cym * xym;
+            if (j + 1 < d.ny) off += cyp * xyp;
+            if (zlo) off += czm * xzm;
+            if (zhi) off += czp * xzp;
+            if constexpr (kFirst) {
+                if (r64) b[c] = bc;
+            }
+            // centre iterate: the coarse correction only lands on active cells
+            float x0 = xc;
+            if constexpr (kProlong) {
+                if (!(dg > 0.f)) x0 = xin_c;
+            }
+            const float r = bc - (dg * x0 - off);
+            if constexpr (kFirst) dprev = x0;
+            const float dn = cd * dprev + cr * rcp_or_zero(dg) * r;
+            xout[c] = x0 + dn;
+            if constexpr ((MODE & SM_WRITE_D) != 0) dvec[c] = dn;
+        }
         xm = xc;
         xc = xp;
         cxc = cxp;
     }
 }
 
+// Residual b - A x on the fine grid, summed over each coarse cell's (up to 8)
+// children. Threads map to fine cells like the smoother (coalesced rows,
+// x-neighbours in registers, z-neighbours from adjacent lanes); the children
+// are added pairwise: z through the lane pair, y through LDS (rows 2m and
+// 2m+1 of a block), x over the block's plane pairs. The block's plane count
+// is even and starts on an even plane, and tz, ty are even (or tz = 1).
 __global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 A, const float *__restrict__ b,
                                                      const float *__restrict__ x, Dims dc,
                                                      float *__restrict__ bc) {
-    int K, J, I0;
-    cell_tile(cg, K, J, I0);
-    if (K >= dc.nz || J >= dc.ny) return;
+    __shared__ float sh[kBlock];
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
+    const int tz = blockDim.x, lane = threadIdx.x, row = threadIdx.y;
+    if (i0 >= df.nx) return;  // uniform over the block
+    const bool on = k < df.nz && j < df.ny;
+    const bool writer = on && (k & 1) == 0 && (j & 1) == 0;
     const Strides s(df);
-    for (int I = I0, Ie = min(I0 + cg.planes, dc.nx); I < Ie; I++) {
-        float sum = 0.f;
-        for (int di = 0; di < 2; di++) {
-            const int i = 2 * I + di;
-            if (i >= df.nx) break;
-            for (int dj = 0; dj < 2; dj++) {
-                const int j = 2 * J + dj;
-                if (j >= df.ny) break;
-                for (int dk = 0; dk < 2; dk++) {
-                    const int k = 2 * K + dk;
-                    if (k >= df.nz) break;
-                    const size_t c = df.idx(i, j, k);
-                    const float dg = A.diag[c];
-                    if (!(dg > 0.f)) continue;
-                    float off = 0.f;
-                    if (i > 0) off += A.cpl[0][c] * x[c - s.sx];
-                    if (i + 1 < df.nx) off += A.cpl[0][c + s.sx] * x[c + s.sx];
-                    if (j > 0) off += A.cpl[1][c] * x[c - s.sy];
-                    if (j + 1 < df.ny) off += A.cpl[1][c + s.sy] * x[c + s.sy];
-                    if (k > 0) off += A.cpl[2][c] * x[c - 1];
-                    if (k + 1 < df.nz) off += A.cpl[2][c + 1] * x[c + 1];
-                    sum += b[c] - (dg * x[c] - off);
-                }
+    const int ie = min(i0 + cg.planes, df.nx);
+    const bool zlo = k > 0, zhi = k + 1 < df.nz;
+    size_t c = on ? df.idx(i0, j, k) : 0;
+    float xm = (on && i0 > 0) ? x[c - s.sx] : 0.f;
+    float xc = on ? x[c] : 0.f;
+    float cxc = on ? A.cpl[0][c] : 0.f;
+    float acc = 0.f;
+    for (int i = i0; i < ie; i++, c += s.sx) {
+        const bool up = i + 1 < df.nx;
+        const float xp = (on && up) ? x[c + s.sx] : 0.f;
+        const float cxp = (on && up) ? A.cpl[0][c + s.sx] : 0.f;
+        const float czm = on ? A.cpl[2][c] : 0.f;
+        float xzm = __shfl_up(xc, 1, tz);
+        float xzp = __shfl_down(xc, 1, tz);
+        float czp = __shfl_down(czm, 1, tz);
+        if (lane == 0) xzm = (on && zlo) ? x[c - 1] : 0.f;
+        if (lane == tz - 1) {
+            xzp = (on && zhi) ? x[c + 1] : 0.f;
+            czp = (on && zhi) ? A.cpl[2][c + 1] : 0.f;
+        }
+        float r = 0.f;
+        if (on) {
+            const float dg = A.diag[c];
+            if (dg > 0.f) {
+                float off = 0.f;
+                if (i > 0) off += cxc * xm;
+                if (up) off += cxp * xp;
+                if (j > 0) off += A.cpl[1][c] * x[c - s.sy];
+                if (j + 1 < df.ny) off += A.cpl[1][c + s.sy] * x[c + s.sy];
+                if (zlo) off += czm * xzm;
+                if (zhi) off += czp * xzp;
+                r = b[c] - (dg * xc - off);
             }
         }
-        bc[dc.idx(I, J, K)] = sum;
+        if (tz > 1) r += __shfl_xor(r, 1, tz);  // z pair
+        sh[row * tz + lane] = r;
+        __syncthreads();
+        if ((row & 1) == 0) r += sh[(row + 1) * tz + lane];  // y pair
+        __syncthreads();
+        acc += r;  // x pair
+        if ((i & 1) == 1 || !up) {
+            if (writer) bc[dc.idx(i >> 1, j >> 1, k >> 1)] = acc;
+            acc = 0.f;
+        }
+        xm = xc;
+        xc = xp;
+        cxc = cxp;
     }
 }
 
@@ -323,7 +382,9 @@ void launch_smooth_first(hipStream_t st, Dims d, Op32 A, const double *r64, floa
 void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, float *b,
                    const float *xin, float *xout, float *dvec, float c0, float cd, float cr,
                    const float *ecoarse) {
-    Launch L = cell_launch(d);
+    // 8 planes per block measured best for the shuffle-based step (256^3:
+    // 122 us vs 124 us at 16 and 123 us at 4)
+    Launch L = cell_launch(d, 0, 8);
     Dims dc = d.coarse();
 #define SFP_SMOOTH_CASE(M)                                                                    \
     case M:                                                                                   \
@@ -348,7 +409,8 @@ void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, 
 
 void launch_restrict(hipStream_t st, Dims df, Op32 A, const float *b, const float *x, float *bc) {
     Dims dc = df.coarse();
-    Launch L = cell_launch(dc, 1);
+    Launch L = cell_launch(df, 0, 8);
+    if (L.cg.planes & 1) L = cell_launch(df, L.cg.planes + 1);  // plane pairs stay in one block
     hipLaunchKernelGGL(k_restrict, L.grid, L.block, 0, st, L.cg, df, A, b, x, dc, bc);
     SFP_LAUNCH_CHECK();
 }

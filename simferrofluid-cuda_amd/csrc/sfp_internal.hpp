@@ -65,7 +65,8 @@ struct Launch {
     unsigned blocks() const { return grid.x; }
 };
 // planes <= 0 picks the number of x planes per block from the grid size.
-Launch cell_launch(Dims d, int planes = 0);
+// planes <= 0: choose up to max_planes planes per block, keeping >= ~2048 blocks
+Launch cell_launch(Dims d, int planes = 0, int max_planes = 16);
 
 // ------------------------------------------------------------------ device memory
 template <class T>
