@@ -143,6 +143,35 @@ sfp_status sfp_magnetic_vertex_pressures(sfp_context ctx, int nverts, const doub
                                          const double *normals, double susceptibility,
                                          double *pressures);
 
+/* ---- after the projection ---------------------------------------------- */
+
+/* Extrapolation::Solve on the three face grids (core/Extrapolation.h:27-37,
+ * core/Extrapolation.cpp:4-30): `steps` sweeps; in each, every invalid face
+ * with at least one valid neighbour on its face grid (-x,+x,-y,+y,-z,+z)
+ * takes their mean and becomes valid, every other invalid face is set to
+ * clear_value. valid[a]: one byte per face of grid a, nonzero = valid (read
+ * only). */
+sfp_status sfp_extrapolate_faces(sfp_context ctx, double *const data[3],
+                                 const unsigned char *const valid[3], double clear_value,
+                                 int steps);
+
+/* The velocity extrapolation of Simulation::ProjectVelocity
+ * (core/Simulation.cpp:236-240): sfp_extrapolate_faces with clear value 0 and
+ * valid = fraction < 1 and level_set <= 0 in a cell on either side of the
+ * face (a side outside the grid counts as air). */
+sfp_status sfp_extrapolate_velocity(sfp_context ctx, double *const velocity[3],
+                                    const double *level_set, const double *const fraction[3],
+                                    int steps);
+
+/* Collider::Enforce (core/Collider.cpp:75-86): at every face whose solid
+ * fraction is exactly 1, velocity becomes vel - ((vel - vel0) . n) n with vel,
+ * vel0 = collider_velocity and n = normal (normalized) trilinearly
+ * interpolated at the face (core/TriLerp.h), from the velocity before the
+ * call. normal[a]: Collider::GetNormal(). */
+sfp_status sfp_collider_enforce(sfp_context ctx, double *const velocity[3],
+                                const double *const fraction[3], const double *const normal[3],
+                                const double *const collider_velocity[3]);
+
 /* ---- device-resident variants ------------------------------------------ */
 /* Same semantics; every array pointer is a device pointer, work is enqueued
  * on sfp_stream(ctx) and the call returns after the solve finished. */
@@ -161,6 +190,13 @@ sfp_status sfp_magnetic_solve_device(sfp_context ctx, const double *level_set,
                                      const double *const external_field[3],
                                      double susceptibility, double band_width, double *potential,
                                      double *const total_field[3], sfp_stats *stats);
+sfp_status sfp_extrapolate_velocity_device(sfp_context ctx, double *const velocity[3],
+                                           const double *level_set,
+                                           const double *const fraction[3], int steps);
+sfp_status sfp_collider_enforce_device(sfp_context ctx, double *const velocity[3],
+                                       const double *const fraction[3],
+                                       const double *const normal[3],
+                                       const double *const collider_velocity[3]);
 
 #ifdef __cplusplus
 }

@@ -381,3 +381,112 @@ void orc_magnetic_vertex_pressures(int nx, int ny, int nz, double spacing, const
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- after the projection
+
+// Simulation::ProjectVelocity's extrapolation mask (core/Simulation.cpp:236-240):
+// fraction < 1 and a liquid cell on either side (outside the grid: not liquid).
+void orc_velocity_valid(int nx, int ny, int nz, const double *phi, const double *const frac[3],
+                        unsigned char *const valid[3]) {
+    const Dims d{{nx, ny, nz}};
+    for (int axis = 0; axis < 3; axis++) {
+        const Dims fd = face_dims(d, axis);
+        const int64_t nf = (int64_t)fd.n[0] * fd.n[1] * fd.n[2];
+        for (int64_t q = 0; q < nf; q++) {
+            int f[3];
+            coord_of(fd, q, f);
+            bool v = false;
+            if (frac[axis][q] < 1) {
+                int c0[3] = {f[0], f[1], f[2]};
+                c0[axis] -= 1;  // StaggeredGrid::AdjCellOfFace(axis, face, 0)
+                const bool liq0 = d.valid(c0[0], c0[1], c0[2]) && phi[d.index(c0[0], c0[1], c0[2])] <= 0;
+                const bool liq1 = d.valid(f[0], f[1], f[2]) && phi[d.index(f[0], f[1], f[2])] <= 0;
+                v = liq0 || liq1;
+            }
+            valid[axis][q] = v ? 1 : 0;
+        }
+    }
+}
+
+// Extrapolation::Solve on the three face grids (core/Extrapolation.h:27-37,
+// core/Extrapolation.cpp:4-30); valid is copied, not modified.
+void orc_extrapolate_faces(int nx, int ny, int nz, double *const data[3],
+                           const unsigned char *const valid_in[3], double clear, int steps) {
+    const Dims d{{nx, ny, nz}};
+    for (int axis = 0; axis < 3; axis++) {
+        const Dims g = face_dims(d, axis);
+        const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+        std::vector<unsigned char> valid(valid_in[axis], valid_in[axis] + n), valid1 = valid;
+        double *gr = data[axis];
+        for (int iter = 0; iter < steps; iter++) {
+            for (int64_t q = 0; q < n; q++) {
+                if (!valid[q]) {
+                    int c[3];
+                    coord_of(g, q, c);
+                    int cnt = 0;
+                    double sum = 0;
+                    for (int ord = 0; ord < 6; ord++) {
+                        int nb[3] = {c[0], c[1], c[2]};
+                        nb[axis_of(ord)] += side_of(ord);
+                        if (g.valid(nb[0], nb[1], nb[2]) && valid[g.index(nb[0], nb[1], nb[2])]) {
+                            sum += gr[g.index(nb[0], nb[1], nb[2])];
+                            cnt++;
+                        }
+                    }
+                    if (cnt > 0) {
+                        gr[q] = sum / cnt;
+                        valid1[q] = 1;
+                    } else {
+                        gr[q] = clear;
+                    }
+                } else {
+                    valid1[q] = 1;
+                }
+            }
+            valid.swap(valid1);
+        }
+    }
+}
+
+// Collider::Enforce (core/Collider.cpp:75-86) on the simulation's staggered
+// grid: origin is StaggeredGrid::GetOrigin() (core/StaggeredGrid.h:13),
+// face grid a starts at origin + (1 - e_a) * spacing / 2.
+void orc_collider_enforce(int nx, int ny, int nz, double spacing, const double origin[3],
+                          double *const vel[3], const double *const frac[3],
+                          const double *const normal[3], const double *const colvel[3]) {
+    const Dims d{{nx, ny, nz}};
+    const double inv = 1 / spacing;
+    double forg[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int c = 0; c < 3; c++) forg[a][c] = origin[c] + ((c == a ? 0.0 : 1.0) * spacing) / 2;
+    std::vector<double> old[3];
+    for (int a = 0; a < 3; a++) {
+        const Dims fd = face_dims(d, a);
+        old[a].assign(vel[a], vel[a] + (int64_t)fd.n[0] * fd.n[1] * fd.n[2]);
+    }
+    for (int axis = 0; axis < 3; axis++) {
+        const Dims fd = face_dims(d, axis);
+        const int64_t nf = (int64_t)fd.n[0] * fd.n[1] * fd.n[2];
+        for (int64_t q = 0; q < nf; q++) {
+            if (!(frac[axis][q] == 1.)) continue;
+            int f[3];
+            coord_of(fd, q, f);
+            double pos[3];
+            for (int a = 0; a < 3; a++) pos[a] = forg[axis][a] + (double)f[a] * spacing;
+            double v[3], v0[3], n[3];
+            for (int b = 0; b < 3; b++) {
+                const Dims gb = face_dims(d, b);
+                v[b] = trilerp(gb, forg[b], spacing, inv, old[b].data(), pos);
+                v0[b] = trilerp(gb, forg[b], spacing, inv, colvel[b], pos);
+                n[b] = trilerp(gb, forg[b], spacing, inv, normal[b], pos);
+            }
+            const double z = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];  // Eigen normalized()
+            if (z > 0) {
+                const double r = std::sqrt(z);
+                for (int b = 0; b < 3; b++) n[b] = n[b] / r;
+            }
+            const double dot = (v[0] - v0[0]) * n[0] + (v[1] - v0[1]) * n[1] + (v[2] - v0[2]) * n[2];
+            vel[axis][q] = v[axis] - dot * n[axis];
+        }
+    }
+}

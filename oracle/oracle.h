@@ -104,6 +104,19 @@ double orc_unet_forward(int batch, int nx, int ny, int nz, const double *params,
 double orc_unet_gradients(int batch, int nx, int ny, int nz, const double *params, const double *input,
                           const double *target, const double *mask, double *grad);
 
+/* ---- after the projection (core/Simulation.cpp:236-241) ----
+ * Extrapolation mask of Simulation::ProjectVelocity: valid[a][face] = 1 when
+ * fraction < 1 and a cell on either side has phi <= 0 (outside: air). */
+void orc_velocity_valid(int nx, int ny, int nz, const double *phi, const double *const frac[3],
+                        unsigned char *const valid[3]);
+/* Extrapolation::Solve on the three face grids, in place on data. */
+void orc_extrapolate_faces(int nx, int ny, int nz, double *const data[3],
+                           const unsigned char *const valid[3], double clear, int steps);
+/* Collider::Enforce, in place on vel; origin = StaggeredGrid::GetOrigin(). */
+void orc_collider_enforce(int nx, int ny, int nz, double spacing, const double origin[3],
+                          double *const vel[3], const double *const frac[3],
+                          const double *const normal[3], const double *const colvel[3]);
+
 /* Number of OpenMP threads the solvers use. */
 int orc_num_threads(void);
 

@@ -65,6 +65,13 @@ def lib():
             f = getattr(L, name)
             f.restype = C.c_double
             f.argtypes = [C.c_int] * 4 + [_dp, _dp, C.c_void_p, C.c_void_p, _dp]
+        U3 = C.POINTER(C.c_ubyte) * 3
+        L.orc_velocity_valid.restype = None
+        L.orc_velocity_valid.argtypes = [C.c_int] * 3 + [_dp, P3, U3]
+        L.orc_extrapolate_faces.restype = None
+        L.orc_extrapolate_faces.argtypes = [C.c_int] * 3 + [P3, U3, C.c_double, C.c_int]
+        L.orc_collider_enforce.restype = None
+        L.orc_collider_enforce.argtypes = [C.c_int] * 3 + [C.c_double, _dp, P3, P3, P3, P3]
         L.orc_num_threads.restype = C.c_int
         _LIB = L
     return _LIB
@@ -218,6 +225,44 @@ def magnetic_vertex_pressures(sc, htotal, positions, normals):
     lib().orc_magnetic_vertex_pressures(nx, ny, nz, geo.spacing, _d(org), hp, float(sc["chi"]),
                                         nv, _d(positions), _d(normals), _d(out))
     return out[:nv]
+
+
+# ---------------------------------------------------------------- after the projection
+def _u3(arrs):
+    arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in arrs]
+    return (C.POINTER(C.c_ubyte) * 3)(*[a.ctypes.data_as(C.POINTER(C.c_ubyte)) for a in arrs]), arrs
+
+
+def velocity_valid(sc):
+    """Extrapolation mask of Simulation::ProjectVelocity, per face grid (uint8)."""
+    geo = sc["geo"]
+    nx, ny, nz = geo.n
+    out = [np.zeros(geo.face_shape(a), np.uint8) for a in range(3)]
+    fp, k1 = _p3(sc["fraction"])
+    up, k2 = _u3(out)
+    lib().orc_velocity_valid(nx, ny, nz, _d(np.ascontiguousarray(sc["phi"], dtype=np.float64)), fp, up)
+    return k2
+
+
+def extrapolate_faces(geo, data, valid, clear=0.0, steps=6):
+    """Extrapolation::Solve on the three face grids; returns new arrays."""
+    nx, ny, nz = geo.n
+    dp, arrs = _p3([np.array(a, dtype=np.float64, copy=True) for a in data])
+    vp, k = _u3(valid)
+    lib().orc_extrapolate_faces(nx, ny, nz, dp, vp, float(clear), int(steps))
+    return arrs
+
+
+def collider_enforce(geo, vel, frac, normal, colvel):
+    """Collider::Enforce; returns new velocity arrays."""
+    nx, ny, nz = geo.n
+    vp, arrs = _p3([np.array(a, dtype=np.float64, copy=True) for a in vel])
+    fp, k1 = _p3(frac)
+    npp, k2 = _p3(normal)
+    cp, k3 = _p3(colvel)
+    org = np.array(geo.origin, dtype=np.float64)
+    lib().orc_collider_enforce(nx, ny, nz, geo.spacing, _d(org), vp, fp, npp, cp)
+    return arrs
 
 
 # ---------------------------------------------------------------- solvers

@@ -243,9 +243,135 @@ sfp_status pressure_host(sfp_context ctx, double *const vel[3], const double *ph
     });
 }
 
+// ---------------------------------------------------------------- post-projection
+void check_steps(int steps) {
+    if (steps < 0) throw ArgError("steps must be >= 0");
+}
+
+void extrapolate_velocity_dev(sfp_context ctx, double *const vel[3], const double *phi,
+                              const double *const frac[3], int steps) {
+    require3((const void *const *)vel, "velocity");
+    require3((const void *const *)frac, "fraction");
+    if (!phi) throw ArgError("level_set is null");
+    check_steps(steps);
+    const hipStream_t s = ctx->ws.stream;
+    const Dims d = ctx->d;
+    for (int a = 0; a < 3; a++) {
+        const Dims fd = d.face(a);
+        ctx->post_valid[0][a].ensure(fd.cells());
+        ctx->post_valid[1][a].ensure(fd.cells());
+        launch_velocity_valid(s, d, a, phi, frac[a], ctx->post_valid[0][a].p);
+        launch_extrapolate(s, fd, vel[a], ctx->post_valid[0][a].p, ctx->post_valid[1][a].p, 0.0, steps);
+    }
+    SFP_HIP(hipStreamSynchronize(s));
+}
+
+void collider_enforce_dev(sfp_context ctx, double *const vel[3], const double *const frac[3],
+                          const double *const normal[3], const double *const colvel[3]) {
+    require3((const void *const *)vel, "velocity");
+    require3((const void *const *)frac, "fraction");
+    require3((const void *const *)normal, "normal");
+    require3((const void *const *)colvel, "collider_velocity");
+    const hipStream_t s = ctx->ws.stream;
+    const Dims d = ctx->d;
+    for (int a = 0; a < 3; a++) {
+        const size_t n = face_count(d, a);
+        ctx->post_old[a].ensure(n);
+        SFP_HIP(hipMemcpyAsync(ctx->post_old[a].p, vel[a], n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    const Faces old{{ctx->post_old[0].p, ctx->post_old[1].p, ctx->post_old[2].p}};
+    launch_collider_enforce(s, d, ctx->spacing, ctx->origin, old, Faces{{frac[0], frac[1], frac[2]}},
+                            Faces{{normal[0], normal[1], normal[2]}},
+                            Faces{{colvel[0], colvel[1], colvel[2]}}, MutFaces{{vel[0], vel[1], vel[2]}});
+    SFP_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace
 
 extern "C" {
+
+sfp_status sfp_extrapolate_faces(sfp_context ctx, double *const data[3],
+                                 const unsigned char *const valid[3], double clear_value,
+                                 int steps) {
+    return guarded(ctx, [&] {
+        require3((const void *const *)data, "data");
+        require3((const void *const *)valid, "valid");
+        check_steps(steps);
+        const hipStream_t s = ctx->ws.stream;
+        const Dims d = ctx->d;
+        for (int a = 0; a < 3; a++) {
+            const Dims fd = d.face(a);
+            const size_t n = fd.cells();
+            double *dd = const_cast<double *>(up(ctx, ctx->s_face[0][a], data[a], n));
+            ctx->post_valid[0][a].ensure(n);
+            ctx->post_valid[1][a].ensure(n);
+            SFP_HIP(hipMemcpyAsync(ctx->post_valid[0][a].p, valid[a], n, hipMemcpyHostToDevice, s));
+            launch_extrapolate(s, fd, dd, ctx->post_valid[0][a].p, ctx->post_valid[1][a].p, clear_value,
+                               steps);
+            down(ctx, data[a], dd, n);
+        }
+        SFP_HIP(hipStreamSynchronize(s));
+    });
+}
+
+sfp_status sfp_extrapolate_velocity(sfp_context ctx, double *const velocity[3],
+                                    const double *level_set, const double *const fraction[3],
+                                    int steps) {
+    return guarded(ctx, [&] {
+        require3((const void *const *)velocity, "velocity");
+        require3((const void *const *)fraction, "fraction");
+        if (!level_set) throw ArgError("level_set is null");
+        const Dims d = ctx->d;
+        double *dv[3];
+        const double *df[3];
+        for (int a = 0; a < 3; a++) {
+            const size_t n = face_count(d, a);
+            dv[a] = const_cast<double *>(up(ctx, ctx->s_face[0][a], velocity[a], n));
+            df[a] = up(ctx, ctx->s_face[1][a], fraction[a], n);
+        }
+        const double *dphi = up(ctx, ctx->s_phi, level_set, d.cells());
+        extrapolate_velocity_dev(ctx, dv, dphi, df, steps);
+        for (int a = 0; a < 3; a++) down(ctx, velocity[a], dv[a], face_count(d, a));
+        SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
+    });
+}
+
+sfp_status sfp_collider_enforce(sfp_context ctx, double *const velocity[3],
+                                const double *const fraction[3], const double *const normal[3],
+                                const double *const collider_velocity[3]) {
+    return guarded(ctx, [&] {
+        require3((const void *const *)velocity, "velocity");
+        require3((const void *const *)fraction, "fraction");
+        require3((const void *const *)normal, "normal");
+        require3((const void *const *)collider_velocity, "collider_velocity");
+        const Dims d = ctx->d;
+        double *dv[3];
+        const double *df[3], *dn[3], *dc[3];
+        for (int a = 0; a < 3; a++) {
+            const size_t n = face_count(d, a);
+            dv[a] = const_cast<double *>(up(ctx, ctx->s_face[0][a], velocity[a], n));
+            df[a] = up(ctx, ctx->s_face[1][a], fraction[a], n);
+            dn[a] = up(ctx, ctx->s_face[2][a], normal[a], n);
+            dc[a] = up(ctx, ctx->s_face[3][a], collider_velocity[a], n);
+        }
+        collider_enforce_dev(ctx, dv, df, dn, dc);
+        for (int a = 0; a < 3; a++) down(ctx, velocity[a], dv[a], face_count(d, a));
+        SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
+    });
+}
+
+sfp_status sfp_extrapolate_velocity_device(sfp_context ctx, double *const velocity[3],
+                                           const double *level_set,
+                                           const double *const fraction[3], int steps) {
+    return guarded(ctx, [&] { extrapolate_velocity_dev(ctx, velocity, level_set, fraction, steps); });
+}
+
+sfp_status sfp_collider_enforce_device(sfp_context ctx, double *const velocity[3],
+                                       const double *const fraction[3],
+                                       const double *const normal[3],
+                                       const double *const collider_velocity[3]) {
+    return guarded(ctx, [&] { collider_enforce_dev(ctx, velocity, fraction, normal, collider_velocity); });
+}
 
 const char *sfp_status_string(sfp_status s) {
     switch (s) {

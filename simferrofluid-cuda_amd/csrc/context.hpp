@@ -28,6 +28,9 @@ struct sfp_context_s {
     sfp::DevBuf<double> s_phi, s_cell;
     sfp::DevBuf<double> s_face[5][3];   // velocity, fraction, solid velocity, jump, field
     sfp::DevBuf<double> s_vert[3];      // positions, normals, pressures
+    // post-projection steps: velocity before Collider::Enforce, validity masks
+    sfp::DevBuf<double> post_old[3];
+    sfp::DevBuf<unsigned char> post_valid[2][3];
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::unique_ptr<sfp::UNet> net;  // learned preconditioner weights, when given
     std::string last_error;

@@ -60,6 +60,17 @@ struct Faces {
 struct MutFaces {
     double *a[3];
 };
+// Post-projection steps (post.hip).
+// valid[f] (face grid axis) = fraction < 1 and a liquid cell on either side.
+void launch_velocity_valid(hipStream_t s, Dims d, int axis, const double *phi, const double *frac,
+                           unsigned char *valid);
+// `steps` sweeps of Extrapolation::Solve on one grid; valid is updated,
+// scratch (same size) is overwritten.
+void launch_extrapolate(hipStream_t s, Dims g, double *data, unsigned char *valid,
+                        unsigned char *scratch, double clear, int steps);
+// Collider::Enforce; old is the velocity before the call (not aliasing vel).
+void launch_collider_enforce(hipStream_t s, Dims d, double spacing, const double origin[3], Faces old,
+                             Faces frac, Faces normal, Faces colvel, MutFaces vel);
 // Geometry of MagneticField's own StaggeredGrid (core/MagneticField.cpp:11-16).
 struct MagGeom {
     double face_origin[3][3];

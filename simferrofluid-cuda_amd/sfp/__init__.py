@@ -32,7 +32,9 @@ EXPORTS = {
               "sfp_reset_warm_start", "sfp_pressure_project", "sfp_pressure_reproject",
               "sfp_magnetic_solve", "sfp_magnetic_vertex_pressures",
               "sfp_pressure_project_device", "sfp_pressure_reproject_device",
-              "sfp_magnetic_solve_device"],
+              "sfp_magnetic_solve_device", "sfp_extrapolate_faces", "sfp_extrapolate_velocity",
+              "sfp_collider_enforce", "sfp_extrapolate_velocity_device",
+              "sfp_collider_enforce_device"],
     "sfp_profile.h": ["sfp_profile_enable", "sfp_profile_query"],
     "sfp_test.h": ["sfp_test_num_levels", "sfp_test_level_dims", "sfp_test_export_level",
                    "sfp_test_export_level32", "sfp_test_apply64", "sfp_test_precondition",
@@ -109,6 +111,15 @@ def lib():
             f = getattr(L, name)
             f.restype = C.c_int
             f.argtypes = [vp, _dp, PP, C.c_double, C.c_double, _dp, vp, C.POINTER(Stats)]
+        U3 = C.POINTER(C.POINTER(C.c_ubyte))
+        L.sfp_extrapolate_faces.restype = C.c_int
+        L.sfp_extrapolate_faces.argtypes = [vp, PP, U3, C.c_double, C.c_int]
+        for name in ("sfp_extrapolate_velocity", "sfp_extrapolate_velocity_device"):
+            getattr(L, name).restype = C.c_int
+            getattr(L, name).argtypes = [vp, PP, _dp, PP, C.c_int]
+        for name in ("sfp_collider_enforce", "sfp_collider_enforce_device"):
+            getattr(L, name).restype = C.c_int
+            getattr(L, name).argtypes = [vp, PP, PP, PP, PP]
         L.sfp_magnetic_vertex_pressures.restype = C.c_int
         L.sfp_magnetic_vertex_pressures.argtypes = [vp, C.c_int, _dp, _dp, C.c_double, _dp]
         L.sfp_profile_enable.restype = C.c_int
@@ -295,6 +306,39 @@ class Context:
         self._check(lib().sfp_magnetic_vertex_pressures(self._h, nv, _d(positions), _d(normals),
                                                         float(chi), _d(out)))
         return out[:nv]
+
+    # ------------------------------------------------------------ after the projection
+    def extrapolate_faces(self, data, valid, clear_value=0.0, steps=6):
+        """Extrapolation::Solve on the three face grids; returns new arrays."""
+        out = [np.array(a, dtype=np.float64, copy=True, order="C") for a in data]
+        masks = [np.ascontiguousarray(m, dtype=np.uint8) for m in valid]
+        mp = (C.POINTER(C.c_ubyte) * 3)(*[m.ctypes.data_as(C.POINTER(C.c_ubyte)) for m in masks])
+        self._check(lib().sfp_extrapolate_faces(self._h, _p3(out), mp, float(clear_value), int(steps)))
+        return out
+
+    def extrapolate_velocity(self, vel, phi, fraction, steps=6):
+        """The velocity extrapolation of Simulation::ProjectVelocity; returns new arrays."""
+        out = [np.array(a, dtype=np.float64, copy=True, order="C") for a in vel]
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        frac = [np.ascontiguousarray(a, dtype=np.float64) for a in fraction]
+        self._check(lib().sfp_extrapolate_velocity(self._h, _p3(out), _d(phi), _p3(frac), int(steps)))
+        return out
+
+    def collider_enforce(self, vel, fraction, normal, collider_velocity):
+        """Collider::Enforce; returns new arrays."""
+        out = [np.array(a, dtype=np.float64, copy=True, order="C") for a in vel]
+        arrs = [[np.ascontiguousarray(a, dtype=np.float64) for a in g]
+                for g in (fraction, normal, collider_velocity)]
+        self._check(lib().sfp_collider_enforce(self._h, _p3(out), *[_p3(g) for g in arrs]))
+        return out
+
+    def extrapolate_velocity_device(self, vel, phi, fraction, steps=6):
+        self._check(lib().sfp_extrapolate_velocity_device(self._h, _p3(vel), _d(phi), _p3(fraction),
+                                                          int(steps)))
+
+    def collider_enforce_device(self, vel, fraction, normal, collider_velocity):
+        self._check(lib().sfp_collider_enforce_device(self._h, _p3(vel), _p3(fraction), _p3(normal),
+                                                      _p3(collider_velocity)))
 
     # ------------------------------------------------------------ device API (int pointers)
     def pressure_project_device(self, vel, phi, frac, us, jump, vol_error, pressure=None,

@@ -1,8 +1,11 @@
 // Drives the drop-in Pressure and MagneticField classes the way the
-// reference's Simulation does (core/Simulation.cpp:193-243) and checks every
-// result against the FP64 oracle. Exit status 0 = all checks passed.
+// reference's Simulation does (core/Simulation.cpp:193-243), then the whole
+// ProjectionStep (magnetic pressure, jump, projection, extrapolation, wall
+// condition), and checks every result against the FP64 oracle.
+// Exit status 0 = all checks passed.
 #include "MagneticField.h"
 #include "Pressure.h"
+#include "Simulation.h"
 #include "oracle.h"
 
 #include <cmath>
@@ -228,6 +231,105 @@ int main() {
         verr = std::max(verr, std::fabs(vpr[v] - vp[v]));
     }
     check(verr <= 8 * 2.2e-16 * vmax, "vertex pressures match the oracle to 8 ulp of the max", verr / vmax);
+
+    // ---------------------------------------------------------------- ProjectionStep
+    std::printf("ProjectionStep::ProjectVelocity (Simulation::ProjectVelocity)\n");
+    // a contour with one vertex on every cell-centre edge the interface crosses
+    Contour contour(cg);
+    SurfaceMesh &cm = contour.GetMesh();
+    for (int a = 0; a < 3; a++) {
+        Grid const &eg = contour.GetEdgeGrid(a);
+        ForEach(eg, [&](Vector3i const &e) {
+            Vector3i const e1 = e + Vector3i::Unit(a);
+            double const p0 = levelSet[e], p1 = levelSet[e1];
+            if ((p0 <= 0) == (p1 <= 0)) return;
+            double const t = p0 / (p0 - p1);
+            Vector3d const x0 = cg.PositionOf(e);
+            Vector3d pos = x0;
+            pos[a] += t * sgrid.GetSpacing();
+            contour.EdgeMark(a)[e] = cm.size();
+            cm.Positions.push_back(pos);
+            cm.Normals.push_back(Vector3d::Unit(a) * (p0 <= 0 ? 1.0 : -1.0));
+            cm.MeanCurvatures.push_back(5.0 + 2.0 * urand(rng));
+        });
+    }
+    for (int a = 0; a < 3; a++)
+        ForEach(fg[a], [&](Vector3i const &f) {
+            bool const lo = f[a] == 0, hi = f[a] == fg[a].GetSize()[a] - 1;
+            collider.Normal()[a][f] = lo ? 1.0 : (hi ? -1.0 : 0.0);
+        });
+    ProjectionStep::Settings st;
+    st.magneticEnabled = true;
+    st.surfaceTensionEnabled = true;
+    st.surfaceTensionCoeff = 0.07;
+    st.liquidDensity = 1000.0;
+    st.chi = 2.0;
+    st.fieldApplied = [](Vector3d const &p, double time) { return Vector3d(0.1 * p.z(), 1.0 + 0.2 * time, 0.0); };
+    SGridData<double> vstep = velocity;
+    ProjectionStep step(sgrid, vstep, levelSet, collider, contour, st);
+    double const dt = 1e-3;
+    step.ProjectVelocity(dt, 0.25, volError);
+    std::printf("\n");
+    std::vector<double> const &mp = step.GetMagneticPressures();
+    check((int)mp.size() == cm.size() && cm.size() > 100, "magnetic pressure per contour vertex", (double)cm.size());
+    // the jump the step handed to the projection, on every interior face
+    double const inv = sgrid.GetInvSpacing();
+    std::vector<double> jumpStep[3];
+    double const *js[3];
+    for (int a = 0; a < 3; a++) {
+        jumpStep[a].assign(fg[a].GetNumVertices(), 0.0);
+        ForEach(fg[a], [&](Vector3i const &f) {
+            if (f[a] == 0 || f[a] == n[a]) return;
+            int const idx = contour.VertexIndexOf(a, f - Vector3i::Unit(a));
+            if (idx < 0) return;
+            double pj = 0;
+            pj += cm.MeanCurvatures[idx] * st.surfaceTensionCoeff / st.liquidDensity * inv * dt;
+            pj -= mp[idx] / st.liquidDensity * inv * dt;
+            jumpStep[a][fg[a].IndexOf(f)] = pj;
+        });
+        js[a] = jumpStep[a].data();
+    }
+    std::vector<double> rhs3(nu);
+    Csr A3;
+    A3.ptr.resize(nu + 1);
+    A3.col.resize(7 * (size_t)nu);
+    A3.val.resize(7 * (size_t)nu);
+    orc_pressure_matrix(n[0], n[1], n[2], levelSet.GetData().data(), frac, vin, us, js, volError, g2m.data(),
+                        m2g.data(), nu, A3.ptr.data(), A3.col.data(), A3.val.data(), rhs3.data());
+    for (int r = 0; r < nu; r++) pc[r] = step.GetPressure().GetPressure()[m2g[r]];
+    double const rel3 = orc_rel_residual(nu, A3.ptr.data(), A3.col.data(), A3.val.data(), rhs3.data(), pc.data());
+    check(rel3 <= 1e-8, "step: oracle relative residual of the GPU pressure (curvature + magnetic jump)", rel3);
+    // given that pressure, update + extrapolation + wall condition match the oracle bit for bit
+    std::vector<double> vchain[3];
+    double *vc[3];
+    for (int a = 0; a < 3; a++) {
+        vchain[a] = velocity[a].GetData();
+        vc[a] = vchain[a].data();
+    }
+    orc_pressure_apply(n[0], n[1], n[2], levelSet.GetData().data(), frac, js, g2m.data(), pc.data(), vc);
+    std::vector<unsigned char> vmask[3];
+    unsigned char *vm[3];
+    unsigned char const *vmc[3];
+    for (int a = 0; a < 3; a++) {
+        vmask[a].assign(fg[a].GetNumVertices(), 0);
+        vm[a] = vmask[a].data();
+        vmc[a] = vm[a];
+    }
+    orc_velocity_valid(n[0], n[1], n[2], levelSet.GetData().data(), frac, vm);
+    orc_extrapolate_faces(n[0], n[1], n[2], vc, vmc, 0.0, 6);
+    double const *nrmp[3] = {collider.GetNormal()[0].GetData().data(), collider.GetNormal()[1].GetData().data(),
+                             collider.GetNormal()[2].GetData().data()};
+    orc_collider_enforce(n[0], n[1], n[2], sgrid.GetSpacing(), origin, vc, frac, nrmp, us);
+    bool chain = true;
+    for (int a = 0; a < 3; a++) chain &= vchain[a] == vstep[a].GetData();
+    check(chain, "step: projected, extrapolated, wall-enforced velocity bit-identical to the oracle chain");
+    // closed form: on a wall face away from edges the normal velocity equals the collider's
+    double werr = 0;
+    ForEach(fg[0], [&](Vector3i const &f) {
+        if (f[0] != 0 || f[1] < 1 || f[2] < 1 || f[1] >= n[1] - 1 || f[2] >= n[2] - 1) return;
+        werr = std::max(werr, std::fabs(vstep[0][f] - collider.Velocity[0][f]));
+    });
+    check(werr <= 1e-15, "step: wall-normal velocity equals the collider velocity on the x = min wall", werr);
 
     std::printf("%s (%d failed)\n", g_fail ? "FAILED" : "PASSED", g_fail);
     return g_fail ? 1 : 0;
