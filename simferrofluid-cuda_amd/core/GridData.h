@@ -5,6 +5,8 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <istream>
+#include <ostream>
 #include <vector>
 
 namespace Pivot {
@@ -33,6 +35,14 @@ class GridData {
     Type const &operator[](Vector3i const &c) const { return m_Data[m_Grid.IndexOf(c)]; }
     Type &operator[](int index) { return m_Data[index]; }
     Type const &operator[](int index) const { return m_Data[index]; }
+
+    // raw values in index order, no header (core/GridData.h:47-48, IO::Write of a span)
+    void Save(std::ostream &out) const {
+        out.write(reinterpret_cast<char const *>(m_Data.data()), (std::streamsize)(m_Data.size() * sizeof(Type)));
+    }
+    void Load(std::istream &in) {
+        in.read(reinterpret_cast<char *>(m_Data.data()), (std::streamsize)(m_Data.size() * sizeof(Type)));
+    }
 
     void SetConstant(Type const &value) { std::fill(m_Data.begin(), m_Data.end(), value); }
     void SetZero() { SetConstant(Type(0)); }

@@ -20,6 +20,13 @@ class SGridData {
     }
 
     std::array<Grid, 3> const &GetGrids() const { return m_Grids; }
+    // axis 0, 1, 2 in turn (core/SGridData.h:35-36)
+    void Save(std::ostream &out) const {
+        for (int i = 0; i < 3; i++) m_Datas[i].Save(out);
+    }
+    void Load(std::istream &in) {
+        for (int i = 0; i < 3; i++) m_Datas[i].Load(in);
+    }
     GridData<Type> &operator[](int axis) { return m_Datas[axis]; }
     GridData<Type> const &operator[](int axis) const { return m_Datas[axis]; }
 

@@ -92,3 +92,24 @@ def test_post_errors():
         assert e.value.status == sfp.ERR_ARGUMENT
         st = sfp.lib().sfp_collider_enforce(ctx.handle, None, None, None, None)
         assert st == sfp.ERR_ARGUMENT
+
+
+def test_checkpoint_round_trip_into_the_solver(oracle, tmp_path):
+    """A scene written as a reference checkpoint and read back drives the GPU
+    projection to the same velocities as the original arrays, and the
+    solution meets the tolerance on the oracle's system."""
+    from sfp import checkpoint
+    sc = scenes.pressure_scene(48, 3)
+    n = sc["geo"].n
+    path = str(tmp_path / "5.sav")
+    print("[case] checkpoint round trip 48^3 pressure", flush=True)
+    checkpoint.write(path, n, sc["velocity"], sc["phi"], init_volume=0.2, vol_error=0.0)
+    ck = checkpoint.read(path, n)
+    sc2 = dict(sc, velocity=ck["velocity"], phi=ck["phi"])
+    with sfp.Context.for_scene(sc) as ctx:
+        v1, p1, st1 = ctx.pressure_project(sc)
+        v2, p2, st2 = ctx.pressure_project(sc2)
+    for a in range(3):
+        np.testing.assert_array_equal(v1[a], v2[a])
+    ps = oracle.pressure_system(sc2)
+    assert oracle.rel_residual(ps["A"], ps["rhs"], p2.ravel()[ps["mat2grid"]]) <= 1e-8
