@@ -15,8 +15,9 @@ Prints one JSON line (see the driver contract); `value` is the mean seconds
 per solve of plain MG-PCG (the library's default). `learned_fcg_s` is the
 same solves with the learned preconditioner (the package's trained U-Net
 weights) over the same step count, and `learned_over_mg_time` their ratio. The `roofline` object times the fine-level Chebyshev smoother
-kernel (the dominant kernel) with HIP events on the library's stream during
-the timed steps. The `cpu_baseline` object times the FP64 CPU oracle
+kernel (the dominant kernel) with HIP events on the library's stream, over
+one more pass of each magnetic system (all cells active; pressure solves skip
+wavefronts of air cells) after the timed steps. The `cpu_baseline` object times the FP64 CPU oracle
 (smoothed-aggregation AMG + SPAI-0, BiCGStab for pressure, CG for magnetic,
 hierarchy rebuilt per solve) on one pressure and one magnetic solve at the
 same size on the host cores.
@@ -88,12 +89,13 @@ def summarize_profiles(d, n=256):
     lines = [f"# k_smooth<READ_D|WRITE_D> at {n}^3: rocprofv3 kernel trace", "",
              "Command (GPU box, repo root):", "",
              "    rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_trace -o trace -- \\",
-             "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 4 --warmup 0 --systems p,m2", "",
+             "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 4 --warmup 0 --systems m2,m8", "",
+             "(magnetic systems: every cell active, so every launch moves the full 36 B per cell)", "",
              f"Fine-level dispatches of `{SMOOTH_KERNEL}` (grid {grid} work-items): {len(fine)}",
              f"average duration {avg:.2f} us (min {min(fine):.2f}, max {max(fine):.2f});",
              f"algorithmic bytes per launch 36 B x {cells} cells = {alg / 1e6:.1f} MB "
              f"-> {alg / (avg * 1e-6) / 1e9:.0f} GB/s = {alg / (avg * 1e-6) / 8e12:.3f} of 8 TB/s.",
-             f"bench.py's HIP-event average over its timed steps: "
+             f"bench.py's HIP-event average over its profiled pass: "
              f"{bench['roofline']['avg_launch_us']:.2f} us.", "",
              "Top kernels by total time (all levels; from trace_kernel_stats.csv):", "",
              "| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
@@ -129,9 +131,9 @@ def summarize_profiles(d, n=256):
     md = [f"# k_smooth<READ_D|WRITE_D> at {n}^3: HBM traffic from FETCH_SIZE / WRITE_SIZE", "",
           "Commands (GPU box, repo root), one counter per pass:", "",
           "    rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/prof_fetch -o fetch -- \\",
-          "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 2 --warmup 0 --systems p,m2",
+          "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 2 --warmup 0 --systems m2,m8",
           "    rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/prof_write -o write -- \\",
-          "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 2 --warmup 0 --systems p,m2",
+          "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 2 --warmup 0 --systems m2,m8",
           "    rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/prof_copy_fetch -o copyf -- python3 bench.py --copy-probe",
           "    rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/prof_copy_write -o copyw -- python3 bench.py --copy-probe",
           "    python3 bench.py --summarize-profiles gpurun_out", "",
@@ -265,8 +267,6 @@ def main():
         name, st = run_step(i)
         log(f"warmup {name}: iters={st['iterations']} rel={st['rel_residual']:.2e} "
             f"setup={st['setup_ms']:.2f}ms solve={st['solve_ms']:.2f}ms")
-    if not args.no_profile:
-        ctx.profile(True)
     torch.cuda.synchronize()
     records = []
     t_start = time.perf_counter()
@@ -288,6 +288,17 @@ def main():
     roof = None
     kernels = {}
     if not args.no_profile:
+        # per-kernel HIP events on the library's stream, over one more pass of
+        # each system after the timed steps (event pairs around every launch
+        # would perturb the timed steps and keep FCG from batching iterations)
+        # Only systems whose every cell is active count: pressure solves skip
+        # wavefronts of air cells, which the 36 B/cell figure would overcount.
+        refill_ring()
+        ctx.profile(True)
+        prof = [i for i, (_, chi) in enumerate(systems) if chi is not None] or list(range(len(systems)))
+        for i in prof:
+            run_step(i)
+        torch.cuda.synchronize()
         for kid, kname in sfp.KERNEL_NAMES.items():
             kp = ctx.kernel_profile(kid)
             if kp["launches"]:

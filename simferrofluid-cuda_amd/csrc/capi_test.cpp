@@ -115,7 +115,7 @@ sfp_status sfp_test_apply64(sfp_context ctx, int which, const float *z, double *
         dw.ensure(n);
         SFP_HIP(hipMemcpyAsync(dz.p, z, n * sizeof(float), hipMemcpyHostToDevice, ctx->ws.stream));
         SFP_HIP(hipMemsetAsync(dr.p, 0, n * sizeof(double), ctx->ws.stream));
-        launch_fcg_matvec(ctx->ws.stream, d, s.fine().op64(), dz.p, dr.p, nullptr, dw.p, 1,
+        launch_fcg_matvec(ctx->ws.stream, d, s.fine().op64(), nullptr, dz.p, dr.p, nullptr, dw.p, 1,
                           ctx->ws.partials.p, ctx->ws.scalars.p);
         d2h(ctx, w, dw.p, n);
         SFP_HIP(hipStreamSynchronize(ctx->ws.stream));

@@ -4,6 +4,10 @@
 
 namespace sfp {
 
+// Iteration kernels take an optional `gate` (&KrylovScalars::stop): when it
+// points at a nonzero value the kernel returns at once (batched iterations
+// after convergence). Null = always run.
+
 // ---------------------------------------------------------------- multigrid (mg.hip)
 // Chebyshev step modes.
 enum SmoothMode : int {
@@ -16,14 +20,15 @@ enum SmoothMode : int {
 // x = c0 * D^{-1} b from a zero start; at the fine level b is first rounded
 // from the FP64 residual r64 (and stored to b).
 void launch_smooth_first(hipStream_t s, Dims d, Op32 A, const double *r64, float *b, float *x,
-                         float c0);
+                         float c0, const double *gate = nullptr);
 // xout = x + d', d' = cd * d + cr * D^{-1} (b - A x) with x = xin, xin + P e,
 // or the fused zero-start step c0 D^{-1} b (SM_FIRST; b from r64 when given).
 void launch_smooth(hipStream_t s, int mode, Dims d, Op32 A, const double *r64, float *b,
                    const float *xin, float *xout, float *dvec, float c0, float cd, float cr,
-                   const float *ecoarse);
+                   const float *ecoarse, const double *gate = nullptr);
 // bc = R (b - A x): sum of the residuals of the (up to 8) children.
-void launch_restrict(hipStream_t s, Dims df, Op32 A, const float *b, const float *x, float *bc);
+void launch_restrict(hipStream_t s, Dims df, Op32 A, const float *b, const float *x, float *bc,
+                     const double *gate = nullptr);
 // Galerkin coarsening by 2x2x2 aggregation, scaled: writes the coarse FP64
 // and FP32 operators.
 void launch_coarsen(hipStream_t s, Dims df, Op64 Af, const float *diag32f, Dims dc,
@@ -32,7 +37,8 @@ void launch_coarsen(hipStream_t s, Dims df, Op64 Af, const float *diag32f, Dims 
 // Dense inverse of the coarsest operator (n = cells <= kMaxCoarse) into ainv (n*n floats).
 constexpr int kMaxCoarse = 64;
 void launch_coarse_invert(hipStream_t s, Dims d, Op64 A, float *ainv);
-void launch_coarse_solve(hipStream_t s, int n, const float *ainv, const float *b, float *x);
+void launch_coarse_solve(hipStream_t s, int n, const float *ainv, const float *b, float *x,
+                         const double *gate = nullptr);
 
 // ---------------------------------------------------------------- Krylov (krylov.hip)
 // Reduction kernels cover this many x planes per block (fewer partials).
@@ -43,15 +49,19 @@ size_t reduction_partials(Dims d);
 void launch_residual64(hipStream_t s, Dims d, Op64 A, const double *b, const double *x,
                        double *r, double *partials, KrylovScalars *sc);
 // w = A z; dots z.w, z.q_prev, z.r; the last block forms beta, p.q and alpha.
-void launch_fcg_matvec(hipStream_t s, Dims d, Op64 A, const float *z, const double *r,
-                       const double *qprev, double *w, int first, double *partials,
-                       KrylovScalars *sc);
-// p = z - beta p; q = w - beta q; x += alpha p; r -= alpha q; rr = r.r.
-void launch_fcg_update(hipStream_t s, Dims d, const float *z, const double *w, double *p,
-                       double *q, double *x, double *r, int first, double *partials,
-                       KrylovScalars *sc);
+// active: the FP32 diagonal (0 = inactive cell) to skip all-inactive
+// wavefronts, where z must be 0; null = every cell (any z).
+void launch_fcg_matvec(hipStream_t s, Dims d, Op64 A, const float *active, const float *z,
+                       const double *r, const double *qprev, double *w, int first,
+                       double *partials, KrylovScalars *sc, const double *gate = nullptr);
+// p = z - beta p; q = w - beta q; x += alpha p; r -= alpha q; rr = r.r
+// (all-inactive wavefronts skipped when `active` is given).
+void launch_fcg_update(hipStream_t s, Dims d, const float *active, const float *z,
+                       const double *w, double *p, double *q, double *x, double *r, int first,
+                       double *partials, KrylovScalars *sc, const double *gate = nullptr);
 // z = D^{-1} r on active cells (Jacobi preconditioner).
-void launch_jacobi(hipStream_t s, Dims d, Op32 A, const double *r, float *z);
+void launch_jacobi(hipStream_t s, Dims d, Op32 A, const double *r, float *z,
+                   const double *gate = nullptr);
 
 // ---------------------------------------------------------------- assembly (assemble.hip)
 struct Faces {

@@ -12,6 +12,9 @@ namespace {
 
 enum FinalizeOp : int { FIN_RESIDUAL = 0, FIN_MATVEC = 1, FIN_UPDATE = 2, FIN_COUNT = 3 };
 
+// true when a batched solve has already converged (see KrylovScalars::stop)
+__device__ __forceinline__ bool gated(const double *gate) { return gate && *gate != 0.0; }
+
 // The stencil kernels below march over the block's x planes with the
 // x-neighbours in registers and take the z-neighbours (value and z coupling)
 // from the adjacent lanes of the tile row; only the row's end lanes load
@@ -78,12 +81,18 @@ __global__ __launch_bounds__(kBlock) void k_residual64(CellGrid cg, Dims d, Op64
     block_partials<2>(v, partials);
 }
 
+// `active` (the FP32 diagonal, 0 = inactive cell) or null: with it, a
+// wavefront whose cells are all inactive is skipped (z is 0 there and the
+// vectors it would write are never read by active cells).
 __global__ __launch_bounds__(kBlock) void k_fcg_matvec(CellGrid cg, Dims d, Op64 A,
+                                                       const float *__restrict__ active,
                                                        const float *__restrict__ z,
                                                        const double *__restrict__ r,
                                                        const double *__restrict__ qprev,
                                                        double *__restrict__ w, int first,
-                                                       double *__restrict__ partials) {
+                                                       double *__restrict__ partials,
+                                                       const double *gate) {
+    if (gated(gate)) return;
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     const Strides s(d);
@@ -93,10 +102,20 @@ __global__ __launch_bounds__(kBlock) void k_fcg_matvec(CellGrid cg, Dims d, Op64
     const int ie = min(i0 + cg.planes, d.nx);
     double v[3] = {0.0, 0.0, 0.0};  // z.w, z.q_prev, z.r
     size_t c = on ? d.idx(i0, j, k) : 0;
-    float zm = (on && i0 > 0) ? z[c - s.sx] : 0.f;
-    float zc = on ? z[c] : 0.f;
-    double cxc = on ? A.cpl[0][c] : 0.0;
+    float zm = 0.f, zc = 0.f;
+    double cxc = 0.0;
+    bool primed = false;  // zm, zc, cxc hold this plane's values
     for (int i = i0; i < ie; i++, c += s.sx) {
+        if (active && !__any(on && active[c] > 0.f)) {
+            primed = false;
+            continue;
+        }
+        if (!primed) {
+            zm = (on && i > 0) ? z[c - s.sx] : 0.f;
+            zc = on ? z[c] : 0.f;
+            cxc = on ? A.cpl[0][c] : 0.0;
+            primed = true;
+        }
         const bool up = i + 1 < d.nx;
         const float zp = (on && up) ? z[c + s.sx] : 0.f;
         const double cxp = (on && up) ? A.cpl[0][c + s.sx] : 0.0;
@@ -132,6 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_matvec(CellGrid cg, Dims d, Op64
 }
 
 __global__ __launch_bounds__(kBlock) void k_fcg_update(CellGrid cg, Dims d,
+                                                       const float *__restrict__ active,
                                                        const float *__restrict__ z,
                                                        const double *__restrict__ w,
                                                        double *__restrict__ p,
@@ -139,7 +159,9 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(CellGrid cg, Dims d,
                                                        double *__restrict__ x,
                                                        double *__restrict__ r, int first,
                                                        double *__restrict__ partials,
-                                                       const KrylovScalars *__restrict__ sc) {
+                                                       const KrylovScalars *__restrict__ sc,
+                                                       const double *gate) {
+    if (gated(gate)) return;
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     const double alpha = sc->alpha, beta = sc->beta;
@@ -147,6 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(CellGrid cg, Dims d,
     if (k < d.nz && j < d.ny) {
         for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
             const size_t c = d.idx(i, j, k);
+            if (active && !__any(active[c] > 0.f)) continue;  // all inactive: p, q, x, r stay
             const double zc = (double)z[c];
             double pc, qc;
             if (first) {
@@ -169,7 +192,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(CellGrid cg, Dims d,
 
 __global__ __launch_bounds__(kBlock) void k_jacobi(CellGrid cg, Dims d, Op32 A,
                                                    const double *__restrict__ r,
-                                                   float *__restrict__ z) {
+                                                   float *__restrict__ z, const double *gate) {
+    if (gated(gate)) return;
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     if (k >= d.nz || j >= d.ny) return;
@@ -196,7 +220,8 @@ __global__ __launch_bounds__(kBlock) void k_count_active(CellGrid cg, Dims d,
 // One block: add the partials and advance the scalar recurrences.
 __global__ __launch_bounds__(kBlock) void k_finalize(int op, unsigned nb, int first,
                                                      const double *__restrict__ partials,
-                                                     KrylovScalars *sc) {
+                                                     KrylovScalars *sc, const double *gate) {
+    if (gated(gate)) return;
     double v[3] = {0.0, 0.0, 0.0};
     if (op == FIN_MATVEC) {
         sum_partials<3>(partials, nb, v);
@@ -233,14 +258,16 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, unsigned nb, int fi
     } else if (op == FIN_UPDATE) {
         sc->rr = v[0];
         sc->pq_prev = sc->pq;
+        sc->iters += 1.0;
+        if (gate && v[0] <= sc->stop_rr) sc->stop = 1.0;  // later gated kernels return at once
     } else {
         sc->pad[0] = v[0];
     }
 }
 
 void finalize(hipStream_t st, int op, unsigned nb, int first, const double *partials,
-              KrylovScalars *sc) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, st, op, nb, first, partials, sc);
+              KrylovScalars *sc, const double *gate = nullptr) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, st, op, nb, first, partials, sc, gate);
     SFP_LAUNCH_CHECK();
 }
 
@@ -264,29 +291,29 @@ void launch_residual64(hipStream_t st, Dims d, Op64 A, const double *b, const do
     finalize(st, FIN_RESIDUAL, L.blocks(), 0, partials, sc);
 }
 
-void launch_fcg_matvec(hipStream_t st, Dims d, Op64 A, const float *z, const double *r,
-                       const double *qprev, double *w, int first, double *partials,
-                       KrylovScalars *sc) {
+void launch_fcg_matvec(hipStream_t st, Dims d, Op64 A, const float *active, const float *z,
+                       const double *r, const double *qprev, double *w, int first,
+                       double *partials, KrylovScalars *sc, const double *gate) {
     Launch L = cell_launch(d, kReducePlanes);
-    hipLaunchKernelGGL(k_fcg_matvec, L.grid, L.block, 0, st, L.cg, d, A, z, r, qprev, w, first,
-                       partials);
+    hipLaunchKernelGGL(k_fcg_matvec, L.grid, L.block, 0, st, L.cg, d, A, active, z, r, qprev, w,
+                       first, partials, gate);
     SFP_LAUNCH_CHECK();
-    finalize(st, FIN_MATVEC, L.blocks(), first, partials, sc);
+    finalize(st, FIN_MATVEC, L.blocks(), first, partials, sc, gate);
 }
 
-void launch_fcg_update(hipStream_t st, Dims d, const float *z, const double *w, double *p,
-                       double *q, double *x, double *r, int first, double *partials,
-                       KrylovScalars *sc) {
+void launch_fcg_update(hipStream_t st, Dims d, const float *active, const float *z,
+                       const double *w, double *p, double *q, double *x, double *r, int first,
+                       double *partials, KrylovScalars *sc, const double *gate) {
     Launch L = cell_launch(d, kReducePlanes);
-    hipLaunchKernelGGL(k_fcg_update, L.grid, L.block, 0, st, L.cg, d, z, w, p, q, x, r, first,
-                       partials, sc);
+    hipLaunchKernelGGL(k_fcg_update, L.grid, L.block, 0, st, L.cg, d, active, z, w, p, q, x, r,
+                       first, partials, sc, gate);
     SFP_LAUNCH_CHECK();
-    finalize(st, FIN_UPDATE, L.blocks(), first, partials, sc);
+    finalize(st, FIN_UPDATE, L.blocks(), first, partials, sc, gate);
 }
 
-void launch_jacobi(hipStream_t st, Dims d, Op32 A, const double *r, float *z) {
+void launch_jacobi(hipStream_t st, Dims d, Op32 A, const double *r, float *z, const double *gate) {
     Launch L = cell_launch(d);
-    hipLaunchKernelGGL(k_jacobi, L.grid, L.block, 0, st, L.cg, d, A, r, z);
+    hipLaunchKernelGGL(k_jacobi, L.grid, L.block, 0, st, L.cg, d, A, r, z, gate);
     SFP_LAUNCH_CHECK();
 }
 

@@ -35,13 +35,18 @@ Launch cell_launch(Dims d, int planes, int max_planes) {
 
 namespace {
 
+// true when a batched solve has already converged (see KrylovScalars::stop)
+__device__ __forceinline__ bool gated(const double *gate) { return gate && *gate != 0.0; }
+
 __device__ __forceinline__ float rcp_or_zero(float dg) {
     return dg > 0.f ? __builtin_amdgcn_rcpf(dg) : 0.f;
 }
 
 template <bool FROM64>
 __global__ __launch_bounds__(kBlock) void k_smooth_first(CellGrid cg, Dims d, Op32 A, const double *r64,
-                                                         float *b, float *x, float c0) {
+                                                         float *b, float *x, float c0,
+                                                         const double *gate) {
+    if (gated(gate)) return;
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     if (k >= d.nz || j >= d.ny) return;
@@ -77,7 +82,9 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
                                                    const float *__restrict__ xin,
                                                    float *__restrict__ xout, float *dvec,
                                                    float c0, float cd, float cr,
-                                                   const float *__restrict__ e, Dims dc) {
+                                                   const float *__restrict__ e, Dims dc,
+                                                   const double *gate) {
+    if (gated(gate)) return;
     constexpr bool kFirst = (MODE & SM_FIRST) != 0;
     constexpr bool kProlong = (MODE & SM_PROLONG) != 0;
     int k, j, i0;
@@ -101,11 +108,23 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
     const int ie = min(i0 + cg.planes, d.nx);
     const bool zlo = k > 0, zhi = k + 1 < d.nz;
     size_t c = on ? d.idx(i0, j, k) : 0;
-    float xm = (on && i0 > 0) ? xval(c - s.sx, i0 - 1, j, k) : 0.f;
-    float xc = on ? xval(c, i0, j, k) : 0.f;
-    float cxc = on ? A.cpl[0][c] : 0.f;
+    float xm = 0.f, xc = 0.f, cxc = 0.f;
+    bool primed = false;  // xm, xc, cxc hold this plane's values
     const bool ylo = on && j > 0, yhi = on && j + 1 < d.ny;
     for (int i = i0; i < ie; i++, c += s.sx) {
+        const float dg = on ? A.diag[c] : 0.f;
+        // a wavefront whose cells are all inactive has nothing to update (their
+        // vectors hold zeros and couple with weight 0): skip it whole
+        if (!__any(on && dg > 0.f)) {
+            primed = false;
+            continue;
+        }
+        if (!primed) {
+            xm = (on && i > 0) ? xval(c - s.sx, i - 1, j, k) : 0.f;
+            xc = on ? xval(c, i, j, k) : 0.f;
+            cxc = on ? A.cpl[0][c] : 0.f;
+            primed = true;
+        }
         // every load of the plane is issued before the lane exchange
         const bool up = i + 1 < d.nx;
         const float xp = (on && up) ? xval(c + s.sx, i + 1, j, k) : 0.f;
@@ -115,7 +134,6 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
         const float xym = ylo ? xval(c - s.sy, i, j - 1, k) : 0.f;
         const float cyp = yhi ? A.cpl[1][c + s.sy] : 0.f;
         const float xyp = yhi ? xval(c + s.sy, i, j + 1, k) : 0.f;
-        const float dg = on ? A.diag[c] : 0.f;
         float bc = 0.f;
         if constexpr (kFirst) {
             bc = on ? bval(c) : 0.f;
@@ -168,7 +186,8 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
 // is even and starts on an even plane, and tz, ty are even (or tz = 1).
 __global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 A, const float *__restrict__ b,
                                                      const float *__restrict__ x, Dims dc,
-                                                     float *__restrict__ bc) {
+                                                     float *__restrict__ bc, const double *gate) {
+    if (gated(gate)) return;
     __shared__ float sh[kBlock];
     int k, j, i0;
     cell_tile(cg, k, j, i0);
@@ -180,27 +199,33 @@ __global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 
     const int ie = min(i0 + cg.planes, df.nx);
     const bool zlo = k > 0, zhi = k + 1 < df.nz;
     size_t c = on ? df.idx(i0, j, k) : 0;
-    float xm = (on && i0 > 0) ? x[c - s.sx] : 0.f;
-    float xc = on ? x[c] : 0.f;
-    float cxc = on ? A.cpl[0][c] : 0.f;
+    float xm = 0.f, xc = 0.f, cxc = 0.f;
+    bool primed = false;  // xm, xc, cxc hold this plane's values
     float acc = 0.f;
     for (int i = i0; i < ie; i++, c += s.sx) {
         const bool up = i + 1 < df.nx;
-        const float xp = (on && up) ? x[c + s.sx] : 0.f;
-        const float cxp = (on && up) ? A.cpl[0][c + s.sx] : 0.f;
-        const float czm = on ? A.cpl[2][c] : 0.f;
-        float xzm = __shfl_up(xc, 1, tz);
-        float xzp = __shfl_down(xc, 1, tz);
-        float czp = __shfl_down(czm, 1, tz);
-        if (lane == 0) xzm = (on && zlo) ? x[c - 1] : 0.f;
-        if (lane == tz - 1) {
-            xzp = (on && zhi) ? x[c + 1] : 0.f;
-            czp = (on && zhi) ? A.cpl[2][c + 1] : 0.f;
-        }
+        const float dg = on ? A.diag[c] : 0.f;
         float r = 0.f;
-        if (on) {
-            const float dg = A.diag[c];
-            if (dg > 0.f) {
+        // a wavefront whose cells are all inactive contributes 0: skip its loads
+        if (__any(on && dg > 0.f)) {
+            if (!primed) {
+                xm = (on && i > 0) ? x[c - s.sx] : 0.f;
+                xc = on ? x[c] : 0.f;
+                cxc = on ? A.cpl[0][c] : 0.f;
+                primed = true;
+            }
+            const float xp = (on && up) ? x[c + s.sx] : 0.f;
+            const float cxp = (on && up) ? A.cpl[0][c + s.sx] : 0.f;
+            const float czm = on ? A.cpl[2][c] : 0.f;
+            float xzm = __shfl_up(xc, 1, tz);
+            float xzp = __shfl_down(xc, 1, tz);
+            float czp = __shfl_down(czm, 1, tz);
+            if (lane == 0) xzm = (on && zlo) ? x[c - 1] : 0.f;
+            if (lane == tz - 1) {
+                xzp = (on && zhi) ? x[c + 1] : 0.f;
+                czp = (on && zhi) ? A.cpl[2][c + 1] : 0.f;
+            }
+            if (on && dg > 0.f) {
                 float off = 0.f;
                 if (i > 0) off += cxc * xm;
                 if (up) off += cxp * xp;
@@ -210,6 +235,11 @@ __global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 
                 if (zhi) off += czp * xzp;
                 r = b[c] - (dg * xc - off);
             }
+            xm = xc;
+            xc = xp;
+            cxc = cxp;
+        } else {
+            primed = false;
         }
         if (tz > 1) r += __shfl_xor(r, 1, tz);  // z pair
         sh[row * tz + lane] = r;
@@ -221,9 +251,6 @@ __global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 
             if (writer) bc[dc.idx(i >> 1, j >> 1, k >> 1)] = acc;
             acc = 0.f;
         }
-        xm = xc;
-        xc = xp;
-        cxc = cxp;
     }
 }
 
@@ -355,7 +382,8 @@ __global__ __launch_bounds__(kBlock) void k_coarse_invert(Dims d, Op64 A, float 
 }
 
 __global__ __launch_bounds__(kBlock) void k_coarse_solve(int n, const float *ainv, const float *b,
-                                                         float *x) {
+                                                         float *x, const double *gate) {
+    if (gated(gate)) return;
     __shared__ float bs[kMaxCoarse];
     const int tid = threadIdx.x;
     for (int t = tid; t < n; t += kBlock) bs[t] = b[t];
@@ -370,18 +398,18 @@ __global__ __launch_bounds__(kBlock) void k_coarse_solve(int n, const float *ain
 }  // namespace
 
 void launch_smooth_first(hipStream_t st, Dims d, Op32 A, const double *r64, float *b, float *x,
-                         float c0) {
+                         float c0, const double *gate) {
     Launch L = cell_launch(d);
     if (r64)
-        hipLaunchKernelGGL(k_smooth_first<true>, L.grid, L.block, 0, st, L.cg, d, A, r64, b, x, c0);
+        hipLaunchKernelGGL(k_smooth_first<true>, L.grid, L.block, 0, st, L.cg, d, A, r64, b, x, c0, gate);
     else
-        hipLaunchKernelGGL(k_smooth_first<false>, L.grid, L.block, 0, st, L.cg, d, A, r64, b, x, c0);
+        hipLaunchKernelGGL(k_smooth_first<false>, L.grid, L.block, 0, st, L.cg, d, A, r64, b, x, c0, gate);
     SFP_LAUNCH_CHECK();
 }
 
 void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, float *b,
                    const float *xin, float *xout, float *dvec, float c0, float cd, float cr,
-                   const float *ecoarse) {
+                   const float *ecoarse, const double *gate) {
     // 8 planes per block measured best for the shuffle-based step (256^3:
     // 122 us vs 124 us at 16 and 123 us at 4)
     Launch L = cell_launch(d, 0, 8);
@@ -389,7 +417,7 @@ void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, 
 #define SFP_SMOOTH_CASE(M)                                                                    \
     case M:                                                                                   \
         hipLaunchKernelGGL(k_smooth<M>, L.grid, L.block, 0, st, L.cg, d, A, r64, b, xin, xout, \
-                           dvec, c0, cd, cr, ecoarse, dc);                                    \
+                           dvec, c0, cd, cr, ecoarse, dc, gate);                              \
         break;
     switch (mode) {
         SFP_SMOOTH_CASE(SM_FIRST)
@@ -407,11 +435,12 @@ void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, 
     SFP_LAUNCH_CHECK();
 }
 
-void launch_restrict(hipStream_t st, Dims df, Op32 A, const float *b, const float *x, float *bc) {
+void launch_restrict(hipStream_t st, Dims df, Op32 A, const float *b, const float *x, float *bc,
+                     const double *gate) {
     Dims dc = df.coarse();
     Launch L = cell_launch(df, 0, 8);
     if (L.cg.planes & 1) L = cell_launch(df, L.cg.planes + 1);  // plane pairs stay in one block
-    hipLaunchKernelGGL(k_restrict, L.grid, L.block, 0, st, L.cg, df, A, b, x, dc, bc);
+    hipLaunchKernelGGL(k_restrict, L.grid, L.block, 0, st, L.cg, df, A, b, x, dc, bc, gate);
     SFP_LAUNCH_CHECK();
 }
 
@@ -431,9 +460,10 @@ void launch_coarse_invert(hipStream_t st, Dims d, Op64 A, float *ainv) {
     SFP_LAUNCH_CHECK();
 }
 
-void launch_coarse_solve(hipStream_t st, int n, const float *ainv, const float *b, float *x) {
+void launch_coarse_solve(hipStream_t st, int n, const float *ainv, const float *b, float *x,
+                         const double *gate) {
     if (n > kMaxCoarse) throw ArgError("coarsest level too large for the dense solve");
-    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(kBlock), 0, st, n, ainv, b, x);
+    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(kBlock), 0, st, n, ainv, b, x, gate);
     SFP_LAUNCH_CHECK();
 }
 

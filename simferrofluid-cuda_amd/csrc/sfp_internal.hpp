@@ -127,7 +127,11 @@ struct KrylovScalars {
     double zw, zq, zr, pq_prev;  // dots of the current iteration
     double alpha, beta, pq;
     double rr, bb;               // ||r||^2, ||b||^2
-    double pad[7];
+    double pad[4];               // scratch of single reductions (count, scales)
+    // batched iterations: `stop` becomes nonzero when rr <= stop_rr after an
+    // update, and every gated kernel of later iterations then returns at once;
+    // `iters` counts the updates done
+    double stop, iters, stop_rr;
 };
 
 // Per-kernel timing collected with HIP events on the context stream.

@@ -78,7 +78,18 @@ void Workspace::read_scalars() {
     SFP_HIP(hipStreamSynchronize(stream));
 }
 
+void Workspace::write_scalars() {
+    SFP_HIP(hipMemcpyAsync(scalars.p, host_sc, sizeof(KrylovScalars), hipMemcpyHostToDevice, stream));
+}
+
 // ---------------------------------------------------------------- System
+// The stencil kernels skip wavefront rows whose cells are all inactive; the
+// skipped cells of the level vectors then keep the zeros written here.
+static void zero_vectors(Level &L) {
+    const size_t n = L.d.cells();
+    for (auto *v : {&L.x0, &L.x1, &L.dv, &L.b}) SFP_HIP(hipMemset(v->p, 0, n * sizeof(float)));
+}
+
 void System::allocate(Dims d) {
     if (d_.nx == d.nx && d_.ny == d.ny && d_.nz == d.nz && !lv_.empty()) return;
     d_ = d;
@@ -97,12 +108,14 @@ void System::allocate(Dims d) {
     L.x1.ensure(n);
     L.dv.ensure(n);
     L.b.ensure(n);
+    zero_vectors(L);
     rhs_.ensure(n);
     x_.ensure(n);
     r_.ensure(n);
     p_.ensure(n);
     q_.ensure(n);
     w_.ensure(n);
+    for (auto *v : {&r_, &p_, &q_, &w_}) SFP_HIP(hipMemset(v->p, 0, n * sizeof(double)));
     SFP_HIP(hipMemset(x_.p, 0, n * sizeof(double)));
     built_coarsest_ = -1;
 }
@@ -127,6 +140,7 @@ void System::build_hierarchy(Workspace &ws, const SolverConfig &cfg) {
             L.x1.ensure(n);
             L.dv.ensure(n);
             L.b.ensure(n);
+            zero_vectors(L);
             lv_.push_back(std::move(L));
         }
         if ((int)lv_.back().d.cells() > kMaxCoarse) throw ArgError("coarsest level too large");
@@ -150,8 +164,8 @@ float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) 
     const bool fine = (l == 0);
     const double cells = (double)L.d.cells();
     if (l + 1 == (int)lv_.size()) {
-        if (r64) launch_smooth_first(st, L.d, L.op32(), r64, L.b.p, L.x1.p, 0.f);
-        launch_coarse_solve(st, (int)L.d.cells(), ainv_.p, L.b.p, L.x0.p);
+        if (r64) launch_smooth_first(st, L.d, L.op32(), r64, L.b.p, L.x1.p, 0.f, ws.gate);
+        launch_coarse_solve(st, (int)L.d.cells(), ainv_.p, L.b.p, L.x0.p, ws.gate);
         return L.x0.p;
     }
     const Op32 A = L.op32();
@@ -162,12 +176,12 @@ float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) 
         // 36 algorithmic bytes per cell (x, b, d, diag, 3 couplings, x', d')
         const bool track = fine && ws.profiling && mode == (SM_READ_D | SM_WRITE_D);
         if (track) ws.timers[K_SMOOTH].begin(st, 36.0 * cells);
-        launch_smooth(st, mode, L.d, A, r, L.b.p, cur, oth, L.dv.p, cb.cr[0], cd, cr, e);
+        launch_smooth(st, mode, L.d, A, r, L.b.p, cur, oth, L.dv.p, cb.cr[0], cd, cr, e, ws.gate);
         if (track) ws.timers[K_SMOOTH].end(st);
         std::swap(cur, oth);
     };
     if (deg == 1) {
-        launch_smooth_first(st, L.d, A, r64, L.b.p, cur, cb.cr[0]);
+        launch_smooth_first(st, L.d, A, r64, L.b.p, cur, cb.cr[0], ws.gate);
     } else {
         // steps 0 and 1 in one pass; then steps 2 .. deg-1
         smooth(SM_FIRST | (deg > 2 ? SM_WRITE_D : 0), cb.cd[1], cb.cr[1], r64, nullptr);
@@ -175,7 +189,7 @@ float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) 
             smooth(SM_READ_D | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s], nullptr, nullptr);
     }
     if (fine && ws.profiling) ws.timers[K_RESTRICT].begin(st, (24.0 + 0.5) * cells);
-    launch_restrict(st, L.d, A, L.b.p, cur, lv_[l + 1].b.p);
+    launch_restrict(st, L.d, A, L.b.p, cur, lv_[l + 1].b.p, ws.gate);
     if (fine && ws.profiling) ws.timers[K_RESTRICT].end(st);
     const float *e = vlevel(ws, cb, l + 1, nullptr);
     return post_smooth(ws, cb, l, cur, e);
@@ -195,7 +209,7 @@ float *System::post_smooth(Workspace &ws, const Cheby &cb, int l, float *cur, co
         const bool t = track && mode == (SM_READ_D | SM_WRITE_D);
         if (t) ws.timers[K_SMOOTH].begin(st, 36.0 * (double)L.d.cells());
         launch_smooth(st, mode, L.d, A, nullptr, L.b.p, cur, oth, L.dv.p, cb.cr[0], s == 0 ? 0.f : cb.cd[s],
-                      s == 0 ? cb.cr[0] : cb.cr[s], s == 0 ? e : nullptr);
+                      s == 0 ? cb.cr[0] : cb.cr[s], s == 0 ? e : nullptr, ws.gate);
         if (t) ws.timers[K_SMOOTH].end(st);
         std::swap(cur, oth);
     }
@@ -204,7 +218,7 @@ float *System::post_smooth(Workspace &ws, const Cheby &cb, int l, float *cur, co
 
 float *System::precondition(Workspace &ws, const SolverConfig &cfg, const double *r64) {
     if (cfg.precond == 2) {  // SFP_PRECOND_JACOBI
-        launch_jacobi(ws.stream, d_, lv_[0].op32(), r64, lv_[0].x0.p);
+        launch_jacobi(ws.stream, d_, lv_[0].op32(), r64, lv_[0].x0.p, ws.gate);
         return lv_[0].x0.p;
     }
     if (cfg.precond == 3 && lv_.size() > 1) return learned_cycle(ws, cfg, r64, kCycleSmooth);
@@ -323,25 +337,42 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
             continue;
         }
         if (info.iterations >= cfg.max_iters) break;
-        const float *z;
-        if (learned)
-            z = precondition_learned(ws, cfg, r_.p);
-        else if (fine)
-            z = precondition(ws, cfg, r_.p);
-        else
-            z = vlevel(ws, Cheby::make(cfg.cheby_degree, cfg.cheby_ratio), level, r_.p);
-        if (profiling) ws.timers[K_MATVEC64].begin(st, (4.0 + 32.0 + 8.0 + 8.0 + 8.0) * cells);
-        launch_fcg_matvec(st, d, A, z, r_.p, q_.p, w_.p, first ? 1 : 0, ws.partials.p,
-                          ws.scalars.p);
-        if (profiling) ws.timers[K_MATVEC64].end(st);
-        if (profiling) ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 5 + 8.0 * 4) * cells);
-        launch_fcg_update(st, d, z, w_.p, p_.p, q_.p, x_.p, r_.p, first ? 1 : 0, ws.partials.p,
-                          ws.scalars.p);
-        if (profiling) ws.timers[K_UPDATE64].end(st);
-        first = false;
-        info.iterations++;
-        if (!learned && cfg.precond == 1 && info.precond_used == 1) info.fallback++;
+        // Up to kBatch iterations are enqueued before the host looks at the
+        // residual: the update's finalize raises `stop` on the device once
+        // ||r|| <= tol ||b||, and every kernel of the remaining iterations of
+        // the batch then returns at once. The learned path (its stall test
+        // needs every residual) and profiled solves run one at a time.
+        const int batch = (learned || profiling) ? 1 : std::min(kBatch, cfg.max_iters - info.iterations);
+        if (batch > 1) {
+            ws.host_sc->stop = 0.0;
+            ws.host_sc->iters = 0.0;
+            ws.host_sc->stop_rr = cfg.tol * cfg.tol * bb;
+            ws.write_scalars();
+            ws.gate = &ws.scalars.p->stop;
+        }
+        for (int it = 0; it < batch; it++) {
+            const float *z;
+            if (learned)
+                z = precondition_learned(ws, cfg, r_.p);
+            else if (fine)
+                z = precondition(ws, cfg, r_.p);
+            else
+                z = vlevel(ws, Cheby::make(cfg.cheby_degree, cfg.cheby_ratio), level, r_.p);
+            if (profiling) ws.timers[K_MATVEC64].begin(st, (4.0 + 32.0 + 8.0 + 8.0 + 8.0) * cells);
+            launch_fcg_matvec(st, d, A, lv_[level].diag32.p, z, r_.p, q_.p, w_.p, first ? 1 : 0,
+                              ws.partials.p, ws.scalars.p, ws.gate);
+            if (profiling) ws.timers[K_MATVEC64].end(st);
+            if (profiling) ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 5 + 8.0 * 4) * cells);
+            launch_fcg_update(st, d, lv_[level].diag32.p, z, w_.p, p_.p, q_.p, x_.p, r_.p,
+                              first ? 1 : 0, ws.partials.p, ws.scalars.p, ws.gate);
+            if (profiling) ws.timers[K_UPDATE64].end(st);
+            first = false;
+        }
         ws.read_scalars();
+        ws.gate = nullptr;
+        const int done = batch > 1 ? (int)ws.host_sc->iters : 1;
+        info.iterations += done;
+        if (!learned && cfg.precond == 1 && info.precond_used == 1) info.fallback += done;
         const double prev = rel;
         rel = std::sqrt(ws.host_sc->rr / bb);
         checked = false;

@@ -11,6 +11,9 @@
 
 namespace sfp {
 
+// FCG iterations enqueued between two reads of the residual on the host.
+constexpr int kBatch = 4;
+
 struct SolverConfig {
     double tol = 1e-8;
     int max_iters = 100;
@@ -59,7 +62,11 @@ struct Workspace {
     UNet *net = nullptr;
     const double *phi = nullptr;
     double spacing = 1.0;
+    // &scalars->stop while a batched solve runs (iteration kernels return at
+    // once when it is nonzero), null otherwise
+    const double *gate = nullptr;
     void read_scalars();               // synchronous copy of *scalars to host_sc
+    void write_scalars();              // host_sc to *scalars (stream-ordered)
 };
 
 struct SolveInfo {
