@@ -321,49 +321,56 @@ __global__ __launch_bounds__(kBlock) void k_coarsen(CellGrid cg, Dims df, Op64 A
 
 // In-place Gauss-Jordan inversion of the dense coarsest matrix (SPD up to
 // pure-Neumann blocks, whose zero pivots are dropped) in LDS, one block.
+// Dense inverse of the coarsest operator (n <= 64 cells) by Gauss-Jordan in
+// LDS, rows stored with a fixed stride of 64 so the element loops split an
+// index with a shift and a mask. Pivots below 1e-12 of the largest diagonal
+// (the pure-Neumann direction) are dropped: their row and column stay 0.
 __global__ __launch_bounds__(kBlock) void k_coarse_invert(Dims d, Op64 A, float *ainv) {
-    __shared__ double M[kMaxCoarse * kMaxCoarse];
-    __shared__ double rowk[kMaxCoarse], colk[kMaxCoarse];
+    constexpr int S = kMaxCoarse;  // row stride
+    static_assert(S == 64, "the element split below assumes a stride of 64");
+    __shared__ double M[S * S];
+    __shared__ double rowk[S], colk[S];
     __shared__ double scale_sh;
     const int n = (int)d.cells();
     const int tid = threadIdx.x;
     const Strides s(d);
-    for (int e = tid; e < n * n; e += kBlock) M[e] = 0.0;
+    for (int e = tid; e < S * S; e += kBlock) M[e] = 0.0;
     __syncthreads();
     for (int c = tid; c < n; c += kBlock) {
         const int i = c / (d.ny * d.nz), j = (c / d.nz) % d.ny, k = c % d.nz;
-        M[c * n + c] = A.diag[c];
-        if (i > 0) M[c * n + (c - (int)s.sx)] = -A.cpl[0][c];
-        if (i + 1 < d.nx) M[c * n + (c + (int)s.sx)] = -A.cpl[0][c + s.sx];
-        if (j > 0) M[c * n + (c - (int)s.sy)] = -A.cpl[1][c];
-        if (j + 1 < d.ny) M[c * n + (c + (int)s.sy)] = -A.cpl[1][c + s.sy];
-        if (k > 0) M[c * n + (c - 1)] = -A.cpl[2][c];
-        if (k + 1 < d.nz) M[c * n + (c + 1)] = -A.cpl[2][c + 1];
+        M[c * S + c] = A.diag[c];
+        if (i > 0) M[c * S + (c - (int)s.sx)] = -A.cpl[0][c];
+        if (i + 1 < d.nx) M[c * S + (c + (int)s.sx)] = -A.cpl[0][c + s.sx];
+        if (j > 0) M[c * S + (c - (int)s.sy)] = -A.cpl[1][c];
+        if (j + 1 < d.ny) M[c * S + (c + (int)s.sy)] = -A.cpl[1][c + s.sy];
+        if (k > 0) M[c * S + (c - 1)] = -A.cpl[2][c];
+        if (k + 1 < d.nz) M[c * S + (c + 1)] = -A.cpl[2][c + 1];
     }
     __syncthreads();
     if (tid == 0) {
         double m = 0.0;
-        for (int c = 0; c < n; c++) m = fmax(m, fabs(M[c * n + c]));
+        for (int c = 0; c < n; c++) m = fmax(m, fabs(M[c * S + c]));
         scale_sh = m;
     }
     __syncthreads();
     const double tiny = 1e-12 * scale_sh;
     for (int kk = 0; kk < n; kk++) {
-        const double p = M[kk * n + kk];
-        for (int t = tid; t < n; t += kBlock) {
-            rowk[t] = M[kk * n + t];
-            colk[t] = M[t * n + kk];
+        const double p = M[kk * S + kk];
+        if (tid < n) {
+            rowk[tid] = M[kk * S + tid];
+            colk[tid] = M[tid * S + kk];
         }
         __syncthreads();
         if (fabs(p) <= tiny) {
-            for (int t = tid; t < n; t += kBlock) {
-                M[kk * n + t] = 0.0;
-                M[t * n + kk] = 0.0;
+            if (tid < n) {
+                M[kk * S + tid] = 0.0;
+                M[tid * S + kk] = 0.0;
             }
         } else {
             const double pinv = 1.0 / p;
-            for (int e = tid; e < n * n; e += kBlock) {
-                const int r = e / n, c = e % n;
+            for (int e = tid; e < S * S; e += kBlock) {
+                const int r = e >> 6, c = e & (S - 1);
+                if (r >= n || c >= n) continue;
                 double v;
                 if (r == kk && c == kk)
                     v = pinv;
@@ -378,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse_invert(Dims d, Op64 A, float 
         }
         __syncthreads();
     }
-    for (int e = tid; e < n * n; e += kBlock) ainv[e] = (float)M[e];
+    for (int e = tid; e < n * n; e += kBlock) ainv[e] = (float)M[(e / n) * S + (e % n)];
 }
 
 __global__ __launch_bounds__(kBlock) void k_coarse_solve(int n, const float *ainv, const float *b,
