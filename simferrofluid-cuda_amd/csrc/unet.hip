@@ -38,6 +38,7 @@ struct ConvArgs {
     uint16_t *relu_copy;  // relu(acc + bias) before the skip, or null
     int B, X, Y, Z;
     int relu;
+    int up;  // 1: `in` is [B][X/2][Y/2][Z/2][CIN], read nearest-up-sampled
 };
 
 template <int CIN>
@@ -87,7 +88,10 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
         const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
         uint4 v = make_uint4(0, 0, 0, 0);
         if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.X && gy < a.Y && gz < a.Z) {
-            const size_t gi = ((((size_t)bb * a.X + gx) * a.Y + gy) * a.Z + gz) * CIN + part * 8;
+            const size_t gi =
+                a.up ? ((((size_t)bb * (a.X >> 1) + (gx >> 1)) * (a.Y >> 1) + (gy >> 1)) * (a.Z >> 1) + (gz >> 1)) *
+                               CIN + part * 8
+                     : ((((size_t)bb * a.X + gx) * a.Y + gy) * a.Z + gz) * CIN + part * 8;
             v = *reinterpret_cast<const uint4 *>(a.in + gi);
         }
         *reinterpret_cast<uint4 *>(tile + vox * CIN + part * 8) = v;
@@ -467,8 +471,9 @@ void dw_dispatch(hipStream_t st, const uint16_t *x, const uint16_t *g, int B, in
 
 void unet_conv(hipStream_t st, int cin, int cout, const uint16_t *in, const uint16_t *w,
                const float *bias, const uint16_t *skip, uint16_t *out, float *out32,
-               uint16_t *relu_copy, int B, int X, int Y, int Z, bool relu) {
-    ConvArgs a{in, w, bias, skip, out, out32, relu_copy, B, X, Y, Z, relu ? 1 : 0};
+               uint16_t *relu_copy, int B, int X, int Y, int Z, bool relu, bool up_input) {
+    if (up_input && (X % 2 || Y % 2 || Z % 2)) throw ArgError("up-sampled input needs even extents");
+    ConvArgs a{in, w, bias, skip, out, out32, relu_copy, B, X, Y, Z, relu ? 1 : 0, up_input ? 1 : 0};
 #define SFP_CONV(CI, CO) \
     if (cin == CI && cout == CO) return conv_dispatch<CI, CO>(st, a);
     SFP_CONV(8, 16)

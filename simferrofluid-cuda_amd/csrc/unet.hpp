@@ -30,9 +30,11 @@ constexpr int conv_taps(int cin) { return cin <= 16 ? 28 : 27; }
 long long unet_param_count();
 
 // launchers (unet.hip)
+// up_input: `in` holds the half-resolution tensor, read nearest-up-sampled
+// (the up-sampling fused into the staging)
 void unet_conv(hipStream_t st, int cin, int cout, const uint16_t *in, const uint16_t *w,
                const float *bias, const uint16_t *skip, uint16_t *out, float *out32,
-               uint16_t *relu_copy, int B, int X, int Y, int Z, bool relu);
+               uint16_t *relu_copy, int B, int X, int Y, int Z, bool relu, bool up_input = false);
 void unet_conv_dw(hipStream_t st, int cin, int cout, const uint16_t *x, const uint16_t *g, int B,
                   int X, int Y, int Z, float *part, float *dw, float *db);
 void unet_pool2(hipStream_t st, const uint16_t *in, int B, int X, int Y, int Z, int C, uint16_t *out);
@@ -81,7 +83,7 @@ class UNet {
   private:
     void cast_weights(hipStream_t st);
     void conv(hipStream_t st, int k, int level, const uint16_t *in, const uint16_t *skip,
-              uint16_t *out, uint16_t *relu_copy);
+              uint16_t *out, uint16_t *relu_copy, bool up_input = false);
     int dims(int level, int axis) const;
     long long off_w_[kUNetConvs], off_b_[kUNetConvs], off_head_;
     DevBuf<float> w_, grad_, m_, v_;

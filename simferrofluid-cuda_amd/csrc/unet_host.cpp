@@ -92,10 +92,10 @@ void UNet::ensure(const UNetShape &s, bool training) {
     const size_t n0 = s.voxels(0), n1 = s.voxels(1), n2 = s.voxels(2);
     x8_.ensure(n0 * 8);
     for (auto *b : {&a0_, &e0_, &r8_, &s0_, &d0_}) b->ensure(n0 * 16);
-    u0_.ensure(n0 * 32);
+    if (training) u0_.ensure(n0 * 32);
     p0_.ensure(n1 * 16);
     for (auto *b : {&a1_, &e1_, &r6_, &s1_, &d1_}) b->ensure(n1 * 32);
-    u1_.ensure(n1 * 64);
+    if (training) u1_.ensure(n1 * 64);
     p1_.ensure(n2 * 32);
     a2_.ensure(n2 * 64);
     e2_.ensure(n2 * 64);
@@ -114,9 +114,9 @@ void UNet::ensure(const UNetShape &s, bool training) {
 }
 
 void UNet::conv(hipStream_t st, int k, int level, const uint16_t *in, const uint16_t *skip,
-                uint16_t *out, uint16_t *relu_copy) {
+                uint16_t *out, uint16_t *relu_copy, bool up_input) {
     unet_conv(st, kConvCin[k], kConvCout[k], in, wf_[k].p, w_.p + off_b_[k], skip, out, nullptr, relu_copy,
-              shape_.B, dims(level, 0), dims(level, 1), dims(level, 2), true);
+              shape_.B, dims(level, 0), dims(level, 1), dims(level, 2), true, up_input);
 }
 
 void UNet::forward(hipStream_t st, const uint16_t *x8, float *z) {
@@ -130,11 +130,21 @@ void UNet::forward(hipStream_t st, const uint16_t *x8, float *z) {
     unet_pool2(st, e1_.p, B, dims(1, 0), dims(1, 1), dims(1, 2), 32, p1_.p);
     conv(st, 4, 2, p1_.p, nullptr, a2_.p, nullptr);
     conv(st, 5, 2, a2_.p, nullptr, e2_.p, nullptr);
-    unet_up2(st, e2_.p, B, dims(2, 0), dims(2, 1), dims(2, 2), 64, u1_.p);
-    conv(st, 6, 1, u1_.p, e1_.p, s1_.p, keep6);
+    // training keeps the up-sampled tensors (the weight gradients read them);
+    // inference reads the half-resolution ones through the convolution
+    if (training_) {
+        unet_up2(st, e2_.p, B, dims(2, 0), dims(2, 1), dims(2, 2), 64, u1_.p);
+        conv(st, 6, 1, u1_.p, e1_.p, s1_.p, keep6);
+    } else {
+        conv(st, 6, 1, e2_.p, e1_.p, s1_.p, keep6, true);
+    }
     conv(st, 7, 1, s1_.p, nullptr, d1_.p, nullptr);
-    unet_up2(st, d1_.p, B, dims(1, 0), dims(1, 1), dims(1, 2), 32, u0_.p);
-    conv(st, 8, 0, u0_.p, e0_.p, s0_.p, keep8);
+    if (training_) {
+        unet_up2(st, d1_.p, B, dims(1, 0), dims(1, 1), dims(1, 2), 32, u0_.p);
+        conv(st, 8, 0, u0_.p, e0_.p, s0_.p, keep8);
+    } else {
+        conv(st, 8, 0, d1_.p, e0_.p, s0_.p, keep8, true);
+    }
     conv(st, 9, 0, s0_.p, nullptr, d0_.p, nullptr);
     unet_head_fwd(st, d0_.p, w_.p + off_head_, shape_.voxels(0), z);
 }

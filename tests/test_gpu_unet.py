@@ -71,6 +71,22 @@ def test_forward_matches_oracle(oracle, shape):
     assert err <= 2e-2
 
 
+def test_inference_and_training_forward_agree():
+    """Inference reads the half-resolution tensors through the convolution
+    (up-sampling fused into the staging), training up-samples explicitly:
+    nearest up-sampling is a copy, so both give the same bits."""
+    shape = (2, 8, 12, 16)
+    print(f"[case] fused vs explicit up-sampling shape={shape}", flush=True)
+    p = U.random_params(41).astype(np.float32)
+    x = np.random.default_rng(42).normal(size=shape + (4,)).astype(np.float32)
+    outs = []
+    for training in (False, True):
+        with sfp.UNet(*shape, training=training) as net:
+            net.set_params(p)
+            outs.append(net.forward(x))
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_gradients_match_oracle(oracle):
     shape = (1, 8, 8, 12)
     print(f"[case] gradients vs oracle shape={shape}", flush=True)
