@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-preconditioner leg (same systems, same step count)")
     ap.add_argument("--systems", default="p,m0.5,m2,m8")
+    ap.add_argument("--workload", choices=["default", "spike512"], default="default",
+                    help="spike512: 20 alternating pressure / magnetic solves on the 512^3 spike "
+                         "interface, magnetic warm-started (prints its own JSON line)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (counter calibration)")
     ap.add_argument("--summarize-profiles", metavar="DIR",
@@ -202,6 +205,9 @@ def main():
         return
     if args.copy_probe:
         copy_probe()
+        return
+    if args.workload == "spike512":
+        spike512(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "the solver runs on exactly one GPU; a multi-GPU "
@@ -379,6 +385,77 @@ def main():
     }
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def spike512(args):
+    """20 consecutive solves alternating pressure projection and magnetic
+    potential on the spike-pattern interface (y = 0.2 - 64 cosine modes of
+    amplitude <= 0.005, chi = 4, H_ext = (0, 1.5, 0)) at 512^3. The interface
+    moves between pairs of solves (every mode's phase advances by 0.1 rad), as
+    in a running simulation; the magnetic solves are warm-started from the
+    previous one (as the reference does) and every pressure solve starts from
+    pristine smoothed random velocities."""
+    import numpy as np
+    import torch
+
+    import sfp
+    from sfp import scenes
+
+    n = 512 if args.n == 256 else args.n
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    msc = scenes.spike_scene(n, seed=0, chi=4.0)
+    geo = msc["geo"]
+    vel0 = []
+    for a in range(3):
+        shp = geo.face_shape(a)
+        u = scenes.uniform(0, scenes.ID_VEL + a, int(np.prod(shp))).reshape(shp) * 2.0 - 1.0
+        vel0.append(torch.from_numpy(scenes._box_smooth(u, 3)).to(dev))
+        del u
+        log(f"spike512: velocity {a} smoothed ({time.time() - t0:.1f}s)")
+    frac = [torch.from_numpy(f).to(dev) for f in scenes.wall_fractions(geo)]
+    us = [torch.zeros(geo.face_shape(a), dtype=torch.float64, device=dev) for a in range(3)]
+    modes = scenes._modes(0, 64, 0.005, mmax=12)
+    phi = torch.from_numpy(msc["phi"]).to(dev)
+    hext = [torch.from_numpy(h).to(dev) for h in msc["hext"]]
+    vel = [v.clone() for v in vel0]
+    out = torch.zeros(geo.n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    log(f"spike512: scene {n}^3 ready in {time.time() - t0:.1f}s")
+    ptr = lambda ts: [t.data_ptr() for t in ts]  # noqa: E731
+    ctx = sfp.Context(geo.n, geo.spacing, geo.origin, tol=args.tol, max_iters=200)
+    recs = []
+    t_all = 0.0
+    for step in range(20):
+        if step % 2 == 0 and step > 0:
+            a, mm, nn, th = modes
+            phi.copy_(torch.from_numpy(scenes._wavy_phi(geo, 0.2, (a, mm, nn, th + 0.1 * (step // 2)))))
+            torch.cuda.synchronize()
+        if step % 2 == 0:
+            for v, v0 in zip(vel, vel0):
+                v.copy_(v0)
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            st = ctx.pressure_project_device(ptr(vel), phi.data_ptr(), ptr(frac), ptr(us), None, 0.0,
+                                             out.data_ptr())
+        else:
+            ts = time.perf_counter()
+            st = ctx.magnetic_solve_device(phi.data_ptr(), ptr(hext), 4.0, 1.5 * geo.spacing,
+                                           out.data_ptr())
+        dt = time.perf_counter() - ts
+        t_all += dt
+        kind = "pressure" if step % 2 == 0 else "magnetic"
+        recs.append({"system": kind, "ms": round(dt * 1e3, 2), "iterations": st["iterations"],
+                     "rel_residual": st["rel_residual"], "converged": bool(st["converged"])})
+        log(f"spike512 step {step} {kind}: {dt * 1e3:.1f} ms, {st['iterations']} its, "
+            f"rel {st['rel_residual']:.2e}")
+    ctx.close()
+    res = {"workload": f"{n}^3 spike interface moving between solve pairs, 20 alternating "
+                       f"pressure/magnetic solves, magnetic warm-started, FP64 rel. residual {args.tol:g}",
+           "s_per_solve": round(t_all / 20, 5), "cells": geo.ncells,
+           "cells_per_s": round(geo.ncells / (t_all / 20), 1),
+           "all_converged": all(r["converged"] for r in recs), "solves": recs}
+    print(json.dumps(res), flush=True)
 
 
 def learned_leg(args, systems, total, run_step, refill_ring, geo, mg_by_sys, log):
