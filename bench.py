@@ -14,10 +14,11 @@ velocities, so each step is a full solve.
 Prints one JSON line (see the driver contract); `value` is the mean seconds
 per solve of plain MG-PCG (the library's default). `learned_fcg_s` is the
 same solves with the learned preconditioner (the package's trained U-Net
-weights) over the same step count, and `learned_over_mg_time` their ratio. The `roofline` object times the fine-level Chebyshev smoother
-kernel (the dominant kernel) with HIP events on the library's stream, over
-one more pass of each magnetic system (all cells active; pressure solves skip
-wavefronts of air cells) after the timed steps. The `cpu_baseline` object times the FP64 CPU oracle
+weights) over the same step count, and `learned_over_mg_time` their ratio.
+The `kernels` object times the hot fine-level kernels with HIP events on the
+library's stream, over one more pass of each magnetic system (all cells
+active; pressure solves skip wavefronts of air cells) after the timed steps;
+`roofline` is the one of them with the most time. The `cpu_baseline` object times the FP64 CPU oracle
 (smoothed-aggregation AMG + SPAI-0, BiCGStab for pressure, CG for magnetic,
 hierarchy rebuilt per solve) on one pressure and one magnetic solve at the
 same size on the host cores.
@@ -64,12 +65,27 @@ def parse():
     return ap.parse_args()
 
 
-SMOOTH_KERNEL = "k_smooth<6>"  # SM_READ_D | SM_WRITE_D
+# Kernels timed with HIP events in the profiled pass (sfp_profile.h ids): the
+# rocprofv3 symbol, algorithmic bytes per cell (each distinct array once per
+# launch, as the library counts them) and what the kernel does.
+HOT_KERNELS = {
+    "smooth_pair": ("k_smooth_pair", 32.0,
+                    "fine-level last two post-smoothing steps in one pass, FP32 7-point stencil: "
+                    "reads x, b, d, diag, 3 couplings, writes x"),
+    "smooth": ("k_smooth<6>", 36.0,
+               "fine-level Chebyshev step with the direction in and out, FP32 7-point stencil"),
+    "matvec64": ("k_fcg_matvec", 60.0,
+                 "flexible-CG w = A z with the FP64 operator and its 3 inner products"),
+    "update64": ("k_fcg_update", 76.0, "flexible-CG update of p, q, x, r (FP64) with r.r"),
+    "restrict": ("k_restrict", 24.5, "fine-level FP32 residual summed over 2x2x2 children"),
+}
 
 
 def summarize_profiles(d, n=256):
-    """Writes profiles/k_smooth_256cube_trace.md, profiles/traffic_smooth_256.json and
-    profiles/traffic_smooth_256cube.md from the CSVs of the measurement commands."""
+    """Writes profiles/<kernel>_256cube_trace.md, profiles/traffic_<kernel>_256.json
+    and profiles/traffic_<kernel>_256cube.md for the kernel bench.py's roofline
+    names (the one with the most event time), from the CSVs of the measurement
+    commands."""
     import csv
     import statistics
 
@@ -79,8 +95,11 @@ def summarize_profiles(d, n=256):
 
     out_dir = os.path.join(ROOT, "profiles")
     os.makedirs(out_dir, exist_ok=True)
+    bench = json.load(open(os.path.join(d, "bench_full.json")))
+    name = bench["roofline"]["name"]
+    sym, bpc, what = HOT_KERNELS[name]
     trace = rows(os.path.join(d, "prof_trace", "trace_kernel_trace.csv"))
-    sm = [r for r in trace if SMOOTH_KERNEL in r["Kernel_Name"]]
+    sm = [r for r in trace if sym in r["Kernel_Name"]]
     grid = max(int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) for r in sm)
     fine_all = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3 for r in sm
                 if int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) == grid]
@@ -90,19 +109,18 @@ def summarize_profiles(d, n=256):
     fine = [t for t in fine_all if t > 0.2 * med]
     gated = len(fine_all) - len(fine)
     stats = rows(os.path.join(d, "prof_trace", "trace_kernel_stats.csv"))
-    bench = json.load(open(os.path.join(d, "bench_full.json")))
     cells = n ** 3
-    alg = 36.0 * cells
+    alg = bpc * cells
     avg = statistics.mean(fine)
-    lines = [f"# k_smooth<READ_D|WRITE_D> at {n}^3: rocprofv3 kernel trace", "",
+    lines = [f"# {sym} at {n}^3: rocprofv3 kernel trace", "", f"{what}.", "",
              "Command (GPU box, repo root):", "",
              "    rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_trace -o trace -- \\",
              "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 4 --warmup 0 --systems m2,m8", "",
-             "(magnetic systems: every cell active, so every launch moves the full 36 B per cell)", "",
-             f"Fine-level dispatches of `{SMOOTH_KERNEL}` (grid {grid} work-items): {len(fine)} "
+             f"(magnetic systems: every cell active, so every launch moves the full {bpc:g} B per cell)", "",
+             f"Fine-level dispatches of `{sym}` (grid {grid} work-items): {len(fine)} "
              f"({gated} more returned at once: iterations enqueued past convergence, left out)",
              f"average duration {avg:.2f} us (min {min(fine):.2f}, max {max(fine):.2f});",
-             f"algorithmic bytes per launch 36 B x {cells} cells = {alg / 1e6:.1f} MB "
+             f"algorithmic bytes per launch {bpc:g} B x {cells} cells = {alg / 1e6:.1f} MB "
              f"-> {alg / (avg * 1e-6) / 1e9:.0f} GB/s = {alg / (avg * 1e-6) / 8e12:.3f} of 8 TB/s.",
              f"bench.py's HIP-event average over its profiled pass: "
              f"{bench['roofline']['avg_launch_us']:.2f} us.", "",
@@ -112,7 +130,7 @@ def summarize_profiles(d, n=256):
         name = r["Name"].replace("sfp::(anonymous namespace)::", "").split("(")[0]
         lines.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
                      f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
-    open(os.path.join(out_dir, f"k_smooth_{n}cube_trace.md"), "w").write("\n".join(lines) + "\n")
+    open(os.path.join(out_dir, f"{name}_{n}cube_trace.md"), "w").write("\n".join(lines) + "\n")
 
     def counter(sub, name):
         rs = rows(os.path.join(d, sub, [f for f in os.listdir(os.path.join(d, sub))
@@ -124,9 +142,9 @@ def summarize_profiles(d, n=256):
         return [v for v in vals if v > 0.2 * m]
 
     fetch = real([float(r["Counter_Value"]) for r in counter("prof_fetch", "FETCH_SIZE")
-                  if SMOOTH_KERNEL in r["Kernel_Name"] and int(r["Grid_Size"]) == grid * 1])
+                  if sym in r["Kernel_Name"] and int(r["Grid_Size"]) == grid * 1])
     write = real([float(r["Counter_Value"]) for r in counter("prof_write", "WRITE_SIZE")
-                  if SMOOTH_KERNEL in r["Kernel_Name"] and int(r["Grid_Size"]) == grid * 1])
+                  if sym in r["Kernel_Name"] and int(r["Grid_Size"]) == grid * 1])
     cf = [float(r["Counter_Value"]) for r in counter("prof_copy_fetch", "FETCH_SIZE")
           if "copyBuffer" in r["Kernel_Name"]]
     cw = [float(r["Counter_Value"]) for r in counter("prof_copy_write", "WRITE_SIZE")
@@ -135,13 +153,13 @@ def summarize_profiles(d, n=256):
     fs, ws = copy_kib / statistics.mean(cf), copy_kib / statistics.mean(cw)
     rd = statistics.mean(fetch) * 1024 * fs
     wr = statistics.mean(write) * 1024 * ws
-    res = {"kernel": SMOOTH_KERNEL, "cells": cells, "grid_work_items": grid,
+    res = {"kernel": sym, "cells": cells, "grid_work_items": grid,
            "fetch_kib_avg": statistics.mean(fetch), "write_kib_avg": statistics.mean(write),
            "fetch_scale": fs, "write_scale": ws, "hbm_bytes_read": rd, "hbm_bytes_written": wr,
            "hbm_bytes_per_launch": rd + wr, "algorithmic_bytes_per_launch": alg,
            "traffic_over_algorithmic": (rd + wr) / alg}
-    json.dump(res, open(os.path.join(out_dir, f"traffic_smooth_{n}.json"), "w"), indent=1)
-    md = [f"# k_smooth<READ_D|WRITE_D> at {n}^3: HBM traffic from FETCH_SIZE / WRITE_SIZE", "",
+    json.dump(res, open(os.path.join(out_dir, f"traffic_{name}_{n}.json"), "w"), indent=1)
+    md = [f"# {sym} at {n}^3: HBM traffic from FETCH_SIZE / WRITE_SIZE", "",
           "Commands (GPU box, repo root), one counter per pass:", "",
           "    rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/prof_fetch -o fetch -- \\",
           "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 2 --warmup 0 --systems m2,m8",
@@ -158,9 +176,9 @@ def summarize_profiles(d, n=256):
           f"| WRITE_SIZE (KiB, mean of {len(write)}) | {statistics.mean(write):.0f} |",
           f"| bytes read (scaled) | {rd / 1e6:.1f} MB |",
           f"| bytes written (scaled) | {wr / 1e6:.1f} MB |",
-          f"| algorithmic bytes (36 B/cell) | {alg / 1e6:.1f} MB |",
+          f"| algorithmic bytes ({bpc:g} B/cell) | {alg / 1e6:.1f} MB |",
           f"| traffic / algorithmic | {(rd + wr) / alg:.3f} |"]
-    open(os.path.join(out_dir, f"traffic_smooth_{n}cube.md"), "w").write("\n".join(md) + "\n")
+    open(os.path.join(out_dir, f"traffic_{name}_{n}cube.md"), "w").write("\n".join(md) + "\n")
     ld = os.path.join(d, "prof_learned", "learned_kernel_stats.csv")
     if os.path.exists(ld):
         st = rows(ld)
@@ -318,23 +336,31 @@ def main():
         for kid, kname in sfp.KERNEL_NAMES.items():
             kp = ctx.kernel_profile(kid)
             if kp["launches"]:
-                kernels[kname] = {"launches": kp["launches"],
-                                  "avg_us": 1e3 * kp["total_ms"] / kp["launches"],
-                                  "GBps": kp["bytes"] / (kp["total_ms"] * 1e-3) / 1e9}
-        kp = ctx.kernel_profile(sfp.KERNEL_SMOOTH)
-        if kp["launches"]:
+                gbps = kp["bytes"] / (kp["total_ms"] * 1e-3) / 1e9
+                kernels[kname] = {"symbol": HOT_KERNELS[kname][0], "launches": kp["launches"],
+                                  "total_ms": round(kp["total_ms"], 3),
+                                  "avg_us": round(1e3 * kp["total_ms"] / kp["launches"], 2),
+                                  "bytes_per_cell": HOT_KERNELS[kname][1],
+                                  "GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBS, 4)}
+        if kernels:
+            # the dominant kernel: the most event time over the profiled pass
+            name = max(kernels, key=lambda k: kernels[k]["total_ms"])
+            kp = ctx.kernel_profile([k for k, v in sfp.KERNEL_NAMES.items() if v == name][0])
+            sym, bpc, what = HOT_KERNELS[name]
             achieved = kp["bytes"] / (kp["total_ms"] * 1e-3) / 1e9
             traffic = None
-            tf = os.path.join(ROOT, "profiles", f"traffic_smooth_{n}.json")
+            tf = os.path.join(ROOT, "profiles", f"traffic_{name}_{n}.json")
             if os.path.exists(tf):
                 with open(tf) as f:
                     traffic = json.load(f).get("hbm_bytes_per_launch")
-            roof = {"kernel": "k_smooth<READ_D|WRITE_D> (fine-level FP32 Chebyshev step, 7-point stencil, 36 B/cell)",
+            roof = {"kernel": f"{sym} ({what}; {bpc:g} B/cell)", "name": name,
                     "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic,
                     "avg_launch_us": round(1e3 * kp["total_ms"] / kp["launches"], 2),
-                    "algorithmic_bytes_per_launch": kp["bytes"] / kp["launches"]}
+                    "algorithmic_bytes_per_launch": kp["bytes"] / kp["launches"],
+                    "share_of_profiled_kernel_time": round(
+                        kp["total_ms"] / sum(v["total_ms"] for v in kernels.values()), 3)}
         ctx.profile(False)
 
     by_sys = {}

@@ -210,4 +210,46 @@ sfp_status sfp_test_restrict_smoothed(sfp_context ctx, int which, const double *
     });
 }
 
+sfp_status sfp_test_smooth_pair(sfp_context ctx, int which, int level, const float *b,
+                                const float *x1, const float *d1, float cdA, float crA,
+                                float cdB, float crB, float *x3_pair, float *x3_steps) {
+    return run(ctx, [&] {
+        System &s = pick(ctx, which);
+        if (level < 0 || level >= s.num_levels()) throw ArgError("bad level");
+        if (!b || !x1 || !d1) throw ArgError("null vector");
+        Level &L = s.level(level);
+        const size_t n = L.d.cells();
+        const hipStream_t st = ctx->ws.stream;
+        DevBuf<float> db, dx1, dd, dx2, dx3;
+        db.ensure(n);
+        dx1.ensure(n);
+        dd.ensure(n);
+        dx2.ensure(n);
+        dx3.ensure(n);
+        SFP_HIP(hipMemcpyAsync(db.p, b, n * sizeof(float), hipMemcpyHostToDevice, st));
+        SFP_HIP(hipMemcpyAsync(dx1.p, x1, n * sizeof(float), hipMemcpyHostToDevice, st));
+        // the buffers the kernels skip on all-inactive rows hold zeros, as the
+        // solver's level vectors do
+        SFP_HIP(hipMemsetAsync(dx2.p, 0, n * sizeof(float), st));
+        SFP_HIP(hipMemsetAsync(dx3.p, 0, n * sizeof(float), st));
+        const Op32 A = L.op32();
+        if (x3_pair) {
+            SFP_HIP(hipMemcpyAsync(dd.p, d1, n * sizeof(float), hipMemcpyHostToDevice, st));
+            launch_smooth_pair(st, L.d, A, db.p, dx1.p, dx3.p, dd.p, cdA, crA, cdB, crB);
+            d2h(ctx, x3_pair, dx3.p, n);
+            SFP_HIP(hipStreamSynchronize(st));
+        }
+        if (x3_steps) {
+            SFP_HIP(hipMemcpyAsync(dd.p, d1, n * sizeof(float), hipMemcpyHostToDevice, st));
+            SFP_HIP(hipMemsetAsync(dx3.p, 0, n * sizeof(float), st));
+            launch_smooth(st, SM_READ_D | SM_WRITE_D, L.d, A, nullptr, db.p, dx1.p, dx2.p, dd.p, 0.f,
+                          cdA, crA, nullptr);
+            launch_smooth(st, SM_READ_D, L.d, A, nullptr, db.p, dx2.p, dx3.p, dd.p, 0.f, cdB, crB,
+                          nullptr);
+            d2h(ctx, x3_steps, dx3.p, n);
+            SFP_HIP(hipStreamSynchronize(st));
+        }
+    });
+}
+
 }  // extern "C"

@@ -62,14 +62,30 @@ __device__ __forceinline__ void block_partials(double (&v)[N], double *partials)
 }
 
 // Inside a one-block kernel: totals of nb block partials, valid in thread 0.
+// Each thread keeps 8 running sums so 8 loads are in flight at a time (tens
+// of thousands of partials in a few microseconds); the order is fixed by nb.
 template <int N>
 __device__ __forceinline__ void sum_partials(const double *partials, unsigned nb, double (&v)[N]) {
+    constexpr unsigned U = 8;
+    double acc[N][U];
 #pragma unroll
-    for (int q = 0; q < N; q++) v[q] = 0.0;
-    for (unsigned b = flat_tid(); b < nb; b += kBlock) {
+    for (int q = 0; q < N; q++)
 #pragma unroll
-        for (int q = 0; q < N; q++) v[q] += partials[(size_t)q * nb + b];
+        for (unsigned u = 0; u < U; u++) acc[q][u] = 0.0;
+    unsigned b = flat_tid();
+    for (; b + (U - 1) * kBlock < nb; b += U * kBlock) {
+#pragma unroll
+        for (unsigned u = 0; u < U; u++)
+#pragma unroll
+            for (int q = 0; q < N; q++) acc[q][u] += partials[(size_t)q * nb + b + u * kBlock];
     }
+    for (; b < nb; b += kBlock)
+#pragma unroll
+        for (int q = 0; q < N; q++) acc[q][0] += partials[(size_t)q * nb + b];
+#pragma unroll
+    for (int q = 0; q < N; q++)
+        v[q] = ((acc[q][0] + acc[q][1]) + (acc[q][2] + acc[q][3])) +
+               ((acc[q][4] + acc[q][5]) + (acc[q][6] + acc[q][7]));
     block_sum<N>(v);
 }
 

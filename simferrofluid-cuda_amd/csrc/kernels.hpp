@@ -26,6 +26,15 @@ void launch_smooth_first(hipStream_t s, Dims d, Op32 A, const double *r64, float
 void launch_smooth(hipStream_t s, int mode, Dims d, Op32 A, const double *r64, float *b,
                    const float *xin, float *xout, float *dvec, float c0, float cd, float cr,
                    const float *ecoarse, const double *gate = nullptr);
+// Two steps in one pass: x2 = x1 + d2, d2 = cdA d1 + crA D^{-1} (b - A x1), then
+// x3 = x2 + cdB d2 + crB D^{-1} (b - A x2); d2 is not stored. x3 must differ
+// from x1. Same results as the two launch_smooth calls, bit for bit.
+// pair_pays: grids on which it beats the two launches on MI355X (its 64 x 16
+// tiles overlap by one cell per side; measured equal at 128^3, faster above).
+inline bool pair_pays(Dims d) { return d.nz >= 64 && d.cells() > ((size_t)1 << 21); }
+void launch_smooth_pair(hipStream_t s, Dims d, Op32 A, const float *b, const float *x1, float *x3,
+                        const float *d1, float cdA, float crA, float cdB, float crB,
+                        const double *gate = nullptr);
 // bc = R (b - A x): sum of the residuals of the (up to 8) children.
 void launch_restrict(hipStream_t s, Dims df, Op32 A, const float *b, const float *x, float *bc,
                      const double *gate = nullptr);

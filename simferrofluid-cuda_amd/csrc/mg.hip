@@ -178,6 +178,147 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
     }
 }
 
+// Two Chebyshev steps in one pass (temporal blocking):
+//   d2 = cdA d1 + crA D^{-1} (b - A x1),   x2 = x1 + d2
+//   x3 = x2 + cdB d2 + crB D^{-1} (b - A x2)
+// with the per-cell arithmetic of two k_smooth launches (SM_READ_D |
+// SM_WRITE_D, then SM_READ_D); d2 stays in registers and is not stored. A
+// block of 64 lanes (z) x kPairRows wavefronts (y) marches over x planes,
+// computing x2 on its whole 64 x kPairRows tile one plane ahead of x3; the
+// rows exchange x2 through LDS, and x3 is written on the tile's interior
+// (lanes 1..62, rows 1..kPairRows-2), so adjacent tiles overlap by one cell
+// on each side. A wavefront whose row is inactive on a plane skips it (its
+// x2 and x3 there are 0, as the buffers already hold). x3 must not alias x1.
+//
+// LDS holds x2 of three consecutive planes: the one barrier per plane then
+// also separates a plane's reads from the store that reuses its buffer
+// (issued two planes later).
+constexpr int kPairRows = 16;
+constexpr int kPairZ = 62, kPairY = kPairRows - 2;  // interior extents (tile strides)
+
+__global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(CellGrid cg, Dims d, Op32 A,
+                                                                const float *__restrict__ b,
+                                                                const float *__restrict__ x1,
+                                                                float *__restrict__ x3,
+                                                                const float *__restrict__ d1,
+                                                                float cdA, float crA, float cdB,
+                                                                float crB, const double *gate) {
+    if (gated(gate)) return;
+    __shared__ float sx2[3][kPairRows][64];
+    const unsigned L = blockIdx.x, per = cg.gx * cg.gy;
+    const unsigned bx = L / per, rem = L - bx * per;
+    const unsigned by = rem / cg.gx, bz = rem - by * cg.gx;
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int k = (int)bz * kPairZ - 1 + lane;
+    const int j = (int)by * kPairY - 1 + row;
+    const int i0 = (int)bx * cg.planes;
+    if (i0 >= d.nx) return;  // uniform over the block
+    const int ie = min(i0 + cg.planes, d.nx);
+    const bool on = k >= 0 && k < d.nz && j >= 0 && j < d.ny;
+    const bool interior = on && lane >= 1 && lane <= kPairZ && row >= 1 && row <= kPairY;
+    const bool zlo = k > 0, zhi = k + 1 < d.nz, ylo = on && j > 0, yhi = on && j + 1 < d.ny;
+    const Strides s(d);
+    const size_t c0 = on ? d.idx(0, j, k) : 0;  // cell (i, j, k) = c0 + i sx when on
+
+    // one plane of step A, kept for step B
+    struct Plane {
+        float x2, d2, dg, bc, cxc, cxp, cym, cyp, czm, czp;
+        bool live;
+    };
+    // step-A pipeline for the next plane q: x1 at q-1 and q, cpl_x at q
+    float xm = 0.f, xc = 0.f, cxc = 0.f;
+    bool primed = false;
+    auto step_a = [&](int q) -> Plane {
+        Plane P{};
+        if (q >= d.nx) return P;
+        const size_t c = c0 + q * s.sx;
+        const float dg = on ? A.diag[c] : 0.f;
+        P.live = __any(on && dg > 0.f);
+        if (!P.live) {
+            primed = false;
+            return P;
+        }
+        if (!primed) {
+            xm = (on && q > 0) ? x1[c - s.sx] : 0.f;
+            xc = on ? x1[c] : 0.f;
+            cxc = on ? A.cpl[0][c] : 0.f;
+            primed = true;
+        }
+        const bool up = q + 1 < d.nx;
+        const float xp = (on && up) ? x1[c + s.sx] : 0.f;
+        const float cxp = (on && up) ? A.cpl[0][c + s.sx] : 0.f;
+        const float czm = on ? A.cpl[2][c] : 0.f;
+        const float cym = ylo ? A.cpl[1][c] : 0.f;
+        const float xym = ylo ? x1[c - s.sy] : 0.f;
+        const float cyp = yhi ? A.cpl[1][c + s.sy] : 0.f;
+        const float xyp = yhi ? x1[c + s.sy] : 0.f;
+        const float bc = on ? b[c] : 0.f;
+        const float dprev = on ? d1[c] : 0.f;
+        float xzm = __shfl_up(xc, 1, 64);
+        float xzp = __shfl_down(xc, 1, 64);
+        float czp = __shfl_down(czm, 1, 64);
+        if (lane == 0) xzm = (on && zlo) ? x1[c - 1] : 0.f;
+        if (lane == 63) {
+            xzp = (on && zhi) ? x1[c + 1] : 0.f;
+            czp = (on && zhi) ? A.cpl[2][c + 1] : 0.f;
+        }
+        if (on) {
+            float off = cxc * xm + cxp * xp;
+            if (j > 0) off += cym * xym;
+            if (j + 1 < d.ny) off += cyp * xyp;
+            if (zlo) off += czm * xzm;
+            if (zhi) off += czp * xzp;
+            const float r = bc - (dg * xc - off);
+            const float dn = cdA * dprev + crA * rcp_or_zero(dg) * r;
+            P.x2 = xc + dn;
+            P.d2 = dn;
+        }
+        P.dg = dg;
+        P.bc = bc;
+        P.cxc = cxc;
+        P.cxp = cxp;
+        P.cym = cym;
+        P.cyp = cyp;
+        P.czm = czm;
+        P.czp = czp;
+        xm = xc;
+        xc = xp;
+        cxc = cxp;
+        return P;
+    };
+
+    Plane pm{}, pc{};  // x2 at planes i - 1 and i
+    if (i0 > 0) pm = step_a(i0 - 1);
+    pc = step_a(i0);
+    int bc_ = i0 % 3;  // LDS buffer of plane i
+    sx2[bc_][row][lane] = pc.x2;
+    for (int i = i0; i < ie; i++) {
+        const int bn = bc_ == 2 ? 0 : bc_ + 1;  // buffer of plane i + 1
+        const Plane pn = step_a(i + 1);
+        sx2[bn][row][lane] = pn.x2;
+        __syncthreads();
+        if (pc.live) {
+            const float xzm = __shfl_up(pc.x2, 1, 64);
+            const float xzp = __shfl_down(pc.x2, 1, 64);
+            if (interior) {
+                const float xym = sx2[bc_][row - 1][lane];
+                const float xyp = sx2[bc_][row + 1][lane];
+                float off = pc.cxc * pm.x2 + pc.cxp * pn.x2;
+                if (j > 0) off += pc.cym * xym;
+                if (j + 1 < d.ny) off += pc.cyp * xyp;
+                if (zlo) off += pc.czm * xzm;
+                if (zhi) off += pc.czp * xzp;
+                const float r = pc.bc - (pc.dg * pc.x2 - off);
+                const float dn = cdB * pc.d2 + crB * rcp_or_zero(pc.dg) * r;
+                x3[c0 + i * s.sx] = pc.x2 + dn;
+            }
+        }
+        pm = pc;
+        pc = pn;
+        bc_ = bn;
+    }
+}
+
 // Residual b - A x on the fine grid, summed over each coarse cell's (up to 8)
 // children. Threads map to fine cells like the smoother (coalesced rows,
 // x-neighbours in registers, z-neighbours from adjacent lanes); the children
@@ -439,6 +580,27 @@ void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, 
             throw ArgError("unsupported smoother mode");
     }
 #undef SFP_SMOOTH_CASE
+    SFP_LAUNCH_CHECK();
+}
+
+Launch pair_launch(Dims d) {
+    Launch L;
+    L.cg.gx = (d.nz + kPairZ - 1) / kPairZ;
+    L.cg.gy = (d.ny + kPairY - 1) / kPairY;
+    const long long tiles = (long long)L.cg.gx * L.cg.gy * d.nx;
+    L.cg.planes = (int)std::min<long long>(16, std::max<long long>(1, tiles / 1024));
+    L.block = dim3(64, kPairRows, 1);
+    L.grid = dim3(L.cg.gx * L.cg.gy * ((d.nx + L.cg.planes - 1) / L.cg.planes), 1, 1);
+    return L;
+}
+
+void launch_smooth_pair(hipStream_t st, Dims d, Op32 A, const float *b, const float *x1, float *x3,
+                        const float *d1, float cdA, float crA, float cdB, float crB,
+                        const double *gate) {
+    if (x1 == x3) throw ArgError("launch_smooth_pair: x3 aliases x1");
+    const Launch L = pair_launch(d);
+    hipLaunchKernelGGL(k_smooth_pair, L.grid, L.block, 0, st, L.cg, d, A, b, x1, x3, d1, cdA, crA,
+                       cdB, crB, gate);
     SFP_LAUNCH_CHECK();
 }
 

@@ -140,6 +140,7 @@ enum KernelId {
     K_MATVEC64,     // FCG matvec + dots (FP64)
     K_UPDATE64,     // FCG vector update (FP64)
     K_RESTRICT,     // fine residual + restriction
+    K_SMOOTH_PAIR,  // fine-level last two post-smoothing steps in one pass (FP32)
     K_COUNT
 };
 

@@ -202,7 +202,11 @@ float *System::post_smooth(Workspace &ws, const Cheby &cb, int l, float *cur, co
     float *oth = cur == L.x0.p ? L.x1.p : L.x0.p;
     const int deg = cb.degree;
     const bool track = l == 0 && ws.profiling;
-    for (int s = 0; s < deg; s++) {
+    // the last two steps run as one pass where the grid is large enough to
+    // pay for the pass's tile overlap (the direction they leave is not stored:
+    // nothing reads it before the next cycle rewrites it)
+    const int fused = (deg >= 3 && pair_pays(L.d)) ? 2 : 0;
+    for (int s = 0; s < deg - fused; s++) {
         // step 0 adds the prolonged correction; the fine-level steps that read
         // and write the direction (36 algorithmic bytes per cell) are timed
         const int mode = (s == 0 ? SM_PROLONG : SM_READ_D) | (s < deg - 1 ? SM_WRITE_D : 0);
@@ -211,6 +215,14 @@ float *System::post_smooth(Workspace &ws, const Cheby &cb, int l, float *cur, co
         launch_smooth(st, mode, L.d, A, nullptr, L.b.p, cur, oth, L.dv.p, cb.cr[0], s == 0 ? 0.f : cb.cd[s],
                       s == 0 ? cb.cr[0] : cb.cr[s], s == 0 ? e : nullptr, ws.gate);
         if (t) ws.timers[K_SMOOTH].end(st);
+        std::swap(cur, oth);
+    }
+    if (fused) {
+        // reads x, b, d, diag, 3 couplings and writes x: 32 algorithmic bytes per cell
+        if (track) ws.timers[K_SMOOTH_PAIR].begin(st, 32.0 * (double)L.d.cells());
+        launch_smooth_pair(st, L.d, A, L.b.p, cur, oth, L.dv.p, cb.cd[deg - 2], cb.cr[deg - 2],
+                           cb.cd[deg - 1], cb.cr[deg - 1], ws.gate);
+        if (track) ws.timers[K_SMOOTH_PAIR].end(st);
         std::swap(cur, oth);
     }
     return cur;

@@ -22,8 +22,8 @@ _ip = C.POINTER(C.c_int)
 
 OK, ERR_ARGUMENT, ERR_INPUT, ERR_HIP, ERR_NO_DEVICE, ERR_STATE, ERR_UNSUPPORTED = range(7)
 PRECOND_MG, PRECOND_LEARNED, PRECOND_JACOBI, PRECOND_SMOOTH = 0, 1, 2, 3
-KERNEL_SMOOTH, KERNEL_MATVEC64, KERNEL_UPDATE64, KERNEL_RESTRICT = 0, 1, 2, 3
-KERNEL_NAMES = {0: "smooth", 1: "matvec64", 2: "update64", 3: "restrict"}
+KERNEL_SMOOTH, KERNEL_MATVEC64, KERNEL_UPDATE64, KERNEL_RESTRICT, KERNEL_SMOOTH_PAIR = 0, 1, 2, 3, 4
+KERNEL_NAMES = {0: "smooth", 1: "matvec64", 2: "update64", 3: "restrict", 4: "smooth_pair"}
 
 # exported symbols, per header (checked by the CPU tests)
 EXPORTS = {
@@ -39,7 +39,7 @@ EXPORTS = {
     "sfp_test.h": ["sfp_test_num_levels", "sfp_test_level_dims", "sfp_test_export_level",
                    "sfp_test_export_level32", "sfp_test_apply64", "sfp_test_precondition",
                    "sfp_test_residual", "sfp_test_solve",
-                   "sfp_test_restrict_smoothed"],
+                   "sfp_test_restrict_smoothed", "sfp_test_smooth_pair"],
     "sfp_learned.h": ["sfp_unet_param_count", "sfp_set_network", "sfp_learned_features",
                       "sfp_unet_create", "sfp_unet_destroy", "sfp_unet_set_params",
                       "sfp_unet_get_params", "sfp_unet_init_params", "sfp_unet_forward",
@@ -144,6 +144,9 @@ def lib():
         L.sfp_test_solve.argtypes = [vp, C.c_int, C.c_int, _dp, _dp, C.POINTER(Stats)]
         L.sfp_test_restrict_smoothed.restype = C.c_int
         L.sfp_test_restrict_smoothed.argtypes = [vp, C.c_int, _dp, _fp]
+        L.sfp_test_smooth_pair.restype = C.c_int
+        L.sfp_test_smooth_pair.argtypes = [vp, C.c_int, C.c_int, _fp, _fp, _fp, C.c_float, C.c_float,
+                                           C.c_float, C.c_float, _fp, _fp]
         L.sfp_unet_param_count.restype = C.c_longlong
         L.sfp_unet_param_count.argtypes = []
         L.sfp_set_network.restype = C.c_int
@@ -443,6 +446,18 @@ class Context:
         rc = np.zeros(self.level_dims(which, 1), np.float32)
         self._check(lib().sfp_test_restrict_smoothed(self._h, int(which), _d(r), _f(rc)))
         return rc
+
+    def smooth_pair(self, which, level, b, x1, d1, cd_a, cr_a, cd_b, cr_b):
+        """The last two post-smoothing steps on a level, from the one-pass
+        kernel and from two single steps: (x3_pair, x3_steps), FP32."""
+        shape = self.level_dims(which, level)
+        f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32).reshape(shape)  # noqa: E731
+        b, x1, d1 = f32(b), f32(x1), f32(d1)
+        x3p = np.zeros(shape, np.float32)
+        x3s = np.zeros(shape, np.float32)
+        self._check(lib().sfp_test_smooth_pair(self._h, int(which), int(level), _f(b), _f(x1), _f(d1),
+                                               cd_a, cr_a, cd_b, cr_b, _f(x3p), _f(x3s)))
+        return x3p, x3s
 
 
 def unet_param_count() -> int:

@@ -1,0 +1,93 @@
+"""GPU tests of the one-pass pair of post-smoothing steps through the C ABI:
+bit-identical to two single Chebyshev steps on pressure and magnetic levels
+(ragged extents, inactive cells, one and several x planes per block, grids
+narrower than a tile), and within FP32 rounding of an FP64 numpy statement of
+the two steps on the same FP32 operator."""
+import numpy as np
+import pytest
+
+import sfp
+from sfp import scenes
+
+pytestmark = pytest.mark.gpu
+
+COEF = (0.31, 0.72, 0.27, 0.66)  # cdA, crA, cdB, crB
+
+
+def np_apply(diag, cpl, x):
+    """A x for the full-grid operator: diagonal plus A[c, c - e_a] = -cpl_a[c]
+    (the coupling stored on the lower face of c; unused on the first layer)."""
+    y = diag * x
+    for a in range(3):
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[a] = slice(0, -1)
+        hi[a] = slice(1, None)
+        lo, hi = tuple(lo), tuple(hi)
+        c = cpl[a][hi]
+        y[hi] -= c * x[lo]
+        y[lo] -= c * x[hi]
+    return y
+
+
+def np_two_steps(diag, cpl, b, x1, d1, cd_a, cr_a, cd_b, cr_b):
+    inv = np.where(diag > 0, 1.0 / np.where(diag > 0, diag, 1.0), 0.0)
+    d2 = cd_a * d1 + cr_a * inv * (b - np_apply(diag, cpl, x1))
+    x2 = x1 + d2
+    return x2 + cd_b * d2 + cr_b * inv * (b - np_apply(diag, cpl, x2))
+
+
+def _system(kind, n, seed):
+    if kind == "pressure":
+        sc = scenes.random_pressure_scene(n, seed=seed)
+        ctx = sfp.Context.for_scene(sc)
+        ctx.pressure_project(sc)
+        return ctx, 0
+    sc = scenes.magnetic_scene(n, seed=seed)
+    ctx = sfp.Context.for_scene(sc)
+    ctx.magnetic_solve(sc)
+    return ctx, 1
+
+
+@pytest.mark.parametrize("kind,n,level", [
+    ("pressure", (72, 66, 130), 0),   # one plane per block, 3 x 5 tiles, air cells
+    ("pressure", (128, 128, 128), 0),  # several planes per block
+    ("magnetic", 64, 0),               # every cell active
+    ("pressure", (96, 80, 112), 1),    # coarse level narrower than a tile in z
+])
+def test_pair_matches_two_steps(kind, n, level):
+    print(f"[case] smooth pair {kind} {n} level {level}", flush=True)
+    ctx, which = _system(kind, n, 3)
+    with ctx:
+        op = ctx.export_level32(which, level)
+        diag = op["diag"]
+        act = diag > 0
+        assert act.any()
+        if kind == "pressure" and level == 0:
+            assert not act.all()
+        rng = np.random.default_rng(11)
+        vec = lambda s: (rng.uniform(-s, s, diag.shape) * act).astype(np.float32)  # noqa: E731
+        b, x1, d1 = vec(1.0), vec(1.0), vec(0.1)
+        x3p, x3s = ctx.smooth_pair(which, level, b, x1, d1, *COEF)
+    np.testing.assert_array_equal(x3p, x3s)
+    assert np.all(x3p[~act] == 0)
+    f64 = lambda a: a.astype(np.float64)  # noqa: E731
+    ref = np_two_steps(f64(diag), [f64(c) for c in op["cpl"]], f64(b), f64(x1), f64(d1), *COEF)
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(x3p, ref, rtol=0, atol=2e-5 * scale)
+    # a planted mistake in the statement (step B's direction term dropped) is caught
+    bad = np_two_steps(f64(diag), [f64(c) for c in op["cpl"]], f64(b), f64(x1), f64(d1),
+                       COEF[0], COEF[1], 0.0, COEF[3])
+    assert np.abs(bad - x3p).max() > 100 * 2e-5 * scale
+
+
+def test_pair_path_solves_to_tolerance(oracle):
+    """A grid where post-smoothing takes the one-pass path (nz >= 64, above
+    128^3 cells) converges to the tolerance on the oracle's system."""
+    sc = scenes.pressure_scene(144, 1)
+    print("[case] pressure 144^3 through the one-pass post-smoothing", flush=True)
+    with sfp.Context.for_scene(sc) as ctx:
+        v, p, st = ctx.pressure_project(sc)
+    assert st["converged"]
+    ps = oracle.pressure_system(sc)
+    assert oracle.rel_residual(ps["A"], ps["rhs"], p.ravel()[ps["mat2grid"]]) <= 1e-8
