@@ -64,10 +64,12 @@ void launch_fcg_matvec(hipStream_t s, Dims d, Op64 A, const float *active, const
                        const double *r, const double *qprev, double *w, int first,
                        double *partials, KrylovScalars *sc, const double *gate = nullptr);
 // p = z - beta p; q = w - beta q; x += alpha p; r -= alpha q; rr = r.r
-// (all-inactive wavefronts skipped when `active` is given).
+// (all-inactive wavefronts skipped when `active` is given); b32 (or null)
+// receives r rounded to FP32.
 void launch_fcg_update(hipStream_t s, Dims d, const float *active, const float *z,
-                       const double *w, double *p, double *q, double *x, double *r, int first,
-                       double *partials, KrylovScalars *sc, const double *gate = nullptr);
+                       const double *w, double *p, double *q, double *x, double *r, float *b32,
+                       int first, double *partials, KrylovScalars *sc,
+                       const double *gate = nullptr);
 // z = D^{-1} r on active cells (Jacobi preconditioner).
 void launch_jacobi(hipStream_t s, Dims d, Op32 A, const double *r, float *z,
                    const double *gate = nullptr);

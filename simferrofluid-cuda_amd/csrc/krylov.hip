@@ -183,7 +183,9 @@ __global__ __launch_bounds__(kBlock) void k_fcg_matvec(CellGrid cg, Dims d, Op64
 // The update is pointwise: it runs over the flat cell index with two
 // consecutive cells per lane (16-byte loads and stores of the FP64 vectors),
 // one pair per thread (kUpdateCells per block). A wavefront whose 128 cells
-// are all inactive is skipped (p, q, x, r stay 0 there).
+// are all inactive is skipped (p, q, x, r stay 0 there). With b32 it also
+// stores the new residual rounded to FP32: the right-hand side the next
+// V-cycle starts from, which then need not read the FP64 residual.
 constexpr unsigned kUpdateCells = 2 * kBlock;
 
 __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__restrict__ active,
@@ -192,7 +194,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__
                                                        double *__restrict__ p,
                                                        double *__restrict__ q,
                                                        double *__restrict__ x,
-                                                       double *__restrict__ r, int first,
+                                                       double *__restrict__ r,
+                                                       float *__restrict__ b32, int first,
                                                        double *__restrict__ partials,
                                                        const KrylovScalars *__restrict__ sc,
                                                        const double *gate) {
@@ -231,6 +234,10 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__
             st_stream((double2v *)(q + c), qc);
             st_stream((double2v *)(x + c), xn);
             st_stream((double2v *)(r + c), rc);
+            if (b32) {
+                const float2v bf = {(float)rc.x, (float)rc.y};
+                *(float2v *)(b32 + c) = bf;
+            }
             v[0] += rc.x * rc.x;
             v[0] += rc.y * rc.y;
         } else if (c < n) {
@@ -242,6 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__
             x[c] += alpha * pc;
             const double rc = r[c] - alpha * qc;
             r[c] = rc;
+            if (b32) b32[c] = (float)rc;
             v[0] += rc * rc;
         }
     }
@@ -368,15 +376,16 @@ void launch_fcg_matvec(hipStream_t st, Dims d, Op64 A, const float *active, cons
 }
 
 void launch_fcg_update(hipStream_t st, Dims d, const float *active, const float *z,
-                       const double *w, double *p, double *q, double *x, double *r, int first,
-                       double *partials, KrylovScalars *sc, const double *gate) {
+                       const double *w, double *p, double *q, double *x, double *r, float *b32,
+                       int first, double *partials, KrylovScalars *sc, const double *gate) {
     // two cells per lane: 16-byte (8-byte for z) aligned vectors
     for (const void *a : {(const void *)w, (const void *)p, (const void *)q, (const void *)x, (const void *)r})
         if ((uintptr_t)a % 16) throw ArgError("launch_fcg_update: FP64 vector not 16-byte aligned");
-    if ((uintptr_t)z % 8) throw ArgError("launch_fcg_update: z not 8-byte aligned");
+    if ((uintptr_t)z % 8 || (uintptr_t)b32 % 8)
+        throw ArgError("launch_fcg_update: z or b32 not 8-byte aligned");
     const unsigned nb = update_blocks(d);
     hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(kBlock), 0, st, d.cells(), active, z, w, p, q,
-                       x, r, first, partials, sc, gate);
+                       x, r, b32, first, partials, sc, gate);
     SFP_LAUNCH_CHECK();
     finalize(st, FIN_UPDATE, nb, first, partials, sc, gate);
 }

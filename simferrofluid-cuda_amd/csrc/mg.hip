@@ -95,12 +95,24 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
     if (i0 >= d.nx) return;  // uniform over the block
     const Strides s(d);
     auto bval = [&](size_t q) -> float { return r64 ? (float)r64[q] : b[q]; };
-    // iterate value at a neighbour point q (cell coordinates qi, qj, qk)
-    auto xval = [&](size_t q, int qi, int qj, int qk) -> float {
+    // coarse-grid offsets (prolongation): plane stride, and this thread's
+    // column and its y neighbours' columns within a coarse plane
+    const size_t dcsx = (size_t)dc.ny * dc.nz;
+    size_t eo = 0, eym = 0, eyp = 0;
+    if constexpr (kProlong) {
+        if (on) {
+            eo = dc.idx(0, j >> 1, k >> 1);
+            if (j > 0) eym = dc.idx(0, (j - 1) >> 1, k >> 1);
+            if (j + 1 < d.ny) eyp = dc.idx(0, (j + 1) >> 1, k >> 1);
+        }
+    }
+    // iterate value at a neighbour point q (cell coordinates qi, qj, qk; for
+    // the prolongation, ecol = the point's column offset on the coarse grid)
+    auto xval = [&](size_t q, int qi, size_t ecol) -> float {
         if constexpr (kFirst) {
             return c0 * rcp_or_zero(A.diag[q]) * bval(q);
         } else if constexpr (kProlong) {
-            return xin[q] + e[dc.idx(qi >> 1, qj >> 1, qk >> 1)];
+            return xin[q] + e[(size_t)(qi >> 1) * dcsx + ecol];
         } else {
             return xin[q];
         }
@@ -120,20 +132,20 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
             continue;
         }
         if (!primed) {
-            xm = (on && i > 0) ? xval(c - s.sx, i - 1, j, k) : 0.f;
-            xc = on ? xval(c, i, j, k) : 0.f;
+            xm = (on && i > 0) ? xval(c - s.sx, i - 1, eo) : 0.f;
+            xc = on ? xval(c, i, eo) : 0.f;
             cxc = on ? A.cpl[0][c] : 0.f;
             primed = true;
         }
         // every load of the plane is issued before the lane exchange
         const bool up = i + 1 < d.nx;
-        const float xp = (on && up) ? xval(c + s.sx, i + 1, j, k) : 0.f;
+        const float xp = (on && up) ? xval(c + s.sx, i + 1, eo) : 0.f;
         const float cxp = (on && up) ? A.cpl[0][c + s.sx] : 0.f;
         const float czm = on ? A.cpl[2][c] : 0.f;
         const float cym = ylo ? A.cpl[1][c] : 0.f;
-        const float xym = ylo ? xval(c - s.sy, i, j - 1, k) : 0.f;
+        const float xym = ylo ? xval(c - s.sy, i, eym) : 0.f;
         const float cyp = yhi ? A.cpl[1][c + s.sy] : 0.f;
-        const float xyp = yhi ? xval(c + s.sy, i, j + 1, k) : 0.f;
+        const float xyp = yhi ? xval(c + s.sy, i, eyp) : 0.f;
         float bc = 0.f;
         if constexpr (kFirst) {
             bc = on ? bval(c) : 0.f;
@@ -147,9 +159,9 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
         float xzm = __shfl_up(xc, 1, tz);
         float xzp = __shfl_down(xc, 1, tz);
         float czp = __shfl_down(czm, 1, tz);
-        if (lane == 0) xzm = (on && zlo) ? xval(c - 1, i, j, k - 1) : 0.f;
+        if (lane == 0) xzm = (on && zlo) ? xval(c - 1, i, kProlong ? dc.idx(0, j >> 1, (k - 1) >> 1) : 0) : 0.f;
         if (lane == tz - 1) {
-            xzp = (on && zhi) ? xval(c + 1, i, j, k + 1) : 0.f;
+            xzp = (on && zhi) ? xval(c + 1, i, kProlong ? dc.idx(0, j >> 1, (k + 1) >> 1) : 0) : 0.f;
             czp = (on && zhi) ? A.cpl[2][c + 1] : 0.f;
         }
         if (on) {

@@ -334,6 +334,11 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     // than stall_ratio; then the multigrid V-cycle
     bool learned = fine && cfg.precond == 1 && learned_ok(ws);
     if (learned) prepare_learned(ws);
+    // The V-cycle preconditioners start from the residual rounded to FP32 in
+    // the level's b: the update stores it, so the cycle after an update reads
+    // b instead of the FP64 residual (the Jacobi preconditioner reads r).
+    float *b32 = cfg.precond != 2 ? lv_[level].b.p : nullptr;
+    bool b_ready = false;  // lv_[level].b holds r rounded
     info.precond_used = learned ? 1 : (cfg.precond == 1 ? 0 : cfg.precond);
     while (true) {
         if (rel <= cfg.tol) {
@@ -346,6 +351,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
             rel = std::sqrt(ws.host_sc->rr / bb);
             checked = true;
             first = true;
+            b_ready = false;
             continue;
         }
         if (info.iterations >= cfg.max_iters) break;
@@ -363,22 +369,26 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
             ws.gate = &ws.scalars.p->stop;
         }
         for (int it = 0; it < batch; it++) {
+            const double *r64 = b_ready ? nullptr : r_.p;  // null: the cycle reads b
             const float *z;
             if (learned)
-                z = precondition_learned(ws, cfg, r_.p);
+                z = precondition_learned(ws, cfg, r64);
             else if (fine)
-                z = precondition(ws, cfg, r_.p);
+                z = precondition(ws, cfg, r64);
             else
-                z = vlevel(ws, Cheby::make(cfg.cheby_degree, cfg.cheby_ratio), level, r_.p);
+                z = vlevel(ws, Cheby::make(cfg.cheby_degree, cfg.cheby_ratio), level, r64);
             if (profiling) ws.timers[K_MATVEC64].begin(st, (4.0 + 32.0 + 8.0 + 8.0 + 8.0) * cells);
             launch_fcg_matvec(st, d, A, lv_[level].diag32.p, z, r_.p, q_.p, w_.p, first ? 1 : 0,
                               ws.partials.p, ws.scalars.p, ws.gate);
             if (profiling) ws.timers[K_MATVEC64].end(st);
-            if (profiling) ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 5 + 8.0 * 4) * cells);
-            launch_fcg_update(st, d, lv_[level].diag32.p, z, w_.p, p_.p, q_.p, x_.p, r_.p,
+            // z, w, p, q, x, r in; p, q, x, r (and b) out
+            if (profiling)
+                ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 5 + 8.0 * 4 + (b32 ? 4.0 : 0.0)) * cells);
+            launch_fcg_update(st, d, lv_[level].diag32.p, z, w_.p, p_.p, q_.p, x_.p, r_.p, b32,
                               first ? 1 : 0, ws.partials.p, ws.scalars.p, ws.gate);
             if (profiling) ws.timers[K_UPDATE64].end(st);
             first = false;
+            b_ready = b32 != nullptr;
         }
         ws.read_scalars();
         ws.gate = nullptr;
