@@ -69,9 +69,12 @@ def parse():
 # rocprofv3 symbol, algorithmic bytes per cell (each distinct array once per
 # launch, as the library counts them) and what the kernel does.
 HOT_KERNELS = {
-    "smooth_pair": ("k_smooth_pair", 32.0,
+    "smooth_pair": ("k_smooth_pair<false>", 32.0,
                     "fine-level last two post-smoothing steps in one pass, FP32 7-point stencil: "
                     "reads x, b, d, diag, 3 couplings, writes x"),
+    "smooth_restrict": ("k_smooth_pair<true>", 32.5,
+                        "fine-level last pre-smoothing step and the residual restriction in one "
+                        "pass, FP32: reads x, b, d, diag, 3 couplings, writes x and the coarse b"),
     "smooth": ("k_smooth<6>", 36.0,
                "fine-level Chebyshev step with the direction in and out, FP32 7-point stencil"),
     "matvec64": ("k_fcg_matvec", 60.0,

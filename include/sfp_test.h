@@ -48,6 +48,16 @@ sfp_status sfp_test_restrict_smoothed(sfp_context ctx, int which, const double *
 sfp_status sfp_test_smooth_pair(sfp_context ctx, int which, int level, const float *b,
                                 const float *x1, const float *d1, float cdA, float crA,
                                 float cdB, float crB, float *x3_pair, float *x3_steps);
+/* The last pre-smoothing step and the restriction on level `level` (which
+ * must have a coarser level): x2 = x1 + cd d1 + cr D^{-1} (b - A x1) and
+ * bc = R (b - A x2), from the one-pass kernel (x2_fused, bc_fused) and from
+ * the separate step and restriction (x2_steps, bc_steps). Host FP32 arrays:
+ * b, x1, d1, x2_* of the level's cells, bc_* of the next level's. Any output
+ * may be NULL. */
+sfp_status sfp_test_smooth_restrict(sfp_context ctx, int which, int level, const float *b,
+                                    const float *x1, const float *d1, float cd, float cr,
+                                    float *x2_fused, float *bc_fused, float *x2_steps,
+                                    float *bc_steps);
 
 #ifdef __cplusplus
 }

@@ -252,4 +252,47 @@ sfp_status sfp_test_smooth_pair(sfp_context ctx, int which, int level, const flo
     });
 }
 
+sfp_status sfp_test_smooth_restrict(sfp_context ctx, int which, int level, const float *b,
+                                    const float *x1, const float *d1, float cd, float cr,
+                                    float *x2_fused, float *bc_fused, float *x2_steps,
+                                    float *bc_steps) {
+    return run(ctx, [&] {
+        System &s = pick(ctx, which);
+        if (level < 0 || level + 1 >= s.num_levels()) throw ArgError("bad level");
+        if (!b || !x1 || !d1) throw ArgError("null vector");
+        Level &L = s.level(level);
+        const Dims dc = s.level(level + 1).d;
+        const size_t n = L.d.cells(), nc = dc.cells();
+        const hipStream_t st = ctx->ws.stream;
+        DevBuf<float> db, dx1, dd, dx2, dbc;
+        db.ensure(n);
+        dx1.ensure(n);
+        dd.ensure(n);
+        dx2.ensure(n);
+        dbc.ensure(nc);
+        SFP_HIP(hipMemcpyAsync(db.p, b, n * sizeof(float), hipMemcpyHostToDevice, st));
+        SFP_HIP(hipMemcpyAsync(dx1.p, x1, n * sizeof(float), hipMemcpyHostToDevice, st));
+        SFP_HIP(hipMemcpyAsync(dd.p, d1, n * sizeof(float), hipMemcpyHostToDevice, st));
+        const Op32 A = L.op32();
+        for (int fused = 1; fused >= 0; fused--) {
+            float *xo = fused ? x2_fused : x2_steps;
+            float *bo = fused ? bc_fused : bc_steps;
+            if (!xo && !bo) continue;
+            // zeros where the kernels skip all-inactive rows, as in the solver
+            SFP_HIP(hipMemsetAsync(dx2.p, 0, n * sizeof(float), st));
+            SFP_HIP(hipMemsetAsync(dbc.p, 0, nc * sizeof(float), st));
+            if (fused) {
+                launch_smooth_restrict(st, L.d, A, db.p, dx1.p, dx2.p, dd.p, cd, cr, dbc.p);
+            } else {
+                launch_smooth(st, SM_READ_D, L.d, A, nullptr, db.p, dx1.p, dx2.p, dd.p, 0.f, cd, cr,
+                              nullptr);
+                launch_restrict(st, L.d, A, db.p, dx2.p, dbc.p);
+            }
+            d2h(ctx, xo, dx2.p, n);
+            d2h(ctx, bo, dbc.p, nc);
+            SFP_HIP(hipStreamSynchronize(st));
+        }
+    });
+}
+
 }  // extern "C"

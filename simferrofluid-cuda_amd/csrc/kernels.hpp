@@ -35,6 +35,13 @@ inline bool pair_pays(Dims d) { return d.nz >= 64 && d.cells() > ((size_t)1 << 2
 void launch_smooth_pair(hipStream_t s, Dims d, Op32 A, const float *b, const float *x1, float *x3,
                         const float *d1, float cdA, float crA, float cdB, float crB,
                         const double *gate = nullptr);
+// The last pre-smoothing step and the restriction in one pass:
+// x2 = x1 + cd d1 + cr D^{-1} (b - A x1), then bc = R (b - A x2) (the coarse
+// right-hand side, as launch_restrict). Same results as launch_smooth
+// (SM_READ_D) then launch_restrict, bit for bit; x2 must differ from x1.
+void launch_smooth_restrict(hipStream_t s, Dims d, Op32 A, const float *b, const float *x1,
+                            float *x2, const float *d1, float cd, float cr, float *bc,
+                            const double *gate = nullptr);
 // bc = R (b - A x): sum of the residuals of the (up to 8) children.
 void launch_restrict(hipStream_t s, Dims df, Op32 A, const float *b, const float *x, float *bc,
                      const double *gate = nullptr);

@@ -190,17 +190,26 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
     }
 }
 
-// Two Chebyshev steps in one pass (temporal blocking):
-//   d2 = cdA d1 + crA D^{-1} (b - A x1),   x2 = x1 + d2
-//   x3 = x2 + cdB d2 + crB D^{-1} (b - A x2)
-// with the per-cell arithmetic of two k_smooth launches (SM_READ_D |
-// SM_WRITE_D, then SM_READ_D); d2 stays in registers and is not stored. A
-// block of 64 lanes (z) x kPairRows wavefronts (y) marches over x planes,
-// computing x2 on its whole 64 x kPairRows tile one plane ahead of x3; the
-// rows exchange x2 through LDS, and x3 is written on the tile's interior
+// Two passes over the grid in one (temporal blocking). Step A is a
+// Chebyshev step from iterate x1 and direction d1:
+//   d2 = cdA d1 + crA D^{-1} (b - A x1),   x2 = x1 + d2.
+// Step B is, with kRestrict false, a second Chebyshev step
+//   x3 = x2 + cdB d2 + crB D^{-1} (b - A x2)   (written to xo; d2 not stored),
+// and with kRestrict true the restriction of the residual of x2,
+//   bc = R (b - A x2)   (x2 written to xo),
+// with the per-cell arithmetic of the separate launches (k_smooth with
+// SM_READ_D | SM_WRITE_D, then SM_READ_D; or k_smooth with SM_READ_D, then
+// k_restrict), so results agree bit for bit.
+//
+// A block of 64 lanes (z) x kPairRows wavefronts (y) marches over x planes,
+// computing x2 on its whole 64 x kPairRows tile one plane ahead of step B;
+// the rows exchange x2 through LDS, and step B covers the tile's interior
 // (lanes 1..62, rows 1..kPairRows-2), so adjacent tiles overlap by one cell
-// on each side. A wavefront whose row is inactive on a plane skips it (its
-// x2 and x3 there are 0, as the buffers already hold). x3 must not alias x1.
+// on each side. The interior starts on even z, y and x and has even extents,
+// so each coarse cell's 8 children lie in one block: z pairs through the lane
+// pair, y pairs through LDS, x pairs in a register. A wavefront whose row is
+// inactive on a plane skips its work there (x2 there is 0, as the buffers
+// already hold). xo must not alias x1.
 //
 // LDS holds x2 of three consecutive planes: the one barrier per plane then
 // also separates a plane's reads from the store that reuses its buffer
@@ -208,15 +217,15 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
 constexpr int kPairRows = 16;
 constexpr int kPairZ = 62, kPairY = kPairRows - 2;  // interior extents (tile strides)
 
-__global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(CellGrid cg, Dims d, Op32 A,
-                                                                const float *__restrict__ b,
-                                                                const float *__restrict__ x1,
-                                                                float *__restrict__ x3,
-                                                                const float *__restrict__ d1,
-                                                                float cdA, float crA, float cdB,
-                                                                float crB, const double *gate) {
+template <bool kRestrict>
+__global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
+    CellGrid cg, Dims d, Op32 A, const float *__restrict__ b, const float *__restrict__ x1,
+    float *__restrict__ xo, const float *__restrict__ d1, float cdA, float crA, float cdB,
+    float crB, Dims dc, float *__restrict__ bc, const double *gate) {
     if (gated(gate)) return;
     __shared__ float sx2[3][kPairRows][64];
+    __shared__ float scy[2][kPairRows][64];  // cpl_y of the plane step B works on
+    __shared__ float sres[kRestrict ? kPairRows : 1][64];
     const unsigned L = blockIdx.x, per = cg.gx * cg.gy;
     const unsigned bx = L / per, rem = L - bx * per;
     const unsigned by = rem / cg.gx, bz = rem - by * cg.gx;
@@ -232,9 +241,12 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(CellGrid cg, 
     const Strides s(d);
     const size_t c0 = on ? d.idx(0, j, k) : 0;  // cell (i, j, k) = c0 + i sx when on
 
-    // one plane of step A, kept for step B
+    // one plane of step A, kept for step B. Step B takes the couplings of the
+    // upper faces from where they are stored: cpl_x(i+1) from the next
+    // plane, cpl_y(j+1) from the next row (LDS), cpl_z(k+1) from the next
+    // lane; where that cell is inactive the coupling is 0 either way.
     struct Plane {
-        float x2, d2, dg, bc, cxc, cxp, cym, cyp, czm, czp;
+        float x2, d2, dg, bc, cxc, cym, czm;
         bool live;
     };
     // step-A pipeline for the next plane q: x1 at q-1 and q, cpl_x at q
@@ -283,16 +295,17 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(CellGrid cg, 
             const float r = bc - (dg * xc - off);
             const float dn = cdA * dprev + crA * rcp_or_zero(dg) * r;
             P.x2 = xc + dn;
-            P.d2 = dn;
+            if constexpr (kRestrict) {
+                if (interior) xo[c] = P.x2;
+            } else {
+                P.d2 = dn;
+            }
         }
         P.dg = dg;
         P.bc = bc;
         P.cxc = cxc;
-        P.cxp = cxp;
         P.cym = cym;
-        P.cyp = cyp;
         P.czm = czm;
-        P.czp = czp;
         xm = xc;
         xc = xp;
         cxc = cxp;
@@ -302,32 +315,74 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(CellGrid cg, 
     Plane pm{}, pc{};  // x2 at planes i - 1 and i
     if (i0 > 0) pm = step_a(i0 - 1);
     pc = step_a(i0);
-    int bc_ = i0 % 3;  // LDS buffer of plane i
-    sx2[bc_][row][lane] = pc.x2;
+    int bcur = i0 % 3;  // LDS buffer of plane i
+    sx2[bcur][row][lane] = pc.x2;
+    scy[i0 & 1][row][lane] = pc.cym;
+    float acc = 0.f;  // restriction: sum over the x pair
+    // the lane of each coarse cell's first child stores the sum
+    const bool writer = kRestrict && interior && (k & 1) == 0 && (j & 1) == 0;
+    const size_t bco = writer ? dc.idx(0, j >> 1, k >> 1) : 0;
+    const size_t dcsx = (size_t)dc.ny * dc.nz;
     for (int i = i0; i < ie; i++) {
-        const int bn = bc_ == 2 ? 0 : bc_ + 1;  // buffer of plane i + 1
+        const int bnext = bcur == 2 ? 0 : bcur + 1;  // buffer of plane i + 1
         const Plane pn = step_a(i + 1);
-        sx2[bn][row][lane] = pn.x2;
+        sx2[bnext][row][lane] = pn.x2;
+        scy[(i + 1) & 1][row][lane] = pn.cym;
         __syncthreads();
-        if (pc.live) {
-            const float xzm = __shfl_up(pc.x2, 1, 64);
-            const float xzp = __shfl_down(pc.x2, 1, 64);
-            if (interior) {
-                const float xym = sx2[bc_][row - 1][lane];
-                const float xyp = sx2[bc_][row + 1][lane];
-                float off = pc.cxc * pm.x2 + pc.cxp * pn.x2;
-                if (j > 0) off += pc.cym * xym;
-                if (j + 1 < d.ny) off += pc.cyp * xyp;
-                if (zlo) off += pc.czm * xzm;
-                if (zhi) off += pc.czp * xzp;
-                const float r = pc.bc - (pc.dg * pc.x2 - off);
-                const float dn = cdB * pc.d2 + crB * rcp_or_zero(pc.dg) * r;
-                x3[c0 + i * s.sx] = pc.x2 + dn;
+        const float cxp = pn.cxc;
+        const float czp = __shfl_down(pc.czm, 1, 64);  // lane 63 is never interior
+        if constexpr (!kRestrict) {
+            if (pc.live) {
+                const float xzm = __shfl_up(pc.x2, 1, 64);
+                const float xzp = __shfl_down(pc.x2, 1, 64);
+                if (interior) {
+                    const float xym = sx2[bcur][row - 1][lane];
+                    const float xyp = sx2[bcur][row + 1][lane];
+                    const float cyp = scy[i & 1][row + 1][lane];
+                    float off = pc.cxc * pm.x2 + cxp * pn.x2;
+                    if (j > 0) off += pc.cym * xym;
+                    if (j + 1 < d.ny) off += cyp * xyp;
+                    if (zlo) off += pc.czm * xzm;
+                    if (zhi) off += czp * xzp;
+                    const float r = pc.bc - (pc.dg * pc.x2 - off);
+                    const float dn = cdB * pc.d2 + crB * rcp_or_zero(pc.dg) * r;
+                    xo[c0 + i * s.sx] = pc.x2 + dn;
+                }
+            }
+        } else {
+            // residual of x2 on the interior's active cells (the operand
+            // order of k_restrict), then the 2 x 2 x 2 sums
+            float r = 0.f;
+            if (pc.live) {
+                const float xzm = __shfl_up(pc.x2, 1, 64);
+                const float xzp = __shfl_down(pc.x2, 1, 64);
+                if (interior && pc.dg > 0.f) {
+                    const float xym = sx2[bcur][row - 1][lane];
+                    const float xyp = sx2[bcur][row + 1][lane];
+                    const float cyp = scy[i & 1][row + 1][lane];
+                    float off = 0.f;
+                    if (i > 0) off += pc.cxc * pm.x2;
+                    if (i + 1 < d.nx) off += cxp * pn.x2;
+                    if (j > 0) off += pc.cym * xym;
+                    if (j + 1 < d.ny) off += cyp * xyp;
+                    if (zlo) off += pc.czm * xzm;
+                    if (zhi) off += czp * xzp;
+                    r = pc.bc - (pc.dg * pc.x2 - off);
+                }
+            }
+            r += __shfl_down(r, 1, 64);  // z pair: lanes of even k take k + 1
+            sres[row][lane] = r;
+            __syncthreads();
+            if (row < kPairRows - 1) r += sres[row + 1][lane];  // y pair (rows of even j)
+            acc += r;                                             // x pair
+            if ((i & 1) == 1 || i + 1 == d.nx) {
+                if (writer) bc[bco + (size_t)(i >> 1) * dcsx] = acc;
+                acc = 0.f;
             }
         }
         pm = pc;
         pc = pn;
-        bc_ = bn;
+        bcur = bnext;
     }
 }
 
@@ -595,12 +650,13 @@ void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, 
     SFP_LAUNCH_CHECK();
 }
 
-Launch pair_launch(Dims d) {
+Launch pair_launch(Dims d, bool even_planes) {
     Launch L;
     L.cg.gx = (d.nz + kPairZ - 1) / kPairZ;
     L.cg.gy = (d.ny + kPairY - 1) / kPairY;
     const long long tiles = (long long)L.cg.gx * L.cg.gy * d.nx;
     L.cg.planes = (int)std::min<long long>(16, std::max<long long>(1, tiles / 1024));
+    if (even_planes && (L.cg.planes & 1)) L.cg.planes++;  // x pairs stay in one block
     L.block = dim3(64, kPairRows, 1);
     L.grid = dim3(L.cg.gx * L.cg.gy * ((d.nx + L.cg.planes - 1) / L.cg.planes), 1, 1);
     return L;
@@ -610,9 +666,19 @@ void launch_smooth_pair(hipStream_t st, Dims d, Op32 A, const float *b, const fl
                         const float *d1, float cdA, float crA, float cdB, float crB,
                         const double *gate) {
     if (x1 == x3) throw ArgError("launch_smooth_pair: x3 aliases x1");
-    const Launch L = pair_launch(d);
-    hipLaunchKernelGGL(k_smooth_pair, L.grid, L.block, 0, st, L.cg, d, A, b, x1, x3, d1, cdA, crA,
-                       cdB, crB, gate);
+    const Launch L = pair_launch(d, false);
+    hipLaunchKernelGGL(k_smooth_pair<false>, L.grid, L.block, 0, st, L.cg, d, A, b, x1, x3, d1, cdA,
+                       crA, cdB, crB, Dims{}, nullptr, gate);
+    SFP_LAUNCH_CHECK();
+}
+
+void launch_smooth_restrict(hipStream_t st, Dims d, Op32 A, const float *b, const float *x1,
+                            float *x2, const float *d1, float cd, float cr, float *bc,
+                            const double *gate) {
+    if (x1 == x2) throw ArgError("launch_smooth_restrict: x2 aliases x1");
+    const Launch L = pair_launch(d, true);
+    hipLaunchKernelGGL(k_smooth_pair<true>, L.grid, L.block, 0, st, L.cg, d, A, b, x1, x2, d1, cd,
+                       cr, 0.f, 0.f, d.coarse(), bc, gate);
     SFP_LAUNCH_CHECK();
 }
 

@@ -141,6 +141,7 @@ enum KernelId {
     K_UPDATE64,     // FCG vector update (FP64)
     K_RESTRICT,     // fine residual + restriction
     K_SMOOTH_PAIR,  // fine-level last two post-smoothing steps in one pass (FP32)
+    K_SMOOTH_RESTRICT,  // fine-level last pre-smoothing step + restriction in one pass
     K_COUNT
 };
 

@@ -180,17 +180,29 @@ float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) 
         if (track) ws.timers[K_SMOOTH].end(st);
         std::swap(cur, oth);
     };
+    // the last step and the restriction run as one pass where the grid pays
+    // for the pass's tile overlap
+    const bool fuse_restrict = deg >= 3 && pair_pays(L.d);
     if (deg == 1) {
         launch_smooth_first(st, L.d, A, r64, L.b.p, cur, cb.cr[0], ws.gate);
     } else {
         // steps 0 and 1 in one pass; then steps 2 .. deg-1
         smooth(SM_FIRST | (deg > 2 ? SM_WRITE_D : 0), cb.cd[1], cb.cr[1], r64, nullptr);
-        for (int s = 2; s < deg; s++)
+        for (int s = 2; s < deg - (fuse_restrict ? 1 : 0); s++)
             smooth(SM_READ_D | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s], nullptr, nullptr);
     }
-    if (fine && ws.profiling) ws.timers[K_RESTRICT].begin(st, (24.0 + 0.5) * cells);
-    launch_restrict(st, L.d, A, L.b.p, cur, lv_[l + 1].b.p, ws.gate);
-    if (fine && ws.profiling) ws.timers[K_RESTRICT].end(st);
+    if (fuse_restrict) {
+        // reads x, b, d, diag, 3 couplings; writes x and the coarse b
+        if (fine && ws.profiling) ws.timers[K_SMOOTH_RESTRICT].begin(st, (32.0 + 0.5) * cells);
+        launch_smooth_restrict(st, L.d, A, L.b.p, cur, oth, L.dv.p, cb.cd[deg - 1], cb.cr[deg - 1],
+                               lv_[l + 1].b.p, ws.gate);
+        if (fine && ws.profiling) ws.timers[K_SMOOTH_RESTRICT].end(st);
+        std::swap(cur, oth);
+    } else {
+        if (fine && ws.profiling) ws.timers[K_RESTRICT].begin(st, (24.0 + 0.5) * cells);
+        launch_restrict(st, L.d, A, L.b.p, cur, lv_[l + 1].b.p, ws.gate);
+        if (fine && ws.profiling) ws.timers[K_RESTRICT].end(st);
+    }
     const float *e = vlevel(ws, cb, l + 1, nullptr);
     return post_smooth(ws, cb, l, cur, e);
 }

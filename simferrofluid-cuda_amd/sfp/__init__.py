@@ -22,8 +22,10 @@ _ip = C.POINTER(C.c_int)
 
 OK, ERR_ARGUMENT, ERR_INPUT, ERR_HIP, ERR_NO_DEVICE, ERR_STATE, ERR_UNSUPPORTED = range(7)
 PRECOND_MG, PRECOND_LEARNED, PRECOND_JACOBI, PRECOND_SMOOTH = 0, 1, 2, 3
-KERNEL_SMOOTH, KERNEL_MATVEC64, KERNEL_UPDATE64, KERNEL_RESTRICT, KERNEL_SMOOTH_PAIR = 0, 1, 2, 3, 4
-KERNEL_NAMES = {0: "smooth", 1: "matvec64", 2: "update64", 3: "restrict", 4: "smooth_pair"}
+KERNEL_SMOOTH, KERNEL_MATVEC64, KERNEL_UPDATE64, KERNEL_RESTRICT = 0, 1, 2, 3
+KERNEL_SMOOTH_PAIR, KERNEL_SMOOTH_RESTRICT = 4, 5
+KERNEL_NAMES = {0: "smooth", 1: "matvec64", 2: "update64", 3: "restrict", 4: "smooth_pair",
+                5: "smooth_restrict"}
 
 # exported symbols, per header (checked by the CPU tests)
 EXPORTS = {
@@ -39,7 +41,8 @@ EXPORTS = {
     "sfp_test.h": ["sfp_test_num_levels", "sfp_test_level_dims", "sfp_test_export_level",
                    "sfp_test_export_level32", "sfp_test_apply64", "sfp_test_precondition",
                    "sfp_test_residual", "sfp_test_solve",
-                   "sfp_test_restrict_smoothed", "sfp_test_smooth_pair"],
+                   "sfp_test_restrict_smoothed", "sfp_test_smooth_pair",
+                   "sfp_test_smooth_restrict"],
     "sfp_learned.h": ["sfp_unet_param_count", "sfp_set_network", "sfp_learned_features",
                       "sfp_unet_create", "sfp_unet_destroy", "sfp_unet_set_params",
                       "sfp_unet_get_params", "sfp_unet_init_params", "sfp_unet_forward",
@@ -144,6 +147,9 @@ def lib():
         L.sfp_test_solve.argtypes = [vp, C.c_int, C.c_int, _dp, _dp, C.POINTER(Stats)]
         L.sfp_test_restrict_smoothed.restype = C.c_int
         L.sfp_test_restrict_smoothed.argtypes = [vp, C.c_int, _dp, _fp]
+        L.sfp_test_smooth_restrict.restype = C.c_int
+        L.sfp_test_smooth_restrict.argtypes = [vp, C.c_int, C.c_int, _fp, _fp, _fp, C.c_float,
+                                               C.c_float, _fp, _fp, _fp, _fp]
         L.sfp_test_smooth_pair.restype = C.c_int
         L.sfp_test_smooth_pair.argtypes = [vp, C.c_int, C.c_int, _fp, _fp, _fp, C.c_float, C.c_float,
                                            C.c_float, C.c_float, _fp, _fp]
@@ -458,6 +464,20 @@ class Context:
         self._check(lib().sfp_test_smooth_pair(self._h, int(which), int(level), _f(b), _f(x1), _f(d1),
                                                cd_a, cr_a, cd_b, cr_b, _f(x3p), _f(x3s)))
         return x3p, x3s
+
+    def smooth_restrict(self, which, level, b, x1, d1, cd, cr):
+        """The last pre-smoothing step and the restriction on a level, from
+        the one-pass kernel and from the separate launches:
+        (x2_fused, bc_fused, x2_steps, bc_steps), FP32."""
+        shape = self.level_dims(which, level)
+        cshape = self.level_dims(which, level + 1)
+        f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32).reshape(shape)  # noqa: E731
+        b, x1, d1 = f32(b), f32(x1), f32(d1)
+        out = [np.zeros(shape, np.float32), np.zeros(cshape, np.float32),
+               np.zeros(shape, np.float32), np.zeros(cshape, np.float32)]
+        self._check(lib().sfp_test_smooth_restrict(self._h, int(which), int(level), _f(b), _f(x1),
+                                                   _f(d1), cd, cr, *[_f(a) for a in out]))
+        return tuple(out)
 
 
 def unet_param_count() -> int:

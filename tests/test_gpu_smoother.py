@@ -1,8 +1,9 @@
-"""GPU tests of the one-pass pair of post-smoothing steps through the C ABI:
-bit-identical to two single Chebyshev steps on pressure and magnetic levels
-(ragged extents, inactive cells, one and several x planes per block, grids
-narrower than a tile), and within FP32 rounding of an FP64 numpy statement of
-the two steps on the same FP32 operator."""
+"""GPU tests of the one-pass kernels of the V-cycle through the C ABI: the
+pair of post-smoothing steps and the last pre-smoothing step fused with the
+restriction. Each is bit-identical to the separate launches on pressure and
+magnetic levels (ragged and odd extents, inactive cells, one and several x
+planes per block, grids narrower than a tile) and within FP32 rounding of an
+FP64 numpy statement on the same FP32 operator."""
 import numpy as np
 import pytest
 
@@ -35,6 +36,14 @@ def np_two_steps(diag, cpl, b, x1, d1, cd_a, cr_a, cd_b, cr_b):
     d2 = cd_a * d1 + cr_a * inv * (b - np_apply(diag, cpl, x1))
     x2 = x1 + d2
     return x2 + cd_b * d2 + cr_b * inv * (b - np_apply(diag, cpl, x2))
+
+
+def np_restrict(r):
+    """Sum over each coarse cell's (up to 8) children."""
+    pad = [(0, n % 2) for n in r.shape]
+    rp = np.pad(r, pad)
+    nx, ny, nz = (n // 2 for n in rp.shape)
+    return rp.reshape(nx, 2, ny, 2, nz, 2).sum(axis=(1, 3, 5))
 
 
 def _system(kind, n, seed):
@@ -81,8 +90,40 @@ def test_pair_matches_two_steps(kind, n, level):
     assert np.abs(bad - x3p).max() > 100 * 2e-5 * scale
 
 
+@pytest.mark.parametrize("kind,n,level", [
+    ("pressure", (72, 66, 130), 0),   # two planes per block, air cells
+    ("pressure", (129, 67, 65), 0),   # odd extents: single-child coarse cells
+    ("magnetic", 64, 0),
+    ("pressure", (96, 80, 112), 1),
+])
+def test_smooth_restrict_matches_separate_launches(kind, n, level):
+    print(f"[case] smooth+restrict {kind} {n} level {level}", flush=True)
+    ctx, which = _system(kind, n, 4)
+    with ctx:
+        op = ctx.export_level32(which, level)
+        diag = op["diag"]
+        act = diag > 0
+        rng = np.random.default_rng(12)
+        vec = lambda s: (rng.uniform(-s, s, diag.shape) * act).astype(np.float32)  # noqa: E731
+        b, x1, d1 = vec(1.0), vec(1.0), vec(0.1)
+        x2f, bcf, x2s, bcs = ctx.smooth_restrict(which, level, b, x1, d1, COEF[0], COEF[1])
+    np.testing.assert_array_equal(x2f, x2s)
+    np.testing.assert_array_equal(bcf, bcs)
+    f64 = lambda a: a.astype(np.float64)  # noqa: E731
+    dg, cpl = f64(diag), [f64(c) for c in op["cpl"]]
+    inv = np.where(dg > 0, 1.0 / np.where(dg > 0, dg, 1.0), 0.0)
+    x2 = f64(x1) + COEF[0] * f64(d1) + COEF[1] * inv * (f64(b) - np_apply(dg, cpl, f64(x1)))
+    np.testing.assert_allclose(x2f, x2, rtol=0, atol=2e-5 * np.abs(x2).max())
+    res = np.where(act, f64(b) - np_apply(dg, cpl, f64(x2f)), 0.0)
+    want = np_restrict(res)
+    assert bcf.shape == want.shape
+    np.testing.assert_allclose(bcf, want, rtol=0, atol=1e-4 * np.abs(want).max())
+    # a planted mistake (restricting with one child per coarse cell) is caught
+    assert np.abs(res[::2, ::2, ::2] - bcf).max() > 1e-2 * np.abs(want).max()
+
+
 def test_pair_path_solves_to_tolerance(oracle):
-    """A grid where post-smoothing takes the one-pass path (nz >= 64, above
+    """A grid where the V-cycle takes the one-pass kernels (nz >= 64, above
     128^3 cells) converges to the tolerance on the oracle's system."""
     sc = scenes.pressure_scene(144, 1)
     print("[case] pressure 144^3 through the one-pass post-smoothing", flush=True)
