@@ -23,10 +23,11 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Sums v[q] over the block; the totals are valid in thread 0.
-template <int N>
+// Sums v[q] over a block of T threads; the totals are valid in thread 0.
+template <int N, int T = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[N]) {
-    __shared__ double sh[N][kWaves];
+    constexpr int W = T / 64;
+    __shared__ double sh[N][W];
     const int tid = flat_tid();
 #pragma unroll
     for (int q = 0; q < N; q++) v[q] = wave_sum(v[q]);
@@ -40,7 +41,7 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
         for (int q = 0; q < N; q++) {
             double s = sh[q][0];
 #pragma unroll
-            for (int w = 1; w < kWaves; w++) s += sh[q][w];
+            for (int w = 1; w < W; w++) s += sh[q][w];
             v[q] = s;
         }
     }
@@ -61,32 +62,38 @@ __device__ __forceinline__ void block_partials(double (&v)[N], double *partials)
     }
 }
 
-// Inside a one-block kernel: totals of nb block partials, valid in thread 0.
-// Each thread keeps 8 running sums so 8 loads are in flight at a time (tens
-// of thousands of partials in a few microseconds); the order is fixed by nb.
-template <int N>
+// Inside a one-block kernel of T threads: totals of nb block partials, valid
+// in thread 0. Each thread keeps U running sums so U loads per sum are in
+// flight at a time (tens of thousands of partials in a few microseconds);
+// the order is fixed by nb and T.
+template <int N, int T = kBlock, unsigned U = 8>
 __device__ __forceinline__ void sum_partials(const double *partials, unsigned nb, double (&v)[N]) {
-    constexpr unsigned U = 8;
+    static_assert(U == 8 || U == 16, "U is 8 or 16");
     double acc[N][U];
 #pragma unroll
     for (int q = 0; q < N; q++)
 #pragma unroll
         for (unsigned u = 0; u < U; u++) acc[q][u] = 0.0;
     unsigned b = flat_tid();
-    for (; b + (U - 1) * kBlock < nb; b += U * kBlock) {
+    for (; b + (U - 1) * T < nb; b += U * T) {
 #pragma unroll
         for (unsigned u = 0; u < U; u++)
 #pragma unroll
-            for (int q = 0; q < N; q++) acc[q][u] += partials[(size_t)q * nb + b + u * kBlock];
+            for (int q = 0; q < N; q++) acc[q][u] += partials[(size_t)q * nb + b + u * T];
     }
-    for (; b < nb; b += kBlock)
+    for (; b < nb; b += T)
 #pragma unroll
         for (int q = 0; q < N; q++) acc[q][0] += partials[(size_t)q * nb + b];
 #pragma unroll
-    for (int q = 0; q < N; q++)
+    for (int q = 0; q < N; q++) {
+        if constexpr (U == 16) {
+#pragma unroll
+            for (unsigned u = 0; u < 8; u++) acc[q][u] += acc[q][u + 8];
+        }
         v[q] = ((acc[q][0] + acc[q][1]) + (acc[q][2] + acc[q][3])) +
                ((acc[q][4] + acc[q][5]) + (acc[q][6] + acc[q][7]));
-    block_sum<N>(v);
+    }
+    block_sum<N, T>(v);
 }
 
 // Tile of a cell-parallel block: z index k (lanes), y index j (wavefronts)

@@ -284,21 +284,25 @@ __global__ __launch_bounds__(kBlock) void k_count_active(CellGrid cg, Dims d,
 }
 
 // One block: add the partials and advance the scalar recurrences.
-__global__ __launch_bounds__(kBlock) void k_finalize(int op, unsigned nb, int first,
+// The finalize kernel is one block of kFinalizeThreads (wide, so the partial
+// sums of the largest grids take few rounds of loads).
+constexpr int kFinalizeThreads = 1024;
+
+__global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int op, unsigned nb, int first,
                                                      const double *__restrict__ partials,
                                                      KrylovScalars *sc, const double *gate) {
     if (gated(gate)) return;
     double v[3] = {0.0, 0.0, 0.0};
     if (op == FIN_MATVEC) {
-        sum_partials<3>(partials, nb, v);
+        sum_partials<3, kFinalizeThreads, 16>(partials, nb, v);
     } else if (op == FIN_RESIDUAL) {
         double t[2];
-        sum_partials<2>(partials, nb, t);
+        sum_partials<2, kFinalizeThreads, 16>(partials, nb, t);
         v[0] = t[0];
         v[1] = t[1];
     } else {
         double t[1];
-        sum_partials<1>(partials, nb, t);
+        sum_partials<1, kFinalizeThreads, 16>(partials, nb, t);
         v[0] = t[0];
     }
     if (flat_tid() != 0) return;
@@ -333,7 +337,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, unsigned nb, int fi
 
 void finalize(hipStream_t st, int op, unsigned nb, int first, const double *partials,
               KrylovScalars *sc, const double *gate = nullptr) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, st, op, nb, first, partials, sc, gate);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, st, op, nb, first, partials,
+                       sc, gate);
     SFP_LAUNCH_CHECK();
 }
 
