@@ -58,6 +58,9 @@ sfp_status sfp_test_smooth_restrict(sfp_context ctx, int which, int level, const
                                     const float *x1, const float *d1, float cd, float cr,
                                     float *x2_fused, float *bc_fused, float *x2_steps,
                                     float *bc_steps);
+/* Run the V-cycle's small levels (at most 4096 cells) in one launch (enable
+ * != 0, the default) or level by level. Results are the same bit for bit. */
+sfp_status sfp_test_set_vcycle_tail(sfp_context ctx, int enable);
 
 #ifdef __cplusplus
 }

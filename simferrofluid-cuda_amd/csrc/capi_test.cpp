@@ -295,4 +295,8 @@ sfp_status sfp_test_smooth_restrict(sfp_context ctx, int which, int level, const
     });
 }
 
+sfp_status sfp_test_set_vcycle_tail(sfp_context ctx, int enable) {
+    return run(ctx, [&] { ctx->ws.vcycle_tail = enable != 0; });
+}
+
 }  // extern "C"

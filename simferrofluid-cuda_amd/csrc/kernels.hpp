@@ -52,6 +52,38 @@ void launch_coarsen(hipStream_t s, Dims df, Op64 Af, const float *diag32f, Dims 
                     float *const cpl32c[3]);
 // Dense inverse of the coarsest operator (n = cells <= kMaxCoarse) into ainv (n*n floats).
 constexpr int kMaxCoarse = 64;
+
+// The coarse end of a V-cycle in one launch of one block: from level `first`
+// (cells <= kTailMaxCells) down to the coarsest and back, with the per-cell
+// arithmetic of the separate smoothing, restriction and dense-solve launches.
+// The level's result is left in lv[0].x1 (tail_result); the other level
+// vectors are not touched. The first level's b must already hold its
+// right-hand side.
+constexpr size_t kTailMaxCells = 4096;
+constexpr int kTailMaxLevels = 8;
+struct TailLevel {
+    Dims d;
+    Op32 A;
+    float *b, *x0, *x1, *dv;
+};
+struct TailArgs {
+    int nlev = 0;  // levels in the tail, the last the coarsest
+    TailLevel lv[kTailMaxLevels];
+    int deg = 3;
+    float cd[8] = {0}, cr[8] = {0};
+    const float *ainv = nullptr;
+};
+// The tail keeps 8 arrays of every level it covers in LDS (gfx950: up to
+// 160 KiB per block).
+constexpr size_t kTailLdsBytes = 149504;
+inline size_t tail_lds_bytes(const TailArgs &t) {
+    size_t b = 0;
+    for (int l = 0; l < t.nlev; l++) b += 8 * sizeof(float) * t.lv[l].d.cells();
+    return b;
+}
+void launch_vcycle_tail(hipStream_t s, const TailArgs &t, const double *gate = nullptr);
+// The buffer that holds the tail's result for its first level.
+inline float *tail_result(const TailArgs &t) { return t.lv[0].x1; }
 void launch_coarse_invert(hipStream_t s, Dims d, Op64 A, float *ainv);
 void launch_coarse_solve(hipStream_t s, int n, const float *ainv, const float *b, float *x,
                          const double *gate = nullptr);

@@ -165,11 +165,11 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
             czp = (on && zhi) ? A.cpl[2][c + 1] : 0.f;
         }
         if (on) {
-            float off = cxc * xm + cxp * xp;
-            if (j > 0) off += cym * xym;
-            if (j + 1 < d.ny) off += cyp * xyp;
-            if (zlo) off += czm * xzm;
-            if (zhi) off += czp * xzp;
+            float off = fmaf(cxc, xm, cxp * xp);
+            if (j > 0) off = fmaf(cym, xym, off);
+            if (j + 1 < d.ny) off = fmaf(cyp, xyp, off);
+            if (zlo) off = fmaf(czm, xzm, off);
+            if (zhi) off = fmaf(czp, xzp, off);
             if constexpr (kFirst) {
                 if (r64) b[c] = bc;
             }
@@ -178,9 +178,9 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
             if constexpr (kProlong) {
                 if (!(dg > 0.f)) x0 = xin_c;
             }
-            const float r = bc - (dg * x0 - off);
+            const float r = bc - fmaf(dg, x0, -off);
             if constexpr (kFirst) dprev = x0;
-            const float dn = cd * dprev + cr * rcp_or_zero(dg) * r;
+            const float dn = fmaf(cd, dprev, cr * rcp_or_zero(dg) * r);
             xout[c] = x0 + dn;
             if constexpr ((MODE & SM_WRITE_D) != 0) dvec[c] = dn;
         }
@@ -211,9 +211,9 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
 // inactive on a plane skips its work there (x2 there is 0, as the buffers
 // already hold). xo must not alias x1.
 //
-// LDS holds x2 of three consecutive planes: the one barrier per plane then
-// also separates a plane's reads from the store that reuses its buffer
-// (issued two planes later).
+// LDS holds x2 (and cpl_y) of three consecutive planes: the one barrier per
+// plane then also separates a plane's reads from the store that reuses its
+// buffer (issued two planes later, by a wavefront at most one barrier ahead).
 constexpr int kPairRows = 16;
 constexpr int kPairZ = 62, kPairY = kPairRows - 2;  // interior extents (tile strides)
 
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
     float crB, Dims dc, float *__restrict__ bc, const double *gate) {
     if (gated(gate)) return;
     __shared__ float sx2[3][kPairRows][64];
-    __shared__ float scy[2][kPairRows][64];  // cpl_y of the plane step B works on
+    __shared__ float scy[3][kPairRows][64];  // cpl_y of the same three planes
     __shared__ float sres[kRestrict ? kPairRows : 1][64];
     const unsigned L = blockIdx.x, per = cg.gx * cg.gy;
     const unsigned bx = L / per, rem = L - bx * per;
@@ -287,13 +287,13 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
             czp = (on && zhi) ? A.cpl[2][c + 1] : 0.f;
         }
         if (on) {
-            float off = cxc * xm + cxp * xp;
-            if (j > 0) off += cym * xym;
-            if (j + 1 < d.ny) off += cyp * xyp;
-            if (zlo) off += czm * xzm;
-            if (zhi) off += czp * xzp;
-            const float r = bc - (dg * xc - off);
-            const float dn = cdA * dprev + crA * rcp_or_zero(dg) * r;
+            float off = fmaf(cxc, xm, cxp * xp);
+            if (j > 0) off = fmaf(cym, xym, off);
+            if (j + 1 < d.ny) off = fmaf(cyp, xyp, off);
+            if (zlo) off = fmaf(czm, xzm, off);
+            if (zhi) off = fmaf(czp, xzp, off);
+            const float r = bc - fmaf(dg, xc, -off);
+            const float dn = fmaf(cdA, dprev, crA * rcp_or_zero(dg) * r);
             P.x2 = xc + dn;
             if constexpr (kRestrict) {
                 if (interior) xo[c] = P.x2;
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
     pc = step_a(i0);
     int bcur = i0 % 3;  // LDS buffer of plane i
     sx2[bcur][row][lane] = pc.x2;
-    scy[i0 & 1][row][lane] = pc.cym;
+    scy[bcur][row][lane] = pc.cym;
     float acc = 0.f;  // restriction: sum over the x pair
     // the lane of each coarse cell's first child stores the sum
     const bool writer = kRestrict && interior && (k & 1) == 0 && (j & 1) == 0;
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
         const int bnext = bcur == 2 ? 0 : bcur + 1;  // buffer of plane i + 1
         const Plane pn = step_a(i + 1);
         sx2[bnext][row][lane] = pn.x2;
-        scy[(i + 1) & 1][row][lane] = pn.cym;
+        scy[bnext][row][lane] = pn.cym;
         __syncthreads();
         const float cxp = pn.cxc;
         const float czp = __shfl_down(pc.czm, 1, 64);  // lane 63 is never interior
@@ -338,14 +338,14 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
                 if (interior) {
                     const float xym = sx2[bcur][row - 1][lane];
                     const float xyp = sx2[bcur][row + 1][lane];
-                    const float cyp = scy[i & 1][row + 1][lane];
-                    float off = pc.cxc * pm.x2 + cxp * pn.x2;
-                    if (j > 0) off += pc.cym * xym;
-                    if (j + 1 < d.ny) off += cyp * xyp;
-                    if (zlo) off += pc.czm * xzm;
-                    if (zhi) off += czp * xzp;
-                    const float r = pc.bc - (pc.dg * pc.x2 - off);
-                    const float dn = cdB * pc.d2 + crB * rcp_or_zero(pc.dg) * r;
+                    const float cyp = scy[bcur][row + 1][lane];
+                    float off = fmaf(pc.cxc, pm.x2, cxp * pn.x2);
+                    if (j > 0) off = fmaf(pc.cym, xym, off);
+                    if (j + 1 < d.ny) off = fmaf(cyp, xyp, off);
+                    if (zlo) off = fmaf(pc.czm, xzm, off);
+                    if (zhi) off = fmaf(czp, xzp, off);
+                    const float r = pc.bc - fmaf(pc.dg, pc.x2, -off);
+                    const float dn = fmaf(cdB, pc.d2, crB * rcp_or_zero(pc.dg) * r);
                     xo[c0 + i * s.sx] = pc.x2 + dn;
                 }
             }
@@ -359,15 +359,15 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
                 if (interior && pc.dg > 0.f) {
                     const float xym = sx2[bcur][row - 1][lane];
                     const float xyp = sx2[bcur][row + 1][lane];
-                    const float cyp = scy[i & 1][row + 1][lane];
+                    const float cyp = scy[bcur][row + 1][lane];
                     float off = 0.f;
-                    if (i > 0) off += pc.cxc * pm.x2;
-                    if (i + 1 < d.nx) off += cxp * pn.x2;
-                    if (j > 0) off += pc.cym * xym;
-                    if (j + 1 < d.ny) off += cyp * xyp;
-                    if (zlo) off += pc.czm * xzm;
-                    if (zhi) off += czp * xzp;
-                    r = pc.bc - (pc.dg * pc.x2 - off);
+                    if (i > 0) off = fmaf(pc.cxc, pm.x2, off);
+                    if (i + 1 < d.nx) off = fmaf(cxp, pn.x2, off);
+                    if (j > 0) off = fmaf(pc.cym, xym, off);
+                    if (j + 1 < d.ny) off = fmaf(cyp, xyp, off);
+                    if (zlo) off = fmaf(pc.czm, xzm, off);
+                    if (zhi) off = fmaf(czp, xzp, off);
+                    r = pc.bc - fmaf(pc.dg, pc.x2, -off);
                 }
             }
             r += __shfl_down(r, 1, 64);  // z pair: lanes of even k take k + 1
@@ -435,13 +435,13 @@ __global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 
             }
             if (on && dg > 0.f) {
                 float off = 0.f;
-                if (i > 0) off += cxc * xm;
-                if (up) off += cxp * xp;
-                if (j > 0) off += A.cpl[1][c] * x[c - s.sy];
-                if (j + 1 < df.ny) off += A.cpl[1][c + s.sy] * x[c + s.sy];
-                if (zlo) off += czm * xzm;
-                if (zhi) off += czp * xzp;
-                r = b[c] - (dg * xc - off);
+                if (i > 0) off = fmaf(cxc, xm, off);
+                if (up) off = fmaf(cxp, xp, off);
+                if (j > 0) off = fmaf(A.cpl[1][c], x[c - s.sy], off);
+                if (j + 1 < df.ny) off = fmaf(A.cpl[1][c + s.sy], x[c + s.sy], off);
+                if (zlo) off = fmaf(czm, xzm, off);
+                if (zhi) off = fmaf(czp, xzp, off);
+                r = b[c] - fmaf(dg, xc, -off);
             }
             xm = xc;
             xc = xp;
@@ -605,9 +605,228 @@ __global__ __launch_bounds__(kBlock) void k_coarse_solve(int n, const float *ain
     __syncthreads();
     for (int r = tid; r < n; r += kBlock) {
         float acc = 0.f;
-        for (int c = 0; c < n; c++) acc += ainv[r * n + c] * bs[c];
+        for (int c = 0; c < n; c++) acc = fmaf(ainv[r * n + c], bs[c], acc);
         x[r] = acc;
     }
+}
+
+// ---------------------------------------------------------------- V-cycle tail
+// One block runs the coarse end of the V-cycle (levels of at most
+// kTailMaxCells cells) with the levels' arrays in LDS: each sweep covers a
+// level's cells with the block's threads and ends at a barrier. The per-cell arithmetic is that of k_smooth,
+// k_restrict and k_coarse_solve (neighbour values equal to what those take
+// from adjacent lanes), so the result is the same bit for bit.
+constexpr int kTailThreads = 1024;
+constexpr int kTailLdsFloats = (int)(kTailLdsBytes / sizeof(float));
+
+template <int MODE>
+__device__ void tail_smooth(const TailLevel &L, const float *__restrict__ xin, float *__restrict__ xout,
+                            float c0, float cd, float cr, const float *__restrict__ e, Dims dc) {
+    constexpr bool kFirst = (MODE & SM_FIRST) != 0;
+    constexpr bool kProlong = (MODE & SM_PROLONG) != 0;
+    const Dims d = L.d;
+    const Op32 A = L.A;
+    const float *__restrict__ b = L.b;
+    const Strides s(d);
+    const int n = (int)d.cells(), nyz = d.ny * d.nz;
+    auto xval = [&](size_t q, int qi, int qj, int qk) -> float {
+        if constexpr (kFirst) {
+            return c0 * rcp_or_zero(A.diag[q]) * b[q];
+        } else if constexpr (kProlong) {
+            return xin[q] + e[dc.idx(qi >> 1, qj >> 1, qk >> 1)];
+        } else {
+            return xin[q];
+        }
+    };
+    for (int t = threadIdx.x; t < n; t += kTailThreads) {
+        const int i = t / nyz, rem = t - i * nyz, j = rem / d.nz, k = rem - j * d.nz;
+        const size_t c = (size_t)t;
+        const float dg = A.diag[c];
+        const bool up = i + 1 < d.nx, zlo = k > 0, zhi = k + 1 < d.nz;
+        const float xm = i > 0 ? xval(c - s.sx, i - 1, j, k) : 0.f;
+        const float xc = xval(c, i, j, k);
+        const float cxc = A.cpl[0][c];
+        const float xp = up ? xval(c + s.sx, i + 1, j, k) : 0.f;
+        const float cxp = up ? A.cpl[0][c + s.sx] : 0.f;
+        const float czm = A.cpl[2][c];
+        const float cym = j > 0 ? A.cpl[1][c] : 0.f;
+        const float xym = j > 0 ? xval(c - s.sy, i, j - 1, k) : 0.f;
+        const float cyp = j + 1 < d.ny ? A.cpl[1][c + s.sy] : 0.f;
+        const float xyp = j + 1 < d.ny ? xval(c + s.sy, i, j + 1, k) : 0.f;
+        const float bc = b[c];
+        float dprev = 0.f;
+        if constexpr ((MODE & SM_READ_D) != 0) dprev = L.dv[c];
+        const float xzm = zlo ? xval(c - 1, i, j, k - 1) : 0.f;
+        const float xzp = zhi ? xval(c + 1, i, j, k + 1) : 0.f;
+        const float czp = zhi ? A.cpl[2][c + 1] : 0.f;
+        float x0 = xc;
+        if constexpr (kProlong) {
+            if (!(dg > 0.f)) x0 = xin[c];
+        }
+        float off = fmaf(cxc, xm, cxp * xp);
+        if (j > 0) off = fmaf(cym, xym, off);
+        if (j + 1 < d.ny) off = fmaf(cyp, xyp, off);
+        if (zlo) off = fmaf(czm, xzm, off);
+        if (zhi) off = fmaf(czp, xzp, off);
+        const float r = bc - fmaf(dg, x0, -off);
+        if constexpr (kFirst) dprev = x0;
+        const float dn = fmaf(cd, dprev, cr * rcp_or_zero(dg) * r);
+        xout[c] = x0 + dn;
+        if constexpr ((MODE & SM_WRITE_D) != 0) L.dv[c] = dn;
+    }
+    __syncthreads();
+}
+
+// bc = R (b - A x): k_restrict's residual per child and its pairwise sums
+// (z pair, then y pair, then x pair onto 0).
+__device__ void tail_restrict(const TailLevel &F, const float *__restrict__ x, float *__restrict__ bcv,
+                              Dims dc) {
+    const Dims d = F.d;
+    const Op32 A = F.A;
+    const Strides s(d);
+    const int nc = (int)dc.cells(), nyz = dc.ny * dc.nz;
+    // residual of a child (0 outside the grid or inactive): all loads from
+    // clamped addresses, then a select, so the 8 children's loads are in
+    // flight together
+    auto res = [&](int iu, int ju, int ku) -> float {
+        const bool in = iu < d.nx && ju < d.ny && ku < d.nz;
+        const int i = min(iu, d.nx - 1), j = min(ju, d.ny - 1), k = min(ku, d.nz - 1);
+        const size_t c = d.idx(i, j, k);
+        const float dg = A.diag[c];
+        const bool up = i + 1 < d.nx, zlo = k > 0, zhi = k + 1 < d.nz;
+        const float xc = x[c], bcv = F.b[c];
+        const float cxm = A.cpl[0][c], cym = A.cpl[1][c], czm = A.cpl[2][c];
+        const float xm = i > 0 ? x[c - s.sx] : 0.f, ym = j > 0 ? x[c - s.sy] : 0.f;
+        const float zm = zlo ? x[c - 1] : 0.f;
+        const float cxp = up ? A.cpl[0][c + s.sx] : 0.f, xp = up ? x[c + s.sx] : 0.f;
+        const bool yup = j + 1 < d.ny;
+        const float cyp = yup ? A.cpl[1][c + s.sy] : 0.f, yp = yup ? x[c + s.sy] : 0.f;
+        const float czp = zhi ? A.cpl[2][c + 1] : 0.f, zp = zhi ? x[c + 1] : 0.f;
+        float off = 0.f;
+        if (i > 0) off = fmaf(cxm, xm, off);
+        if (up) off = fmaf(cxp, xp, off);
+        if (j > 0) off = fmaf(cym, ym, off);
+        if (yup) off = fmaf(cyp, yp, off);
+        if (zlo) off = fmaf(czm, zm, off);
+        if (zhi) off = fmaf(czp, zp, off);
+        const float r = bcv - fmaf(dg, xc, -off);
+        return (in && dg > 0.f) ? r : 0.f;
+    };
+    for (int t = threadIdx.x; t < nc; t += kTailThreads) {
+        const int I = t / nyz, rem = t - I * nyz, J = rem / dc.nz, K = rem - J * dc.nz;
+        float acc = 0.f;
+        for (int di = 0; di < 2 && 2 * I + di < d.nx; di++) {
+            const int i = 2 * I + di;
+            float zp[2];
+            for (int dj = 0; dj < 2; dj++) {
+                const int j = 2 * J + dj;
+                const float r0 = res(i, j, 2 * K);
+                zp[dj] = d.nz > 1 ? r0 + res(i, j, 2 * K + 1) : r0;
+            }
+            acc += zp[0] + zp[1];
+        }
+        bcv[t] = acc;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kTailThreads) void k_vcycle_tail(TailArgs t, const double *gate) {
+    if (gated(gate)) return;
+    // every array of the tail's levels lives in LDS for the whole cycle (8
+    // per level, levels one after another): the operators and the first
+    // level's b are copied in, the first level's result is copied out
+    __shared__ float lds[kTailLdsFloats];
+    auto level = [&](int l) -> TailLevel {
+        size_t off = 0;
+        for (int q = 0; q < l; q++) off += 8 * t.lv[q].d.cells();
+        const size_t n = t.lv[l].d.cells();
+        float *p = lds + off;
+        return TailLevel{t.lv[l].d, Op32{p, {p + n, p + 2 * n, p + 3 * n}}, p + 4 * n, p + 5 * n,
+                         p + 6 * n, p + 7 * n};
+    };
+    for (int l = 0; l < t.nlev; l++) {
+        const TailLevel L = level(l);
+        const TailLevel &G = t.lv[l];
+        for (int q = threadIdx.x; q < (int)G.d.cells(); q += kTailThreads) {
+            const_cast<float *>(L.A.diag)[q] = G.A.diag[q];
+            const_cast<float *>(L.A.cpl[0])[q] = G.A.cpl[0][q];
+            const_cast<float *>(L.A.cpl[1])[q] = G.A.cpl[1][q];
+            const_cast<float *>(L.A.cpl[2])[q] = G.A.cpl[2][q];
+            if (l == 0) L.b[q] = G.b[q];
+        }
+    }
+    __syncthreads();
+    const int last = t.nlev - 1;
+    unsigned in_x1 = 0;  // bit l: level l's pre-smoothed iterate is in x1
+    // descend: pre-smoothing and restriction (as the separate launches)
+    for (int l = 0; l < last; l++) {
+        const TailLevel L = level(l);
+        float *c = L.x0, *o = L.x1;
+        const Dims dc = t.lv[l + 1].d;
+        if (t.deg == 1) {
+            for (int q = threadIdx.x; q < (int)L.d.cells(); q += kTailThreads)
+                c[q] = t.cr[0] * rcp_or_zero(L.A.diag[q]) * L.b[q];
+            __syncthreads();
+        } else {
+            if (t.deg > 2)
+                tail_smooth<SM_FIRST | SM_WRITE_D>(L, c, o, t.cr[0], t.cd[1], t.cr[1], nullptr, dc);
+            else
+                tail_smooth<SM_FIRST>(L, c, o, t.cr[0], t.cd[1], t.cr[1], nullptr, dc);
+            float *tmp = c;
+            c = o;
+            o = tmp;
+            for (int s = 2; s < t.deg; s++) {
+                if (s < t.deg - 1)
+                    tail_smooth<SM_READ_D | SM_WRITE_D>(L, c, o, t.cr[0], t.cd[s], t.cr[s], nullptr, dc);
+                else
+                    tail_smooth<SM_READ_D>(L, c, o, t.cr[0], t.cd[s], t.cr[s], nullptr, dc);
+                tmp = c;
+                c = o;
+                o = tmp;
+            }
+        }
+        tail_restrict(L, c, level(l + 1).b, dc);
+        if (c == L.x1) in_x1 |= 1u << l;
+    }
+    // coarsest: dense solve (k_coarse_solve's order)
+    const TailLevel C = level(last);
+    {
+        const int n = (int)C.d.cells();
+        for (int r = threadIdx.x; r < n; r += kTailThreads) {
+            float acc = 0.f;
+            for (int c = 0; c < n; c++) acc = fmaf(t.ainv[r * n + c], C.b[c], acc);
+            C.x0[r] = acc;
+        }
+        __syncthreads();
+    }
+    // ascend: prolongation fused into the first post-smoothing step
+    const float *e = C.x0;
+    for (int l = last - 1; l >= 0; l--) {
+        const TailLevel L = level(l);
+        const Dims dc = t.lv[l + 1].d;
+        float *c = (in_x1 >> l) & 1u ? L.x1 : L.x0;
+        float *o = c == L.x0 ? L.x1 : L.x0;
+        for (int s = 0; s < t.deg; s++) {
+            const bool wd = s < t.deg - 1;
+            if (s == 0) {
+                if (wd)
+                    tail_smooth<SM_PROLONG | SM_WRITE_D>(L, c, o, t.cr[0], 0.f, t.cr[0], e, dc);
+                else
+                    tail_smooth<SM_PROLONG>(L, c, o, t.cr[0], 0.f, t.cr[0], e, dc);
+            } else {
+                if (wd)
+                    tail_smooth<SM_READ_D | SM_WRITE_D>(L, c, o, t.cr[0], t.cd[s], t.cr[s], nullptr, dc);
+                else
+                    tail_smooth<SM_READ_D>(L, c, o, t.cr[0], t.cd[s], t.cr[s], nullptr, dc);
+            }
+            float *tmp = c;
+            c = o;
+            o = tmp;
+        }
+        e = c;
+    }
+    // the first level's result (in its x1, where the level-by-level cycle leaves it)
+    for (int q = threadIdx.x; q < (int)t.lv[0].d.cells(); q += kTailThreads) t.lv[0].x1[q] = e[q];
 }
 
 }  // namespace
@@ -711,6 +930,16 @@ void launch_coarse_solve(hipStream_t st, int n, const float *ainv, const float *
                          const double *gate) {
     if (n > kMaxCoarse) throw ArgError("coarsest level too large for the dense solve");
     hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(kBlock), 0, st, n, ainv, b, x, gate);
+    SFP_LAUNCH_CHECK();
+}
+
+void launch_vcycle_tail(hipStream_t st, const TailArgs &t, const double *gate) {
+    if (t.nlev < 2 || t.nlev > kTailMaxLevels) throw ArgError("vcycle tail: bad level count");
+    if (t.deg < 1 || t.deg > 8) throw ArgError("vcycle tail: bad degree");
+    if (t.lv[0].d.cells() > kTailMaxCells) throw ArgError("vcycle tail: level too large");
+    if ((int)t.lv[t.nlev - 1].d.cells() > kMaxCoarse) throw ArgError("vcycle tail: coarsest too large");
+    if (tail_lds_bytes(t) > kTailLdsBytes) throw ArgError("vcycle tail: levels exceed the LDS");
+    hipLaunchKernelGGL(k_vcycle_tail, dim3(1), dim3(kTailThreads), 0, st, t, gate);
     SFP_LAUNCH_CHECK();
 }
 

@@ -158,7 +158,31 @@ void System::build_hierarchy(Workspace &ws, const SolverConfig &cfg) {
     launch_coarse_invert(ws.stream, lv_.back().d, lv_.back().op64(), ainv_.p);
 }
 
+float *System::vcycle_tail(Workspace &ws, const Cheby &cb, int l) {
+    const int nlev = (int)lv_.size() - l;
+    if (!ws.vcycle_tail || nlev < 2 || nlev > kTailMaxLevels || lv_[l].d.cells() > kTailMaxCells)
+        return nullptr;
+    TailArgs t;
+    t.nlev = nlev;
+    for (int q = 0; q < nlev; q++) {
+        Level &L = lv_[l + q];
+        t.lv[q] = TailLevel{L.d, L.op32(), L.b.p, L.x0.p, L.x1.p, L.dv.p};
+    }
+    t.deg = cb.degree;
+    for (int q = 0; q < 8; q++) {
+        t.cd[q] = cb.cd[q];
+        t.cr[q] = cb.cr[q];
+    }
+    t.ainv = ainv_.p;
+    if (tail_lds_bytes(t) > kTailLdsBytes) return nullptr;
+    launch_vcycle_tail(ws.stream, t, ws.gate);
+    return tail_result(t);
+}
+
 float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) {
+    if (!r64) {
+        if (float *z = vcycle_tail(ws, cb, l)) return z;
+    }
     Level &L = lv_[l];
     const hipStream_t st = ws.stream;
     const bool fine = (l == 0);

@@ -65,6 +65,8 @@ struct Workspace {
     // &scalars->stop while a batched solve runs (iteration kernels return at
     // once when it is nonzero), null otherwise
     const double *gate = nullptr;
+    // run the V-cycle's small levels (<= kTailMaxCells cells) in one launch
+    bool vcycle_tail = true;
     void read_scalars();               // synchronous copy of *scalars to host_sc
     void write_scalars();              // host_sc to *scalars (stream-ordered)
 };
@@ -126,6 +128,9 @@ class System {
     float *post_smooth(Workspace &ws, const Cheby &cb, int l, float *cur, const float *e);
     SolveInfo fcg(Workspace &ws, const SolverConfig &cfg, int level, bool warm);
     float *vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64);
+    // levels l .. coarsest in one launch (vlevel with r64 null), or null
+    // where the tail does not apply
+    float *vcycle_tail(Workspace &ws, const Cheby &cb, int l);
     void network_correction(Workspace &ws, float *x);
     void coarse_phi(Workspace &ws);
     DevBuf<double> phic_;  // level set restricted to the first coarse level

@@ -1,9 +1,10 @@
 """GPU tests of the one-pass kernels of the V-cycle through the C ABI: the
-pair of post-smoothing steps and the last pre-smoothing step fused with the
-restriction. Each is bit-identical to the separate launches on pressure and
-magnetic levels (ragged and odd extents, inactive cells, one and several x
-planes per block, grids narrower than a tile) and within FP32 rounding of an
-FP64 numpy statement on the same FP32 operator."""
+pair of post-smoothing steps, the last pre-smoothing step fused with the
+restriction, and the small levels run in one launch. Each is bit-identical to
+the separate launches on pressure and magnetic levels (ragged and odd extents,
+inactive cells, one and several x planes per block, grids narrower than a
+tile); the first two are also within FP32 rounding of an FP64 numpy
+statement on the same FP32 operator."""
 import numpy as np
 import pytest
 
@@ -78,6 +79,11 @@ def test_pair_matches_two_steps(kind, n, level):
         vec = lambda s: (rng.uniform(-s, s, diag.shape) * act).astype(np.float32)  # noqa: E731
         b, x1, d1 = vec(1.0), vec(1.0), vec(0.1)
         x3p, x3s = ctx.smooth_pair(which, level, b, x1, d1, *COEF)
+        # repeated launches: the same bits each time (no race between the
+        # block's wavefronts over the shared planes)
+        for _ in range(4):
+            again, _ = ctx.smooth_pair(which, level, b, x1, d1, *COEF)
+            np.testing.assert_array_equal(again, x3p)
     np.testing.assert_array_equal(x3p, x3s)
     assert np.all(x3p[~act] == 0)
     f64 = lambda a: a.astype(np.float64)  # noqa: E731
@@ -107,6 +113,10 @@ def test_smooth_restrict_matches_separate_launches(kind, n, level):
         vec = lambda s: (rng.uniform(-s, s, diag.shape) * act).astype(np.float32)  # noqa: E731
         b, x1, d1 = vec(1.0), vec(1.0), vec(0.1)
         x2f, bcf, x2s, bcs = ctx.smooth_restrict(which, level, b, x1, d1, COEF[0], COEF[1])
+        for _ in range(4):
+            x2a, bca, _, _ = ctx.smooth_restrict(which, level, b, x1, d1, COEF[0], COEF[1])
+            np.testing.assert_array_equal(x2a, x2f)
+            np.testing.assert_array_equal(bca, bcf)
     np.testing.assert_array_equal(x2f, x2s)
     np.testing.assert_array_equal(bcf, bcs)
     f64 = lambda a: a.astype(np.float64)  # noqa: E731
@@ -122,6 +132,23 @@ def test_smooth_restrict_matches_separate_launches(kind, n, level):
     assert np.abs(res[::2, ::2, ::2] - bcf).max() > 1e-2 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("kind", ["pressure", "magnetic"])
+def test_repeated_solves_are_bitwise_identical(kind):
+    """Solves of one system on a grid that takes the one-pass kernels repeat
+    bit for bit (iteration count and solution)."""
+    print(f"[case] repeated {kind} solves at 160^3", flush=True)
+    sc = scenes.pressure_scene(160, 2) if kind == "pressure" else scenes.magnetic_scene(160, seed=2)
+    with sfp.Context.for_scene(sc) as ctx:
+        runs = []
+        for _ in range(3):
+            ctx.reset_warm_start()
+            runs.append(ctx.pressure_project(sc) if kind == "pressure" else ctx.magnetic_solve(sc))
+    for r in runs[1:]:
+        assert r[-1]["iterations"] == runs[0][-1]["iterations"]
+        np.testing.assert_array_equal(r[1] if kind == "pressure" else r[0],
+                                      runs[0][1] if kind == "pressure" else runs[0][0])
+
+
 def test_pair_path_solves_to_tolerance(oracle):
     """A grid where the V-cycle takes the one-pass kernels (nz >= 64, above
     128^3 cells) converges to the tolerance on the oracle's system."""
@@ -132,3 +159,37 @@ def test_pair_path_solves_to_tolerance(oracle):
     assert st["converged"]
     ps = oracle.pressure_system(sc)
     assert oracle.rel_residual(ps["A"], ps["rhs"], p.ravel()[ps["mat2grid"]]) <= 1e-8
+
+
+@pytest.mark.parametrize("kind,n", [("pressure", (14, 14, 14)), ("pressure", (40, 37, 45)),
+                                    ("magnetic", 33), ("pressure", (96, 80, 112))])
+@pytest.mark.parametrize("degree", [1, 2, 3, 4])
+def test_vcycle_tail_matches_level_by_level(kind, n, degree):
+    """The V-cycle with its small levels in one launch gives the same
+    preconditioned vector, bit for bit, as level-by-level launches, and the
+    whole solve the same solution and iteration count."""
+    print(f"[case] vcycle tail {kind} {n} degree {degree}", flush=True)
+    sc = scenes.random_pressure_scene(n, seed=5) if kind == "pressure" else \
+        scenes.magnetic_scene(n, seed=5)
+    which = 0 if kind == "pressure" else 1
+    with sfp.Context.for_scene(sc, cheby_degree=degree) as ctx:
+        solve = ctx.pressure_project if which == 0 else ctx.magnetic_solve
+        out = {}
+        for tail in (False, True):
+            ctx.set_vcycle_tail(tail)
+            ctx.reset_warm_start()
+            res = solve(sc)
+            rng = np.random.default_rng(3)
+            diag = ctx.export_level32(which, 0)["diag"]
+            r = rng.standard_normal(diag.shape) * (diag > 0)
+            out[tail] = (res, ctx.precondition(which, r))
+    (a, za), (b, zb) = out[False], out[True]
+    np.testing.assert_array_equal(za, zb)
+    assert a[-1]["iterations"] == b[-1]["iterations"]
+    for u, v in zip(a[:-1], b[:-1]):
+        if isinstance(u, list):
+            for p, q in zip(u, v):
+                np.testing.assert_array_equal(p, q)
+        else:
+            np.testing.assert_array_equal(u, v)
+
