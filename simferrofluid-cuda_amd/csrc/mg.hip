@@ -1,7 +1,8 @@
 // Geometric-multigrid kernels for gfx950: FP32 Chebyshev smoothing with the
-// coarse-grid correction fused into the first post-smoothing step,
-// residual + restriction, FP64 Galerkin coarsening over 2x2x2 aggregates and
-// the dense coarsest solve.
+// coarse-grid correction fused into the first post-smoothing step, one-pass
+// pairs of steps (and of the last step with the restriction), residual +
+// restriction, FP64 Galerkin coarsening over 2x2x2 aggregates, the dense
+// coarsest solve, and the small levels of a cycle in one single-block launch.
 //
 // Threads map to cells with z (the unit-stride axis of the reference layout)
 // across the 64 lanes of a wavefront and y across the 4 wavefronts of a
@@ -462,78 +463,116 @@ __global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 
     }
 }
 
-// Sum of the fine couplings along `axis` on the fine layer `layer` (the
-// interface between two aggregates), over the 2x2 children in the other axes.
-__device__ __forceinline__ double layer_coupling(Dims df, const double *cpl, int axis, int layer,
-                                                 int I, int J, int K) {
-    double sum = 0.0;
-    int lo[3] = {2 * I, 2 * J, 2 * K};
-    lo[axis] = layer;
-    int hi[3] = {2 * I + 1, 2 * J + 1, 2 * K + 1};
-    hi[axis] = layer;
-    for (int i = lo[0]; i <= hi[0] && i < df.nx; i++)
-        for (int j = lo[1]; j <= hi[1] && j < df.ny; j++)
-            for (int k = lo[2]; k <= hi[2] && k < df.nz; k++) sum += cpl[df.idx(i, j, k)];
-    return sum;
-}
+// Galerkin coarsening over the fine grid (the mapping of k_restrict): each
+// fine cell forms its contributions to its aggregate -- the active cell's
+// row sum diag - (its 6 couplings), its lower-face couplings where it lies on
+// the aggregate's lower layer of an axis, its upper-face couplings where it
+// lies on the upper layer -- and the 8 children are added pairwise (z through
+// the lane pair, y through LDS, x over the block's plane pairs). Same operator
+// as a per-aggregate loop over the children, up to the order of the sums.
+constexpr int kCoarsenSums = 8;  // row sum, 3 lower, 3 upper, active count
 
-__global__ __launch_bounds__(kBlock) void k_coarsen(CellGrid cg, Dims df, Op64 Af, const float *diag32f,
-                                                    Dims dc, double scale, double *diag64c,
-                                                    MutFaces cpl64c, float *diag32c,
-                                                    float *cpl32c0, float *cpl32c1,
-                                                    float *cpl32c2) {
-    int K, J, I0;
-    cell_tile(cg, K, J, I0);
-    if (K >= dc.nz || J >= dc.ny) return;
+__global__ __launch_bounds__(kBlock) void k_coarsen_fine(CellGrid cg, Dims df, Op64 Af,
+                                                         const float *__restrict__ diag32f, Dims dc,
+                                                         double scale, double *diag64c,
+                                                         MutFaces cpl64c, float *diag32c,
+                                                         float *cpl32c0, float *cpl32c1,
+                                                         float *cpl32c2) {
+    __shared__ double sh[kCoarsenSums][kBlock];
+    int k, j, i0;
+    cell_tile(cg, k, j, i0);
+    const int tz = blockDim.x, lane = threadIdx.x, row = threadIdx.y;
+    if (i0 >= df.nx) return;  // uniform over the block
+    const bool on = k < df.nz && j < df.ny;
+    const bool writer = on && (k & 1) == 0 && (j & 1) == 0;
     const Strides s(df);
+    const int ie = min(i0 + cg.planes, df.nx);
+    const int t = row * tz + lane;
     float *cpl32c[3] = {cpl32c0, cpl32c1, cpl32c2};
-    for (int I = I0, Ie = min(I0 + cg.planes, dc.nx); I < Ie; I++) {
-        bool active = false;
-        double extra = 0.0;
-        for (int i = 2 * I; i < 2 * I + 2 && i < df.nx; i++)
-            for (int j = 2 * J; j < 2 * J + 2 && j < df.ny; j++)
-                for (int k = 2 * K; k < 2 * K + 2 && k < df.nz; k++) {
-                    const size_t c = df.idx(i, j, k);
-                    if (!(diag32f[c] > 0.f)) continue;
-                    active = true;
-                    double off = Af.cpl[0][c] + Af.cpl[1][c] + Af.cpl[2][c];
-                    if (i + 1 < df.nx) off += Af.cpl[0][c + s.sx];
-                    if (j + 1 < df.ny) off += Af.cpl[1][c + s.sy];
-                    if (k + 1 < df.nz) off += Af.cpl[2][c + 1];
-                    extra += Af.diag[c] - off;
+    double acc[kCoarsenSums];
+#pragma unroll
+    for (int q = 0; q < kCoarsenSums; q++) acc[q] = 0.0;
+    for (int i = i0; i < ie; i++) {
+        double v[kCoarsenSums];
+#pragma unroll
+        for (int q = 0; q < kCoarsenSums; q++) v[q] = 0.0;
+        if (on) {
+            const size_t c = df.idx(i, j, k);
+            const bool up[3] = {i + 1 < df.nx, j + 1 < df.ny, k + 1 < df.nz};
+            const double cl[3] = {Af.cpl[0][c], Af.cpl[1][c], Af.cpl[2][c]};
+            const double cu[3] = {up[0] ? Af.cpl[0][c + s.sx] : 0.0, up[1] ? Af.cpl[1][c + s.sy] : 0.0,
+                                  up[2] ? Af.cpl[2][c + 1] : 0.0};
+            const int idx[3] = {i, j, k};
+            if (diag32f[c] > 0.f) {
+                double off = cl[0] + cl[1] + cl[2];
+                if (up[0]) off += cu[0];
+                if (up[1]) off += cu[1];
+                if (up[2]) off += cu[2];
+                v[0] = Af.diag[c] - off;
+                v[7] = 1.0;
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                if ((idx[a] & 1) == 0) v[1 + a] = cl[a];  // lower layer: the face below
+                else v[4 + a] = cu[a];                     // upper layer: the face above
+            }
+        }
+        if (tz > 1) {
+#pragma unroll
+            for (int q = 0; q < kCoarsenSums; q++) v[q] += __shfl_xor(v[q], 1, tz);  // z pair
+        }
+#pragma unroll
+        for (int q = 0; q < kCoarsenSums; q++) sh[q][t] = v[q];
+        __syncthreads();
+        if ((row & 1) == 0) {
+#pragma unroll
+            for (int q = 0; q < kCoarsenSums; q++) v[q] += sh[q][t + tz];  // y pair
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kCoarsenSums; q++) acc[q] += v[q];  // x pair
+        if ((i & 1) == 1 || i + 1 == df.nx) {
+            if (writer) {
+                const int I = i >> 1, J = j >> 1, K = k >> 1;
+                const int IC[3] = {I, J, K};
+                double lower[3], upper[3];
+                for (int a = 0; a < 3; a++) {
+                    lower[a] = IC[a] > 0 ? scale * acc[1 + a] : 0.0;
+                    upper[a] = IC[a] + 1 < dc.n(a) ? scale * acc[4 + a] : 0.0;
                 }
-        const int IC[3] = {I, J, K};
-        double lower[3], upper[3];
-        for (int a = 0; a < 3; a++) {
-            lower[a] = IC[a] > 0 ? scale * layer_coupling(df, Af.cpl[a], a, 2 * IC[a], I, J, K) : 0.0;
-            upper[a] = IC[a] + 1 < dc.n(a)
-                           ? scale * layer_coupling(df, Af.cpl[a], a, 2 * IC[a] + 2, I, J, K)
-                           : 0.0;
-        }
-        const size_t cc = dc.idx(I, J, K);
-        if (active) {
-            const double dg = scale * extra + lower[0] + upper[0] + lower[1] + upper[1] + lower[2] + upper[2];
-            diag64c[cc] = dg;
-            diag32c[cc] = dg > 0.0 ? (float)dg : 0.f;
-        } else {
-            diag64c[cc] = 1.0;
-            diag32c[cc] = 0.f;
-        }
-        for (int a = 0; a < 3; a++) {
-            const double v = active ? lower[a] : 0.0;
-            cpl64c.a[a][cc] = v;
-            cpl32c[a][cc] = (float)v;
+                const bool active = acc[7] > 0.0;
+                const size_t cc = dc.idx(I, J, K);
+                if (active) {
+                    const double dg = scale * acc[0] + lower[0] + upper[0] + lower[1] + upper[1] +
+                                      lower[2] + upper[2];
+                    diag64c[cc] = dg;
+                    diag32c[cc] = dg > 0.0 ? (float)dg : 0.f;
+                } else {
+                    diag64c[cc] = 1.0;
+                    diag32c[cc] = 0.f;
+                }
+                for (int a = 0; a < 3; a++) {
+                    const double cv = active ? lower[a] : 0.0;
+                    cpl64c.a[a][cc] = cv;
+                    cpl32c[a][cc] = (float)cv;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kCoarsenSums; q++) acc[q] = 0.0;
         }
     }
 }
 
-// In-place Gauss-Jordan inversion of the dense coarsest matrix (SPD up to
-// pure-Neumann blocks, whose zero pivots are dropped) in LDS, one block.
-// Dense inverse of the coarsest operator (n <= 64 cells) by Gauss-Jordan in
-// LDS, rows stored with a fixed stride of 64 so the element loops split an
-// index with a shift and a mask. Pivots below 1e-12 of the largest diagonal
-// (the pure-Neumann direction) are dropped: their row and column stay 0.
-__global__ __launch_bounds__(kBlock) void k_coarse_invert(Dims d, Op64 A, float *ainv) {
+// Dense inverse of the coarsest operator (n <= 64 cells; SPD up to
+// pure-Neumann blocks) by in-place Gauss-Jordan in LDS, rows stored with a
+// fixed stride of 64 so the element loops split an index with a shift and a
+// mask. Pivots below 1e-12 of the largest diagonal (the pure-Neumann
+// direction) are dropped: their row and column stay 0. One block of
+// kInvertThreads: every thread owns S * S / kInvertThreads matrix entries, so
+// a pivot step is a few FP64 operations per thread between two barriers.
+constexpr int kInvertThreads = 1024;
+
+__global__ __launch_bounds__(kInvertThreads) void k_coarse_invert(Dims d, Op64 A, float *ainv) {
     constexpr int S = kMaxCoarse;  // row stride
     static_assert(S == 64, "the element split below assumes a stride of 64");
     __shared__ double M[S * S];
@@ -542,9 +581,9 @@ __global__ __launch_bounds__(kBlock) void k_coarse_invert(Dims d, Op64 A, float 
     const int n = (int)d.cells();
     const int tid = threadIdx.x;
     const Strides s(d);
-    for (int e = tid; e < S * S; e += kBlock) M[e] = 0.0;
+    for (int e = tid; e < S * S; e += kInvertThreads) M[e] = 0.0;
     __syncthreads();
-    for (int c = tid; c < n; c += kBlock) {
+    for (int c = tid; c < n; c += kInvertThreads) {
         const int i = c / (d.ny * d.nz), j = (c / d.nz) % d.ny, k = c % d.nz;
         M[c * S + c] = A.diag[c];
         if (i > 0) M[c * S + (c - (int)s.sx)] = -A.cpl[0][c];
@@ -575,8 +614,8 @@ __global__ __launch_bounds__(kBlock) void k_coarse_invert(Dims d, Op64 A, float 
                 M[tid * S + kk] = 0.0;
             }
         } else {
-            const double pinv = 1.0 / p;
-            for (int e = tid; e < S * S; e += kBlock) {
+            const double pinv = 1.0 / p;  // the same value in every thread
+            for (int e = tid; e < S * S; e += kInvertThreads) {
                 const int r = e >> 6, c = e & (S - 1);
                 if (r >= n || c >= n) continue;
                 double v;
@@ -593,7 +632,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse_invert(Dims d, Op64 A, float 
         }
         __syncthreads();
     }
-    for (int e = tid; e < n * n; e += kBlock) ainv[e] = (float)M[(e / n) * S + (e % n)];
+    for (int e = tid; e < n * n; e += kInvertThreads) ainv[e] = (float)M[(e / n) * S + (e % n)];
 }
 
 __global__ __launch_bounds__(kBlock) void k_coarse_solve(int n, const float *ainv, const float *b,
@@ -913,16 +952,18 @@ void launch_restrict(hipStream_t st, Dims df, Op32 A, const float *b, const floa
 void launch_coarsen(hipStream_t st, Dims df, Op64 Af, const float *diag32f, Dims dc,
                     double scale, double *diag64c, double *const cpl64c[3], float *diag32c,
                     float *const cpl32c[3]) {
-    Launch L = cell_launch(dc);
+    // over the fine grid, with even plane pairs per block (as the restriction)
+    Launch L = cell_launch(df, 0, 8);
+    if (L.cg.planes & 1) L = cell_launch(df, L.cg.planes + 1);
     MutFaces m{{cpl64c[0], cpl64c[1], cpl64c[2]}};
-    hipLaunchKernelGGL(k_coarsen, L.grid, L.block, 0, st, L.cg, df, Af, diag32f, dc, scale, diag64c, m,
-                       diag32c, cpl32c[0], cpl32c[1], cpl32c[2]);
+    hipLaunchKernelGGL(k_coarsen_fine, L.grid, L.block, 0, st, L.cg, df, Af, diag32f, dc, scale, diag64c,
+                       m, diag32c, cpl32c[0], cpl32c[1], cpl32c[2]);
     SFP_LAUNCH_CHECK();
 }
 
 void launch_coarse_invert(hipStream_t st, Dims d, Op64 A, float *ainv) {
     if ((int)d.cells() > kMaxCoarse) throw ArgError("coarsest level too large for the dense solve");
-    hipLaunchKernelGGL(k_coarse_invert, dim3(1), dim3(kBlock), 0, st, d, A, ainv);
+    hipLaunchKernelGGL(k_coarse_invert, dim3(1), dim3(kInvertThreads), 0, st, d, A, ainv);
     SFP_LAUNCH_CHECK();
 }
 
