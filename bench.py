@@ -79,8 +79,9 @@ HOT_KERNELS = {
                "fine-level Chebyshev step with the direction in and out, FP32 7-point stencil"),
     "matvec64": ("k_fcg_matvec", 60.0,
                  "flexible-CG w = A z with the FP64 operator and its 3 inner products"),
-    "update64": ("k_fcg_update", 80.0,
-                 "flexible-CG update of p, q, x, r (FP64) with r.r and the FP32 copy of r"),
+    "update64": ("k_fcg_update", 88.0,
+                 "flexible-CG update of p, q, x, r (FP64) with r.r, the FP32 copy b of r and "
+                 "the next cycle's first iterate c0 D^-1 b"),
     "restrict": ("k_restrict", 24.5, "fine-level FP32 residual summed over 2x2x2 children"),
 }
 

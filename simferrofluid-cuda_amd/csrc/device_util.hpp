@@ -17,6 +17,12 @@ __device__ __forceinline__ unsigned flat_bid() {
 }
 __device__ __forceinline__ unsigned num_blocks() { return gridDim.x * gridDim.y * gridDim.z; }
 
+// 1 / dg on active cells (dg > 0), 0 on inactive ones. Every kernel that
+// scales by the inverse diagonal uses this one approximation.
+__device__ __forceinline__ float rcp_or_zero(float dg) {
+    return dg > 0.f ? __builtin_amdgcn_rcpf(dg) : 0.f;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

@@ -14,7 +14,9 @@ enum SmoothMode : int {
     SM_FIRST = 1,      // input iterate is the zero-start first step c0 D^{-1} b, fused in
     SM_READ_D = 2,     // previous direction read from dvec
     SM_WRITE_D = 4,    // store the new direction
-    SM_PROLONG = 8     // input iterate is x + P e (coarse correction fused in)
+    SM_PROLONG = 8,    // input iterate is x + P e (coarse correction fused in)
+    SM_XFIRST = 16     // input iterate is the zero-start first step c0 D^{-1} b, stored
+                       // in xin (by the FCG update); the previous direction equals it
 };
 
 // x = c0 * D^{-1} b from a zero start; at the fine level b is first rounded
@@ -104,10 +106,11 @@ void launch_fcg_matvec(hipStream_t s, Dims d, Op64 A, const float *active, const
                        double *partials, KrylovScalars *sc, const double *gate = nullptr);
 // p = z - beta p; q = w - beta q; x += alpha p; r -= alpha q; rr = r.r
 // (all-inactive wavefronts skipped when `active` is given); b32 (or null)
-// receives r rounded to FP32.
+// receives r rounded to FP32 and x0 (or null) the first Chebyshev iterate
+// c0 D^{-1} b32, with D the FP32 diagonal `active`.
 void launch_fcg_update(hipStream_t s, Dims d, const float *active, const float *z,
                        const double *w, double *p, double *q, double *x, double *r, float *b32,
-                       int first, double *partials, KrylovScalars *sc,
+                       float *x0, float c0, int first, double *partials, KrylovScalars *sc,
                        const double *gate = nullptr);
 // z = D^{-1} r on active cells (Jacobi preconditioner).
 void launch_jacobi(hipStream_t s, Dims d, Op32 A, const double *r, float *z,

@@ -189,13 +189,14 @@ __global__ __launch_bounds__(kBlock) void k_fcg_matvec(CellGrid cg, Dims d, Op64
 constexpr unsigned kUpdateCells = 2 * kBlock;
 
 __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__restrict__ active,
-                                                       const float *__restrict__ z,
+                                                       const float *z,  // may be x0: read before x0 is written
                                                        const double *__restrict__ w,
                                                        double *__restrict__ p,
                                                        double *__restrict__ q,
                                                        double *__restrict__ x,
                                                        double *__restrict__ r,
-                                                       float *__restrict__ b32, int first,
+                                                       float *__restrict__ b32,
+                                                       float *x0, float c0, int first,
                                                        double *__restrict__ partials,
                                                        const KrylovScalars *__restrict__ sc,
                                                        const double *gate) {
@@ -205,7 +206,12 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__
     const size_t c = (size_t)blockIdx.x * kUpdateCells + 2 * (size_t)flat_tid();
     const bool pair = c + 1 < n;  // else at most the grid's last cell
     bool act = c < n;
-    if (active) act = pair ? (active[c] > 0.f || active[c + 1] > 0.f) : (c < n && active[c] > 0.f);
+    float2v dg = {0.f, 0.f};  // FP32 diagonal (the activity flag)
+    if (active) {
+        if (pair) dg = *(const float2v *)(active + c);
+        else if (c < n) dg.x = active[c];
+        act = dg.x > 0.f || dg.y > 0.f;
+    }
     if (__any(act)) {
         if (pair) {
             const float2v zf = *(const float2v *)(z + c);
@@ -237,6 +243,10 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__
             if (b32) {
                 const float2v bf = {(float)rc.x, (float)rc.y};
                 *(float2v *)(b32 + c) = bf;
+                if (x0) {
+                    const float2v xf = {c0 * rcp_or_zero(dg.x) * bf.x, c0 * rcp_or_zero(dg.y) * bf.y};
+                    *(float2v *)(x0 + c) = xf;
+                }
             }
             v[0] += rc.x * rc.x;
             v[0] += rc.y * rc.y;
@@ -249,7 +259,10 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__
             x[c] += alpha * pc;
             const double rc = r[c] - alpha * qc;
             r[c] = rc;
-            if (b32) b32[c] = (float)rc;
+            if (b32) {
+                b32[c] = (float)rc;
+                if (x0) x0[c] = c0 * rcp_or_zero(dg.x) * b32[c];
+            }
             v[0] += rc * rc;
         }
     }
@@ -382,15 +395,17 @@ void launch_fcg_matvec(hipStream_t st, Dims d, Op64 A, const float *active, cons
 
 void launch_fcg_update(hipStream_t st, Dims d, const float *active, const float *z,
                        const double *w, double *p, double *q, double *x, double *r, float *b32,
-                       int first, double *partials, KrylovScalars *sc, const double *gate) {
+                       float *x0, float c0, int first, double *partials, KrylovScalars *sc,
+                       const double *gate) {
     // two cells per lane: 16-byte (8-byte for z) aligned vectors
     for (const void *a : {(const void *)w, (const void *)p, (const void *)q, (const void *)x, (const void *)r})
         if ((uintptr_t)a % 16) throw ArgError("launch_fcg_update: FP64 vector not 16-byte aligned");
-    if ((uintptr_t)z % 8 || (uintptr_t)b32 % 8)
-        throw ArgError("launch_fcg_update: z or b32 not 8-byte aligned");
+    if ((uintptr_t)z % 8 || (uintptr_t)b32 % 8 || (uintptr_t)x0 % 8)
+        throw ArgError("launch_fcg_update: z, b32 or x0 not 8-byte aligned");
+    if (x0 && (!b32 || !active)) throw ArgError("launch_fcg_update: x0 needs b32 and the diagonal");
     const unsigned nb = update_blocks(d);
     hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(kBlock), 0, st, d.cells(), active, z, w, p, q,
-                       x, r, b32, first, partials, sc, gate);
+                       x, r, b32, x0, c0, first, partials, sc, gate);
     SFP_LAUNCH_CHECK();
     finalize(st, FIN_UPDATE, nb, first, partials, sc, gate);
 }

@@ -39,9 +39,6 @@ namespace {
 // true when a batched solve has already converged (see KrylovScalars::stop)
 __device__ __forceinline__ bool gated(const double *gate) { return gate && *gate != 0.0; }
 
-__device__ __forceinline__ float rcp_or_zero(float dg) {
-    return dg > 0.f ? __builtin_amdgcn_rcpf(dg) : 0.f;
-}
 
 template <bool FROM64>
 __global__ __launch_bounds__(kBlock) void k_smooth_first(CellGrid cg, Dims d, Op32 A, const double *r64,
@@ -88,6 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
     if (gated(gate)) return;
     constexpr bool kFirst = (MODE & SM_FIRST) != 0;
     constexpr bool kProlong = (MODE & SM_PROLONG) != 0;
+    constexpr bool kXFirst = (MODE & SM_XFIRST) != 0;
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     const int tz = blockDim.x;
@@ -180,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
                 if (!(dg > 0.f)) x0 = xin_c;
             }
             const float r = bc - fmaf(dg, x0, -off);
-            if constexpr (kFirst) dprev = x0;
+            if constexpr (kFirst || kXFirst) dprev = x0;
             const float dn = fmaf(cd, dprev, cr * rcp_or_zero(dg) * r);
             xout[c] = x0 + dn;
             if constexpr ((MODE & SM_WRITE_D) != 0) dvec[c] = dn;
@@ -895,6 +893,8 @@ void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, 
     switch (mode) {
         SFP_SMOOTH_CASE(SM_FIRST)
         SFP_SMOOTH_CASE(SM_FIRST | SM_WRITE_D)
+        SFP_SMOOTH_CASE(SM_XFIRST)
+        SFP_SMOOTH_CASE(SM_XFIRST | SM_WRITE_D)
         SFP_SMOOTH_CASE(SM_READ_D)
         SFP_SMOOTH_CASE(SM_READ_D | SM_WRITE_D)
         SFP_SMOOTH_CASE(SM_PROLONG)

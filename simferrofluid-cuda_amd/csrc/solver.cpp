@@ -210,8 +210,10 @@ float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) 
     if (deg == 1) {
         launch_smooth_first(st, L.d, A, r64, L.b.p, cur, cb.cr[0], ws.gate);
     } else {
-        // steps 0 and 1 in one pass; then steps 2 .. deg-1
-        smooth(SM_FIRST | (deg > 2 ? SM_WRITE_D : 0), cb.cd[1], cb.cr[1], r64, nullptr);
+        // steps 0 and 1 in one pass (step 0 read from x0 when the FCG update
+        // stored it); then steps 2 .. deg-1
+        const int first = (!r64 && ws.x0_level == l) ? SM_XFIRST : SM_FIRST;
+        smooth(first | (deg > 2 ? SM_WRITE_D : 0), cb.cd[1], cb.cr[1], r64, nullptr);
         for (int s = 2; s < deg - (fuse_restrict ? 1 : 0); s++)
             smooth(SM_READ_D | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s], nullptr, nullptr);
     }
@@ -374,7 +376,8 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     // the level's b: the update stores it, so the cycle after an update reads
     // b instead of the FP64 residual (the Jacobi preconditioner reads r).
     float *b32 = cfg.precond != 2 ? lv_[level].b.p : nullptr;
-    bool b_ready = false;  // lv_[level].b holds r rounded
+    bool b_ready = false;  // lv_[level].b holds r rounded (and lv_[level].x0 its first iterate)
+    const float c0 = Cheby::make(cfg.cheby_degree, cfg.cheby_ratio).cr[0];
     info.precond_used = learned ? 1 : (cfg.precond == 1 ? 0 : cfg.precond);
     while (true) {
         if (rel <= cfg.tol) {
@@ -406,6 +409,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
         }
         for (int it = 0; it < batch; it++) {
             const double *r64 = b_ready ? nullptr : r_.p;  // null: the cycle reads b
+            ws.x0_level = b_ready ? level : -1;
             const float *z;
             if (learned)
                 z = precondition_learned(ws, cfg, r64);
@@ -417,11 +421,13 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
             launch_fcg_matvec(st, d, A, lv_[level].diag32.p, z, r_.p, q_.p, w_.p, first ? 1 : 0,
                               ws.partials.p, ws.scalars.p, ws.gate);
             if (profiling) ws.timers[K_MATVEC64].end(st);
-            // z, w, p, q, x, r in; p, q, x, r (and b) out
+            // z, w, p, q, x, r in; p, q, x, r out (and diag in, b and x0 out)
             if (profiling)
-                ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 5 + 8.0 * 4 + (b32 ? 4.0 : 0.0)) * cells);
+                ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 5 + 8.0 * 4 + (b32 ? 4.0 + 4.0 + 4.0 : 0.0)) * cells);
+            ws.x0_level = -1;
             launch_fcg_update(st, d, lv_[level].diag32.p, z, w_.p, p_.p, q_.p, x_.p, r_.p, b32,
-                              first ? 1 : 0, ws.partials.p, ws.scalars.p, ws.gate);
+                              b32 ? lv_[level].x0.p : nullptr, c0, first ? 1 : 0, ws.partials.p,
+                              ws.scalars.p, ws.gate);
             if (profiling) ws.timers[K_UPDATE64].end(st);
             first = false;
             b_ready = b32 != nullptr;

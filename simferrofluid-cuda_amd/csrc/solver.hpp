@@ -67,6 +67,9 @@ struct Workspace {
     const double *gate = nullptr;
     // run the V-cycle's small levels (<= kTailMaxCells cells) in one launch
     bool vcycle_tail = true;
+    // level whose x0 holds the first Chebyshev iterate of its b (stored by
+    // the FCG update for the next cycle), or -1
+    int x0_level = -1;
     void read_scalars();               // synchronous copy of *scalars to host_sc
     void write_scalars();              // host_sc to *scalars (stream-ordered)
 };
