@@ -132,8 +132,8 @@ def summarize_profiles(d, n=256):
              "Top kernels by total time (all levels; from trace_kernel_stats.csv):", "",
              "| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
     for r in stats[:14]:
-        name = r["Name"].replace("sfp::(anonymous namespace)::", "").split("(")[0]
-        lines.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+        kname = r["Name"].replace("sfp::(anonymous namespace)::", "").split("(")[0]
+        lines.append(f"| `{kname}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
                      f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
     open(os.path.join(out_dir, f"{name}_{n}cube_trace.md"), "w").write("\n".join(lines) + "\n")
 
@@ -199,8 +199,8 @@ def summarize_profiles(d, n=256):
               f"{sum(float(r['TotalDurationNs']) for r in unet) / 1e6:.3f} ms of {total / 1e6:.3f} ms.", "",
               "| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
         for r in st[:24]:
-            name = r["Name"].replace("sfp::(anonymous namespace)::", "").split("(")[0]
-            lm.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+            kname = r["Name"].replace("sfp::(anonymous namespace)::", "").split("(")[0]
+            lm.append(f"| `{kname}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
                       f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
         open(os.path.join(out_dir, f"learned_{n}cube_trace.md"), "w").write("\n".join(lm) + "\n")
     print(json.dumps(res))
