@@ -156,6 +156,18 @@ void System::build_hierarchy(Workspace &ws, const SolverConfig &cfg) {
                        C.diag32.p, c32);
     }
     launch_coarse_invert(ws.stream, lv_.back().d, lv_.back().op64(), ainv_.p);
+    // Every kernel skips groups of inactive cells (rows, tiles, pairs of
+    // rows) and writes 0 at the inactive cells of the groups it processes
+    // only if their inputs are 0 there; the groups differ between kernels.
+    // A new operator may have a new active set, so the vectors start each
+    // solve at 0 (values left on cells that were active before would
+    // otherwise leak into z, w, p, q and the inner products).
+    for (Level &L : lv_) {
+        const size_t n = L.d.cells();
+        for (auto *v : {&L.x0, &L.x1, &L.dv, &L.b})
+            SFP_HIP(hipMemsetAsync(v->p, 0, n * sizeof(float), ws.stream));
+    }
+    for (auto *v : {&p_, &q_, &w_}) SFP_HIP(hipMemsetAsync(v->p, 0, d_.cells() * sizeof(double), ws.stream));
 }
 
 float *System::vcycle_tail(Workspace &ws, const Cheby &cb, int l) {
