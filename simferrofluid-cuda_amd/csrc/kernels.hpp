@@ -44,6 +44,9 @@ void launch_smooth_pair(hipStream_t s, Dims d, Op32 A, const float *b, const flo
 void launch_smooth_restrict(hipStream_t s, Dims d, Op32 A, const float *b, const float *x1,
                             float *x2, const float *d1, float cd, float cr, float *bc,
                             const double *gate = nullptr);
+// x += P e on active cells (diag32 > 0), in place.
+void launch_prolong_add(hipStream_t s, Dims d, const float *diag32, const float *ecoarse, float *x,
+                        const double *gate = nullptr);
 // bc = R (b - A x): sum of the residuals of the (up to 8) children.
 void launch_restrict(hipStream_t s, Dims df, Op32 A, const float *b, const float *x, float *bc,
                      const double *gate = nullptr);

@@ -256,10 +256,15 @@ float *System::post_smooth(Workspace &ws, const Cheby &cb, int l, float *cur, co
     // pay for the pass's tile overlap (the direction they leave is not stored:
     // nothing reads it before the next cycle rewrites it)
     const int fused = (deg >= 3 && pair_pays(L.d)) ? 2 : 0;
+    // on large grids the correction is added by its own pointwise pass and
+    // step 0 is a plain step (cheaper than gathering the coarse values at
+    // every neighbour); equal values either way
+    const bool add_first = pair_pays(L.d);
+    if (add_first) launch_prolong_add(st, L.d, L.diag32.p, e, cur, ws.gate);
     for (int s = 0; s < deg - fused; s++) {
         // step 0 adds the prolonged correction; the fine-level steps that read
         // and write the direction (36 algorithmic bytes per cell) are timed
-        const int mode = (s == 0 ? SM_PROLONG : SM_READ_D) | (s < deg - 1 ? SM_WRITE_D : 0);
+        const int mode = (s == 0 ? (add_first ? 0 : SM_PROLONG) : SM_READ_D) | (s < deg - 1 ? SM_WRITE_D : 0);
         const bool t = track && mode == (SM_READ_D | SM_WRITE_D);
         if (t) ws.timers[K_SMOOTH].begin(st, 36.0 * (double)L.d.cells());
         launch_smooth(st, mode, L.d, A, nullptr, L.b.p, cur, oth, L.dv.p, cb.cr[0], s == 0 ? 0.f : cb.cd[s],
