@@ -149,6 +149,31 @@ def test_magnetic_solve_parity(oracle, name):
         assert np.array_equal(H[a], H_ref[a])
 
 
+@pytest.mark.parametrize("n", [(96, 40, 6), (7, 130, 3), (9, 11, 200)])
+def test_anisotropic_grids_solve_to_tolerance(oracle, n):
+    """Flat, thin and tall grids: hierarchies whose axes stop halving at
+    different levels (single-cell axes, levels of every size class) still
+    converge to the tolerance on the oracle's systems."""
+    print(f"[case] anisotropic {n}", flush=True)
+    geo = scenes.Geometry(n, 1.0 / max(n))
+    X, Y, Z = geo.cell_centers()
+    phi = np.ascontiguousarray(np.broadcast_to(
+        Y - 0.47 * n[1] / max(n) - 0.03 * np.cos(7 * X + 3 * Z), geo.n)).copy()
+    msc = dict(geo=geo, phi=phi, hext=[np.full(geo.face_shape(a), (0.2, 1.0, -0.1)[a]) for a in range(3)],
+               chi=2.0, band=1.5 * geo.spacing)
+    ref = oracle.magnetic_system(msc)
+    with sfp.Context.for_scene(msc, tol=1e-9) as ctx:
+        pot, _, st = ctx.magnetic_solve(msc)
+    assert st["converged"] == 1
+    assert oracle.rel_residual(ref["A"], ref["rhs"], pot.ravel()) <= 1.01e-9
+    psc = scenes.random_pressure_scene(n, seed=3)
+    pref = oracle.pressure_system(psc)
+    with sfp.Context.for_scene(psc, tol=1e-9) as ctx:
+        _, p, st = ctx.pressure_project(psc)
+    assert st["converged"] == 1
+    assert oracle.rel_residual(pref["A"], pref["rhs"], p.ravel()[pref["mat2grid"]]) <= 1.01e-9
+
+
 def test_magnetic_warm_start_and_vertex_pressures(oracle):
     sc = _mag("wavy_32_chi2")
     print("[case] magnetic warm start + vertex pressures", flush=True)
