@@ -500,7 +500,7 @@ def learned_leg(args, systems, total, run_step, refill_ring, geo, mg_by_sys, log
     params, meta = learn.load_weights()
     lctx = sfp.Context(geo.n, geo.spacing, geo.origin, tol=args.tol, max_iters=200)
     lctx.set_network(params)
-    lctx.set_params(precond=sfp.PRECOND_LEARNED)
+    lctx.set_params(precond=sfp.PRECOND_LEARNED, **learn.smoother(meta))
     refill_ring()
     for i in range(args.warmup):
         name, st = run_step(i, lctx)
@@ -530,7 +530,8 @@ def learned_leg(args, systems, total, run_step, refill_ring, geo, mg_by_sys, log
     return {"s_per_solve": round(elapsed / args.steps, 6), "per_system": by_sys,
             "converged_all": all(st["converged"] for st in records),
             "weights": {"sha256": meta["sha256"], "params": meta["count"],
-                        "trained_at": meta.get("n"), "steps": meta.get("steps")}}
+                        "trained_at": meta.get("n"), "steps": meta.get("steps"),
+                        "smoother": learn.smoother(meta)}}
 
 
 def cpu_leg(psc, msc, pressure_out, potential_out, ctx, phi_p, frac, us, vel0, phi_m, hext, geo,

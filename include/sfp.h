@@ -54,6 +54,8 @@ typedef struct sfp_params {
     double stall_ratio;  /* learned preconditioner: fall back to MG when
                             ||r_k|| / ||r_{k-1}|| exceeds this */
     int verbose;         /* 1: print "{iters} iters" per solve like the reference */
+    int coarse_cheby_degree;   /* Chebyshev degree on the levels below the finest (1..8) */
+    double coarse_cheby_ratio; /* their smoothing interval ratio */
 } sfp_params;
 
 typedef struct sfp_stats {

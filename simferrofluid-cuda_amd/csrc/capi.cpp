@@ -27,6 +27,8 @@ sfp_params to_params(const SolverConfig &c) {
     p.coarse_scale = c.coarse_scale;
     p.stall_ratio = c.stall_ratio;
     p.verbose = c.verbose;
+    p.coarse_cheby_degree = c.coarse_cheby_degree;
+    p.coarse_cheby_ratio = c.coarse_cheby_ratio;
     return p;
 }
 
@@ -455,7 +457,8 @@ sfp_status sfp_set_params(sfp_context ctx, const sfp_params *p) {
     if (!ctx || !p) return SFP_ERR_ARGUMENT;
     if (!(p->tol > 0) || p->max_iters < 0 || p->precond < 0 || p->precond > 3 ||
         p->mg_coarsest < 2 || p->mg_coarsest > 4 || p->cheby_degree < 1 || p->cheby_degree > 8 ||
-        !(p->cheby_ratio > 1) || !(p->coarse_scale > 0) || !(p->stall_ratio > 0)) {
+        !(p->cheby_ratio > 1) || !(p->coarse_scale > 0) || !(p->stall_ratio > 0) ||
+        p->coarse_cheby_degree < 1 || p->coarse_cheby_degree > 8 || !(p->coarse_cheby_ratio > 1)) {
         ctx->last_error = "parameter out of range";
         return SFP_ERR_ARGUMENT;
     }
@@ -473,6 +476,8 @@ sfp_status sfp_set_params(sfp_context ctx, const sfp_params *p) {
     c.coarse_scale = p->coarse_scale;
     c.stall_ratio = p->stall_ratio;
     c.verbose = p->verbose;
+    c.coarse_cheby_degree = p->coarse_cheby_degree;
+    c.coarse_cheby_ratio = p->coarse_cheby_ratio;
     return SFP_OK;
 }
 

@@ -74,8 +74,9 @@ struct TailLevel {
 struct TailArgs {
     int nlev = 0;  // levels in the tail, the last the coarsest
     TailLevel lv[kTailMaxLevels];
-    int deg = 3;
-    float cd[8] = {0}, cr[8] = {0};
+    // Chebyshev degree and coefficients: [0] for the first level, [1] below it
+    int deg[2] = {3, 3};
+    float cd[2][8] = {}, cr[2][8] = {};
     const float *ainv = nullptr;
 };
 // The tail keeps 8 arrays of every level it covers in LDS (gfx950: up to

@@ -816,28 +816,32 @@ __global__ __launch_bounds__(kTailThreads) void k_vcycle_tail(TailArgs t, const 
     __syncthreads();
     const int last = t.nlev - 1;
     unsigned in_x1 = 0;  // bit l: level l's pre-smoothed iterate is in x1
-    // descend: pre-smoothing and restriction (as the separate launches)
+    // descend: pre-smoothing and restriction (as the separate launches);
+    // coefficient set 0 on the tail's first level, set 1 below it
     for (int l = 0; l < last; l++) {
         const TailLevel L = level(l);
+        const int k = l == 0 ? 0 : 1;
+        const int deg = t.deg[k];
+        const float *cd = t.cd[k], *cr = t.cr[k];
         float *c = L.x0, *o = L.x1;
         const Dims dc = t.lv[l + 1].d;
-        if (t.deg == 1) {
+        if (deg == 1) {
             for (int q = threadIdx.x; q < (int)L.d.cells(); q += kTailThreads)
-                c[q] = t.cr[0] * rcp_or_zero(L.A.diag[q]) * L.b[q];
+                c[q] = cr[0] * rcp_or_zero(L.A.diag[q]) * L.b[q];
             __syncthreads();
         } else {
-            if (t.deg > 2)
-                tail_smooth<SM_FIRST | SM_WRITE_D>(L, c, o, t.cr[0], t.cd[1], t.cr[1], nullptr, dc);
+            if (deg > 2)
+                tail_smooth<SM_FIRST | SM_WRITE_D>(L, c, o, cr[0], cd[1], cr[1], nullptr, dc);
             else
-                tail_smooth<SM_FIRST>(L, c, o, t.cr[0], t.cd[1], t.cr[1], nullptr, dc);
+                tail_smooth<SM_FIRST>(L, c, o, cr[0], cd[1], cr[1], nullptr, dc);
             float *tmp = c;
             c = o;
             o = tmp;
-            for (int s = 2; s < t.deg; s++) {
-                if (s < t.deg - 1)
-                    tail_smooth<SM_READ_D | SM_WRITE_D>(L, c, o, t.cr[0], t.cd[s], t.cr[s], nullptr, dc);
+            for (int s = 2; s < deg; s++) {
+                if (s < deg - 1)
+                    tail_smooth<SM_READ_D | SM_WRITE_D>(L, c, o, cr[0], cd[s], cr[s], nullptr, dc);
                 else
-                    tail_smooth<SM_READ_D>(L, c, o, t.cr[0], t.cd[s], t.cr[s], nullptr, dc);
+                    tail_smooth<SM_READ_D>(L, c, o, cr[0], cd[s], cr[s], nullptr, dc);
                 tmp = c;
                 c = o;
                 o = tmp;
@@ -861,21 +865,24 @@ __global__ __launch_bounds__(kTailThreads) void k_vcycle_tail(TailArgs t, const 
     const float *e = C.x0;
     for (int l = last - 1; l >= 0; l--) {
         const TailLevel L = level(l);
+        const int k = l == 0 ? 0 : 1;
+        const int deg = t.deg[k];
+        const float *cd = t.cd[k], *cr = t.cr[k];
         const Dims dc = t.lv[l + 1].d;
         float *c = (in_x1 >> l) & 1u ? L.x1 : L.x0;
         float *o = c == L.x0 ? L.x1 : L.x0;
-        for (int s = 0; s < t.deg; s++) {
-            const bool wd = s < t.deg - 1;
+        for (int s = 0; s < deg; s++) {
+            const bool wd = s < deg - 1;
             if (s == 0) {
                 if (wd)
-                    tail_smooth<SM_PROLONG | SM_WRITE_D>(L, c, o, t.cr[0], 0.f, t.cr[0], e, dc);
+                    tail_smooth<SM_PROLONG | SM_WRITE_D>(L, c, o, cr[0], 0.f, cr[0], e, dc);
                 else
-                    tail_smooth<SM_PROLONG>(L, c, o, t.cr[0], 0.f, t.cr[0], e, dc);
+                    tail_smooth<SM_PROLONG>(L, c, o, cr[0], 0.f, cr[0], e, dc);
             } else {
                 if (wd)
-                    tail_smooth<SM_READ_D | SM_WRITE_D>(L, c, o, t.cr[0], t.cd[s], t.cr[s], nullptr, dc);
+                    tail_smooth<SM_READ_D | SM_WRITE_D>(L, c, o, cr[0], cd[s], cr[s], nullptr, dc);
                 else
-                    tail_smooth<SM_READ_D>(L, c, o, t.cr[0], t.cd[s], t.cr[s], nullptr, dc);
+                    tail_smooth<SM_READ_D>(L, c, o, cr[0], cd[s], cr[s], nullptr, dc);
             }
             float *tmp = c;
             c = o;
@@ -1004,7 +1011,8 @@ void launch_coarse_solve(hipStream_t st, int n, const float *ainv, const float *
 
 void launch_vcycle_tail(hipStream_t st, const TailArgs &t, const double *gate) {
     if (t.nlev < 2 || t.nlev > kTailMaxLevels) throw ArgError("vcycle tail: bad level count");
-    if (t.deg < 1 || t.deg > 8) throw ArgError("vcycle tail: bad degree");
+    for (int k = 0; k < 2; k++)
+        if (t.deg[k] < 1 || t.deg[k] > 8) throw ArgError("vcycle tail: bad degree");
     if (t.lv[0].d.cells() > kTailMaxCells) throw ArgError("vcycle tail: level too large");
     if ((int)t.lv[t.nlev - 1].d.cells() > kMaxCoarse) throw ArgError("vcycle tail: coarsest too large");
     if (tail_lds_bytes(t) > kTailLdsBytes) throw ArgError("vcycle tail: levels exceed the LDS");

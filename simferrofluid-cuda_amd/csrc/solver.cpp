@@ -170,7 +170,7 @@ void System::build_hierarchy(Workspace &ws, const SolverConfig &cfg) {
     for (auto *v : {&p_, &q_, &w_}) SFP_HIP(hipMemsetAsync(v->p, 0, d_.cells() * sizeof(double), ws.stream));
 }
 
-float *System::vcycle_tail(Workspace &ws, const Cheby &cb, int l) {
+float *System::vcycle_tail(Workspace &ws, const Smoothing &sm, int l) {
     const int nlev = (int)lv_.size() - l;
     if (!ws.vcycle_tail || nlev < 2 || nlev > kTailMaxLevels || lv_[l].d.cells() > kTailMaxCells)
         return nullptr;
@@ -180,10 +180,13 @@ float *System::vcycle_tail(Workspace &ws, const Cheby &cb, int l) {
         Level &L = lv_[l + q];
         t.lv[q] = TailLevel{L.d, L.op32(), L.b.p, L.x0.p, L.x1.p, L.dv.p};
     }
-    t.deg = cb.degree;
-    for (int q = 0; q < 8; q++) {
-        t.cd[q] = cb.cd[q];
-        t.cr[q] = cb.cr[q];
+    for (int set = 0; set < 2; set++) {
+        const Cheby &cb = set == 0 ? sm.at(l) : sm.coarse;  // the first level's, the others'
+        t.deg[set] = cb.degree;
+        for (int q = 0; q < 8; q++) {
+            t.cd[set][q] = cb.cd[q];
+            t.cr[set][q] = cb.cr[q];
+        }
     }
     t.ainv = ainv_.p;
     if (tail_lds_bytes(t) > kTailLdsBytes) return nullptr;
@@ -191,10 +194,11 @@ float *System::vcycle_tail(Workspace &ws, const Cheby &cb, int l) {
     return tail_result(t);
 }
 
-float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) {
+float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r64) {
     if (!r64) {
-        if (float *z = vcycle_tail(ws, cb, l)) return z;
+        if (float *z = vcycle_tail(ws, sm, l)) return z;
     }
+    const Cheby &cb = sm.at(l);
     Level &L = lv_[l];
     const hipStream_t st = ws.stream;
     const bool fine = (l == 0);
@@ -241,11 +245,12 @@ float *System::vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64) 
         launch_restrict(st, L.d, A, L.b.p, cur, lv_[l + 1].b.p, ws.gate);
         if (fine && ws.profiling) ws.timers[K_RESTRICT].end(st);
     }
-    const float *e = vlevel(ws, cb, l + 1, nullptr);
-    return post_smooth(ws, cb, l, cur, e);
+    const float *e = vlevel(ws, sm, l + 1, nullptr);
+    return post_smooth(ws, sm, l, cur, e);
 }
 
-float *System::post_smooth(Workspace &ws, const Cheby &cb, int l, float *cur, const float *e) {
+float *System::post_smooth(Workspace &ws, const Smoothing &sm, int l, float *cur, const float *e) {
+    const Cheby &cb = sm.at(l);
     Level &L = lv_[l];
     const hipStream_t st = ws.stream;
     const Op32 A = L.op32();
@@ -289,14 +294,14 @@ float *System::precondition(Workspace &ws, const SolverConfig &cfg, const double
         return lv_[0].x0.p;
     }
     if (cfg.precond == 3 && lv_.size() > 1) return learned_cycle(ws, cfg, r64, kCycleSmooth);
-    return vlevel(ws, Cheby::make(cfg.cheby_degree, cfg.cheby_ratio), 0, r64);
+    return vlevel(ws, Smoothing::make(cfg), 0, r64);
 }
 
 float *System::learned_cycle(Workspace &ws, const SolverConfig &cfg, const double *r64, Cycle c) {
-    const Cheby cb = Cheby::make(cfg.cheby_degree, cfg.cheby_ratio);
+    const Smoothing sm = Smoothing::make(cfg);
     Level &L = lv_[0], &C = lv_[1];
     const hipStream_t st = ws.stream;
-    float *z = vlevel(ws, cb, 0, r64);
+    float *z = vlevel(ws, sm, 0, r64);
     // residual the V-cycle leaves, restricted to the first coarse level
     launch_restrict(st, L.d, L.op32(), L.b.p, z, C.b.p);
     if (c == kCycleRestrictOnly) return C.b.p;
@@ -305,7 +310,7 @@ float *System::learned_cycle(Workspace &ws, const SolverConfig &cfg, const doubl
     } else {  // the same cycle without the network: a zero correction
         SFP_HIP(hipMemsetAsync(C.x0.p, 0, C.d.cells() * sizeof(float), st));
     }
-    return post_smooth(ws, cb, 0, z, C.x0.p);
+    return post_smooth(ws, sm, 0, z, C.x0.p);
 }
 
 bool System::learned_ok(const Workspace &ws) const {
@@ -394,7 +399,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     // b instead of the FP64 residual (the Jacobi preconditioner reads r).
     float *b32 = cfg.precond != 2 ? lv_[level].b.p : nullptr;
     bool b_ready = false;  // lv_[level].b holds r rounded (and lv_[level].x0 its first iterate)
-    const float c0 = Cheby::make(cfg.cheby_degree, cfg.cheby_ratio).cr[0];
+    const float c0 = Smoothing::make(cfg).at(level).cr[0];
     info.precond_used = learned ? 1 : (cfg.precond == 1 ? 0 : cfg.precond);
     while (true) {
         if (rel <= cfg.tol) {
@@ -433,7 +438,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
             else if (fine)
                 z = precondition(ws, cfg, r64);
             else
-                z = vlevel(ws, Cheby::make(cfg.cheby_degree, cfg.cheby_ratio), level, r64);
+                z = vlevel(ws, Smoothing::make(cfg), level, r64);
             if (profiling) ws.timers[K_MATVEC64].begin(st, (4.0 + 32.0 + 8.0 + 8.0 + 8.0) * cells);
             launch_fcg_matvec(st, d, A, lv_[level].diag32.p, z, r_.p, q_.p, w_.p, first ? 1 : 0,
                               ws.partials.p, ws.scalars.p, ws.gate);

@@ -21,6 +21,8 @@ struct SolverConfig {
     int coarsest = 4;
     int cheby_degree = 3;
     double cheby_ratio = 4.0;
+    int coarse_cheby_degree = 5;  // the smoother below the fine level
+    double coarse_cheby_ratio = 6.0;
     double coarse_scale = 0.5;
     double stall_ratio = 0.5;
     int verbose = 0;
@@ -31,6 +33,15 @@ struct Cheby {
     int degree = 3;
     float cd[8] = {0}, cr[8] = {0};
     static Cheby make(int degree, double ratio);
+};
+// The smoother of each level: `fine` on level 0, `coarse` below it.
+struct Smoothing {
+    Cheby fine, coarse;
+    const Cheby &at(int l) const { return l == 0 ? fine : coarse; }
+    static Smoothing make(const SolverConfig &cfg) {
+        return Smoothing{Cheby::make(cfg.cheby_degree, cfg.cheby_ratio),
+                         Cheby::make(cfg.coarse_cheby_degree, cfg.coarse_cheby_ratio)};
+    }
 };
 
 // Timing of one tracked kernel: pairs of events recorded around each launch.
@@ -128,12 +139,12 @@ class System {
   private:
     enum Cycle { kCycleLearned, kCycleSmooth, kCycleRestrictOnly };
     float *learned_cycle(Workspace &ws, const SolverConfig &cfg, const double *r64, Cycle c);
-    float *post_smooth(Workspace &ws, const Cheby &cb, int l, float *cur, const float *e);
+    float *post_smooth(Workspace &ws, const Smoothing &sm, int l, float *cur, const float *e);
     SolveInfo fcg(Workspace &ws, const SolverConfig &cfg, int level, bool warm);
-    float *vlevel(Workspace &ws, const Cheby &cb, int l, const double *r64);
+    float *vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r64);
     // levels l .. coarsest in one launch (vlevel with r64 null), or null
     // where the tail does not apply
-    float *vcycle_tail(Workspace &ws, const Cheby &cb, int l);
+    float *vcycle_tail(Workspace &ws, const Smoothing &sm, int l);
     void network_correction(Workspace &ws, float *x);
     void coarse_phi(Workspace &ws);
     DevBuf<double> phic_;  // level set restricted to the first coarse level

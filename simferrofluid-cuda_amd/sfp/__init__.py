@@ -53,7 +53,8 @@ EXPORTS = {
 class Params(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("precond", C.c_int),
                 ("mg_coarsest", C.c_int), ("cheby_degree", C.c_int), ("cheby_ratio", C.c_double),
-                ("coarse_scale", C.c_double), ("stall_ratio", C.c_double), ("verbose", C.c_int)]
+                ("coarse_scale", C.c_double), ("stall_ratio", C.c_double), ("verbose", C.c_int),
+                ("coarse_cheby_degree", C.c_int), ("coarse_cheby_ratio", C.c_double)]
 
 
 class Stats(C.Structure):

@@ -43,6 +43,14 @@ from . import scenes
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "weights",
                        "unet_precond.f32")
+# The V-cycle smoother settings a network is trained under: its samples are the
+# residuals that cycle leaves, so the learned preconditioner runs with them.
+SMOOTHER_KEYS = ("cheby_degree", "cheby_ratio", "coarse_cheby_degree", "coarse_cheby_ratio")
+
+
+def smoother(meta):
+    """The smoother settings recorded with trained weights, as set_params keywords."""
+    return {k: meta["smoother"][k] for k in SMOOTHER_KEYS}
 
 
 # ------------------------------------------------------------------ scenes
@@ -140,11 +148,11 @@ def harvest_system(ctx: Context, which: int, phi, rng, cg_keep=(2, 6, 16, 40), n
 
 def harvest(n=128, pressure_seeds=(100, 101, 102, 103, 104, 105, 106, 107),
             magnetic=((108, 0.5), (109, 1.0), (110, 2.0), (111, 4.0), (112, 8.0), (113, 2.0)),
-            seed=0, log=print):
+            seed=0, log=print, smoothing=None):
     samples = []
     rng = np.random.default_rng(seed)
     with Context(n, 1.0 / n, scenes.Geometry(n, 1.0 / n).origin, tol=1e-7, max_iters=200,
-                 precond=PRECOND_MG) as ctx:
+                 precond=PRECOND_MG, **(smoothing or {})) as ctx:
         for s in pressure_seeds:
             sc = drops_scene(n, s)
             ctx.pressure_project(sc, want_pressure=False)
@@ -236,12 +244,21 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--cheby", default=None, metavar="DEG:RATIO:COARSE_DEG:COARSE_RATIO",
+                    help="V-cycle smoother to train under (default: the library's)")
     a = ap.parse_args(argv)
     import torch  # its HIP runtime must initialise before libsfp's
     torch.cuda.init()
     t0 = time.time()
     log = lambda s: print(s, flush=True)  # noqa: E731
-    samples = harvest(a.n, seed=a.seed, log=log)
+    from . import default_params
+    dp = default_params()
+    sm = {k: getattr(dp, k) for k in SMOOTHER_KEYS}
+    if a.cheby:
+        v = a.cheby.split(":")
+        sm = dict(cheby_degree=int(v[0]), cheby_ratio=float(v[1]), coarse_cheby_degree=int(v[2]),
+                  coarse_cheby_ratio=float(v[3]))
+    samples = harvest(a.n, seed=a.seed, log=log, smoothing=sm)
     th = time.time() - t0
     log(f"{len(samples)} samples in {th:.1f}s")
     params, losses = train(samples, a.steps, a.lr, a.seed, a.batch, log=log)
@@ -249,7 +266,7 @@ def main(argv=None):
                                      samples=len(samples), harvest_s=round(th, 1),
                                      train_s=round(time.time() - t0 - th, 1),
                                      loss_first100=float(np.mean(losses[:100])),
-                                     loss_last100=float(np.mean(losses[-100:]))))
+                                     loss_last100=float(np.mean(losses[-100:])), smoother=sm))
     log(f"saved {a.out}")
     return 0
 

@@ -241,6 +241,11 @@ def test_call_order_errors():
         with pytest.raises(sfp.SfpError) as e:
             ctx.set_params(cheby_degree=0)
         assert e.value.status == sfp.ERR_ARGUMENT
+        for bad in (dict(coarse_cheby_degree=0), dict(coarse_cheby_degree=9),
+                    dict(coarse_cheby_ratio=1.0)):
+            with pytest.raises(sfp.SfpError) as e:
+                ctx.set_params(**bad)
+            assert e.value.status == sfp.ERR_ARGUMENT
         with pytest.raises(sfp.SfpError) as e:
             ctx.set_params(precond=sfp.PRECOND_LEARNED)
         assert e.value.status == sfp.ERR_UNSUPPORTED

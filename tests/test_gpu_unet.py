@@ -283,27 +283,30 @@ def _trained():
     from sfp import learn
     if not os.path.exists(learn.WEIGHTS):
         pytest.skip("no trained weights in the package")
-    return learn.load_weights()[0]
+    params, meta = learn.load_weights()
+    return params, learn.smoother(meta)
 
 
 @pytest.mark.parametrize("kind", ["pressure", "magnetic"])
 def test_ablation_trained_network(kind):
     """What the trained network gains over the same cycle without it (a zero
-    correction: the V-cycle plus the same post-smoothing) and over plain
-    MG-PCG, at a fixed budget of 5 iterations, on 128^3 scenes it was not
+    correction: the V-cycle plus the same post-smoothing, both with the
+    smoother settings the weights were trained under) and over plain
+    MG-PCG (the library's default smoother), at a fixed budget of 5
+    iterations, on 128^3 scenes it was not
     trained on; and the learned solve with fall-back meets the tolerance and
     agrees with plain MG-PCG. Measured with the committed weights: the
     pressure residual after 5 iterations is several times lower than both
     baselines; on the magnetic system the network gains nothing (its error is
     dominated by global modes the coarse levels already handle), so there the
     test only bounds the loss."""
-    params = _trained()
+    params, sm = _trained()
     sc = scenes.pressure_scene(128, 11) if kind == "pressure" else scenes.magnetic_scene(128, 12, 2.0)
     print(f"[case] ablation {kind} 128^3", flush=True)
     budget = 5
     kw = dict(max_iters=budget, tol=1e-14)
-    _, _, st_l = _solve(sc, sfp.PRECOND_LEARNED, params, kind, stall_ratio=10.0, **kw)
-    _, _, st_s = _solve(sc, sfp.PRECOND_SMOOTH, None, kind, **kw)
+    _, _, st_l = _solve(sc, sfp.PRECOND_LEARNED, params, kind, stall_ratio=10.0, **kw, **sm)
+    _, _, st_s = _solve(sc, sfp.PRECOND_SMOOTH, None, kind, **kw, **sm)
     _, _, st_m = _solve(sc, sfp.PRECOND_MG, None, kind, **kw)
     _, _, st_j = _solve(sc, sfp.PRECOND_JACOBI, None, kind, **kw)
     print(f"after {budget} its: learned {st_l['rel_residual']:.3e}  without network "
@@ -316,7 +319,7 @@ def test_ablation_trained_network(kind):
         assert st_l["rel_residual"] < 0.5 * st_m["rel_residual"]
     else:
         assert st_l["rel_residual"] < 3.0 * st_s["rel_residual"]
-    vel_l, pres_l, st = _solve(sc, sfp.PRECOND_LEARNED, params, kind)
+    vel_l, pres_l, st = _solve(sc, sfp.PRECOND_LEARNED, params, kind, **sm)
     _, pres_m, _ = _solve(sc, sfp.PRECOND_MG, None, kind)
     print("learned with fall-back:", st)
     assert st["converged"] and st["rel_residual"] <= 1e-8

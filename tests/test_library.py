@@ -48,8 +48,9 @@ def test_struct_layouts_match_c():
 #include "sfp.h"
 #include "sfp_profile.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(sfp_params), offsetof(sfp_params, max_iters),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(sfp_params), offsetof(sfp_params, max_iters),
          offsetof(sfp_params, cheby_ratio), offsetof(sfp_params, stall_ratio), offsetof(sfp_params, verbose),
+         offsetof(sfp_params, coarse_cheby_ratio),
          sizeof(sfp_stats), offsetof(sfp_stats, rel_residual), offsetof(sfp_stats, unknowns),
          sizeof(sfp_kernel_profile), offsetof(sfp_kernel_profile, bytes));
   return 0;
@@ -64,7 +65,7 @@ int main(void) {
                                                check=True).stdout.split()]
     P, S, K = sfp.Params, sfp.Stats, sfp.KernelProfile
     assert vals == [C.sizeof(P), P.max_iters.offset, P.cheby_ratio.offset, P.stall_ratio.offset,
-                    P.verbose.offset, C.sizeof(S), S.rel_residual.offset, S.unknowns.offset,
+                    P.verbose.offset, P.coarse_cheby_ratio.offset, C.sizeof(S), S.rel_residual.offset, S.unknowns.offset,
                     C.sizeof(K), K.bytes.offset]
 
 
@@ -72,6 +73,7 @@ def test_default_params_and_status_strings():
     p = sfp.default_params()
     assert p.tol == 1e-8 and p.max_iters == 100 and p.precond == sfp.PRECOND_MG
     assert p.mg_coarsest == 4 and p.cheby_degree == 3
+    assert p.coarse_cheby_degree == 5 and p.coarse_cheby_ratio == 6.0
     L = sfp.lib()
     for st in range(7):
         assert L.sfp_status_string(st)
@@ -135,6 +137,8 @@ def test_trained_weights_load_and_checksum():
     assert params.dtype == np.float32 and params.size == meta["count"] == 321745
     assert np.all(np.isfinite(params))
     assert meta["n"] == 128 and meta["samples"] > 0
+    sm = learn.smoother(meta)
+    assert set(sm) == set(learn.SMOOTHER_KEYS) and all(v > 0 for v in sm.values())
 
 
 def test_apply_full_matches_csr():
