@@ -77,8 +77,9 @@ HOT_KERNELS = {
                         "pass, FP32: reads x, b, d, diag, 3 couplings, writes x and the coarse b"),
     "smooth": ("k_smooth<6>", 36.0,
                "fine-level Chebyshev step with the direction in and out, FP32 7-point stencil"),
-    "matvec64": ("k_fcg_matvec", 60.0,
-                 "flexible-CG w = A z with the FP64 operator and its 3 inner products"),
+    "matvec64": ("k_fcg_matvec_flat", 60.0,
+                 "flexible-CG w = A z with the FP64 operator and its 3 inner products "
+                 "(two cells of a z row per lane)"),
     "update64": ("k_fcg_update", 88.0,
                  "flexible-CG update of p, q, x, r (FP64) with r.r, the FP32 copy b of r and "
                  "the next cycle's first iterate c0 D^-1 b"),

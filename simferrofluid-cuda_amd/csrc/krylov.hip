@@ -180,6 +180,92 @@ __global__ __launch_bounds__(kBlock) void k_fcg_matvec(CellGrid cg, Dims d, Op64
     block_partials<3>(v, partials);
 }
 
+// The same matvec over the flat cell index, two consecutive cells of one z row
+// per lane (grids with nz even): 16-byte loads of the FP64 operator and
+// vectors, 8-byte loads of z and its x and y neighbours, one extra z value and
+// z coupling at each end of the pair. Every cell's w and dot terms are formed
+// in the tiled kernel's order, so w is the same to the bit. A wavefront whose
+// 128 cells are all inactive is skipped.
+constexpr unsigned kMatvecCells = 2 * kBlock;
+
+__global__ __launch_bounds__(kBlock) void k_fcg_matvec_flat(Dims d, Op64 A,
+                                                            const float *__restrict__ active,
+                                                            const float *__restrict__ z,
+                                                            const double *__restrict__ r,
+                                                            const double *__restrict__ qprev,
+                                                            double *__restrict__ w, int first,
+                                                            double *__restrict__ partials,
+                                                            const double *gate) {
+    if (gated(gate)) return;
+    const Strides s(d);
+    const size_t c = (size_t)blockIdx.x * kMatvecCells + 2 * (size_t)flat_tid();
+    const bool on = c < d.cells();  // cells() is even: then c + 1 is a cell too
+    double v[3] = {0.0, 0.0, 0.0};  // z.w, z.q_prev, z.r
+    bool act = on;
+    if (active && on) {
+        const float2v a = *(const float2v *)(active + c);
+        act = a.x > 0.f || a.y > 0.f;
+    }
+    if (__any(act) && on) {
+        // (i, j, k) of the first cell (the launcher keeps cells() < 2^32)
+        const unsigned cu = (unsigned)c;
+        const unsigned t = cu / (unsigned)d.nz;
+        const int k = (int)(cu - t * (unsigned)d.nz);
+        const unsigned iu = t / (unsigned)d.ny;
+        const int i = (int)iu, j = (int)(t - iu * (unsigned)d.ny);
+        const bool xl = i > 0, xh = i + 1 < d.nx, yl = j > 0, yh = j + 1 < d.ny;
+        const bool zl = k > 0, zh = k + 2 < d.nz;  // neighbours below the first, above the second
+        const float2v f0 = {0.f, 0.f};
+        const double2v d0 = {0.0, 0.0};
+        const float2v zc = *(const float2v *)(z + c);
+        const float2v zxm = xl ? *(const float2v *)(z + c - s.sx) : f0;
+        const float2v zxp = xh ? *(const float2v *)(z + c + s.sx) : f0;
+        const float2v zym = yl ? *(const float2v *)(z + c - s.sy) : f0;
+        const float2v zyp = yh ? *(const float2v *)(z + c + s.sy) : f0;
+        const float zzm = zl ? z[c - 1] : 0.f;
+        const float zzp = zh ? z[c + 2] : 0.f;
+        const double2v dg = *(const double2v *)(A.diag + c);
+        const double2v cxm = xl ? *(const double2v *)(A.cpl[0] + c) : d0;
+        const double2v cxp = xh ? *(const double2v *)(A.cpl[0] + c + s.sx) : d0;
+        const double2v cym = yl ? *(const double2v *)(A.cpl[1] + c) : d0;
+        const double2v cyp = yh ? *(const double2v *)(A.cpl[1] + c + s.sy) : d0;
+        const double2v cz = *(const double2v *)(A.cpl[2] + c);  // below each cell of the pair
+        const double czp = zh ? A.cpl[2][c + 2] : 0.0;          // above the second
+        const double2v qp = first ? d0 : ld_stream((const double2v *)(qprev + c));
+        const double2v rc = ld_stream((const double2v *)(r + c));
+        double o0 = 0.0, o1 = 0.0;
+        if (xl) {
+            o0 += cxm.x * (double)zxm.x;
+            o1 += cxm.y * (double)zxm.y;
+        }
+        if (xh) {
+            o0 += cxp.x * (double)zxp.x;
+            o1 += cxp.y * (double)zxp.y;
+        }
+        if (yl) {
+            o0 += cym.x * (double)zym.x;
+            o1 += cym.y * (double)zym.y;
+        }
+        if (yh) {
+            o0 += cyp.x * (double)zyp.x;
+            o1 += cyp.y * (double)zyp.y;
+        }
+        if (zl) o0 += cz.x * (double)zzm;
+        o0 += cz.y * (double)zc.y;
+        o1 += cz.y * (double)zc.x;
+        if (zh) o1 += czp * (double)zzp;
+        const double z0 = (double)zc.x, z1 = (double)zc.y;
+        double2v wc;
+        wc.x = dg.x * z0 - o0;
+        wc.y = dg.y * z1 - o1;
+        st_stream((double2v *)(w + c), wc);
+        v[0] = z0 * wc.x + z1 * wc.y;
+        if (!first) v[1] = z0 * qp.x + z1 * qp.y;
+        v[2] = z0 * rc.x + z1 * rc.y;
+    }
+    block_partials<3>(v, partials);
+}
+
 // The update is pointwise: it runs over the flat cell index with two
 // consecutive cells per lane (16-byte loads and stores of the FP64 vectors),
 // one pair per thread (kUpdateCells per block). A wavefront whose 128 cells
@@ -357,14 +443,21 @@ void finalize(hipStream_t st, int op, unsigned nb, int first, const double *part
 
 }  // namespace
 
-// FCG matvec: 4 x planes per block (more blocks in flight than the 16 of the
-// other reductions: 6 % faster at 256^3 on MI355X).
+// FCG matvec: the flat kernel where z rows hold whole cell pairs (nz even;
+// 11 % faster than x-plane tiles at 256^3 and 9 % at 384^3 on MI355X),
+// else x-plane tiles of 4 planes per block (more blocks in flight than the 16
+// of the other reductions: 6 % faster at 256^3).
+static bool matvec_flat(Dims d) { return d.nz % 2 == 0 && d.cells() < (size_t(1) << 32); }
 static Launch matvec_launch(Dims d) { return cell_launch(d, 0, 4); }
+static unsigned matvec_blocks(Dims d) {
+    return matvec_flat(d) ? (unsigned)((d.cells() + kMatvecCells - 1) / kMatvecCells)
+                          : matvec_launch(d).blocks();
+}
 static unsigned update_blocks(Dims d) { return (unsigned)((d.cells() + kUpdateCells - 1) / kUpdateCells); }
 
 size_t reduction_partials(Dims d) {
     return std::max({3 * (size_t)cell_launch(d, kReducePlanes).blocks(),
-                     3 * (size_t)matvec_launch(d).blocks(), (size_t)update_blocks(d)});
+                     3 * (size_t)matvec_blocks(d), (size_t)update_blocks(d)});
 }
 
 void launch_count_active(hipStream_t st, Dims d, const float *diag32, double *partials,
@@ -386,6 +479,20 @@ void launch_residual64(hipStream_t st, Dims d, Op64 A, const double *b, const do
 void launch_fcg_matvec(hipStream_t st, Dims d, Op64 A, const float *active, const float *z,
                        const double *r, const double *qprev, double *w, int first,
                        double *partials, KrylovScalars *sc, const double *gate) {
+    if (matvec_flat(d)) {
+        // pairs of cells: 16-byte (8-byte for z and the flags) aligned vectors
+        for (const void *a : {(const void *)A.diag, (const void *)A.cpl[0], (const void *)A.cpl[1],
+                              (const void *)A.cpl[2], (const void *)r, (const void *)qprev, (const void *)w})
+            if ((uintptr_t)a % 16) throw ArgError("launch_fcg_matvec: FP64 array not 16-byte aligned");
+        if ((uintptr_t)z % 8 || (uintptr_t)active % 8)
+            throw ArgError("launch_fcg_matvec: z or the diagonal not 8-byte aligned");
+        const unsigned nb = matvec_blocks(d);
+        hipLaunchKernelGGL(k_fcg_matvec_flat, dim3(nb), dim3(kBlock), 0, st, d, A, active, z, r,
+                           qprev, w, first, partials, gate);
+        SFP_LAUNCH_CHECK();
+        finalize(st, FIN_MATVEC, nb, first, partials, sc, gate);
+        return;
+    }
     Launch L = matvec_launch(d);
     hipLaunchKernelGGL(k_fcg_matvec, L.grid, L.block, 0, st, L.cg, d, A, active, z, r, qprev, w,
                        first, partials, gate);

@@ -85,9 +85,12 @@ def test_pressure_default_tol_and_reproject(oracle):
         assert np.array_equal(vel2[a], ref_vel[a])
 
 
-def test_pressure_apply64_matches_csr(oracle):
-    sc = scenes.random_pressure_scene((12, 14, 16), seed=2)
-    print("[case] apply64", flush=True)
+@pytest.mark.parametrize("shape", [(12, 14, 16), (12, 14, 15), (9, 6, 130)])
+def test_pressure_apply64_matches_csr(oracle, shape):
+    """The FP64 matvec of the FCG loop (over cell pairs where nz is even, over
+    x-plane tiles where it is odd) against the oracle's CSR matrix."""
+    sc = scenes.random_pressure_scene(shape, seed=2)
+    print(f"[case] apply64 {shape}", flush=True)
     ref = oracle.pressure_system(sc)
     with sfp.Context.for_scene(sc) as ctx:
         ctx.pressure_project(sc)
