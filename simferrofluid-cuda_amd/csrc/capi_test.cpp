@@ -299,4 +299,8 @@ sfp_status sfp_test_set_vcycle_tail(sfp_context ctx, int enable) {
     return run(ctx, [&] { ctx->ws.vcycle_tail = enable != 0; });
 }
 
+sfp_status sfp_test_set_fused_pairs(sfp_context ctx, int enable) {
+    return run(ctx, [&] { ctx->ws.fused_pairs = enable != 0; });
+}
+
 }  // extern "C"

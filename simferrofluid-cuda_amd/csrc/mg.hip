@@ -11,6 +11,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <initializer_list>
 
 namespace sfp {
 
@@ -187,6 +188,76 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
         xc = xp;
         cxc = cxp;
     }
+}
+
+// The same step (modes without SM_FIRST and SM_PROLONG) over the flat cell
+// index, four consecutive cells of one z row per lane (grids with nz a
+// multiple of 4): 16-byte loads of every array at the cell, its x and y
+// neighbours and the upper x and y couplings, one more iterate value and z
+// coupling at each end of the group. Every cell's arithmetic is k_smooth's
+// (same operands, same order), so the two give the same bits. A wavefront
+// whose 256 cells are all inactive writes nothing.
+constexpr unsigned kSmoothFlatCells = 4 * kBlock;
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_smooth_flat(Dims d, Op32 A, const float *__restrict__ b,
+                                                        const float *__restrict__ xin,
+                                                        float *__restrict__ xout, float *dvec,
+                                                        float cd, float cr, const double *gate) {
+    static_assert((MODE & (SM_FIRST | SM_PROLONG)) == 0, "flat step: plain input iterate only");
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    if (gated(gate)) return;
+    const Strides s(d);
+    const size_t c = (size_t)blockIdx.x * kSmoothFlatCells + 4 * (size_t)flat_tid();
+    const bool on = c < d.cells();  // cells() is a multiple of 4
+    const f4 z4 = {0.f, 0.f, 0.f, 0.f};
+    const f4 dg = on ? *(const f4 *)(A.diag + c) : z4;
+    const bool act = dg.x > 0.f || dg.y > 0.f || dg.z > 0.f || dg.w > 0.f;
+    if (!__any(act) || !on) return;
+    // (i, j, k) of the first cell (the launcher keeps cells() < 2^32)
+    const unsigned cu = (unsigned)c;
+    const unsigned t = cu / (unsigned)d.nz;
+    const int k = (int)(cu - t * (unsigned)d.nz);
+    const unsigned iu = t / (unsigned)d.ny;
+    const int i = (int)iu, j = (int)(t - iu * (unsigned)d.ny);
+    const bool up = i + 1 < d.nx, ylo = j > 0, yhi = j + 1 < d.ny;
+    const bool zl = k > 0, zh = k + 4 < d.nz;  // neighbours below the first, above the last
+    const f4 xc = *(const f4 *)(xin + c);
+    const f4 xm = i > 0 ? *(const f4 *)(xin + c - s.sx) : z4;
+    const f4 xp = up ? *(const f4 *)(xin + c + s.sx) : z4;
+    const f4 cxc = *(const f4 *)(A.cpl[0] + c);
+    const f4 cxp = up ? *(const f4 *)(A.cpl[0] + c + s.sx) : z4;
+    const f4 cym = ylo ? *(const f4 *)(A.cpl[1] + c) : z4;
+    const f4 xym = ylo ? *(const f4 *)(xin + c - s.sy) : z4;
+    const f4 cyp = yhi ? *(const f4 *)(A.cpl[1] + c + s.sy) : z4;
+    const f4 xyp = yhi ? *(const f4 *)(xin + c + s.sy) : z4;
+    const f4 cz = *(const f4 *)(A.cpl[2] + c);
+    const float xzlo = zl ? xin[c - 1] : 0.f;
+    const float xzhi = zh ? xin[c + 4] : 0.f;
+    const float czhi = zh ? A.cpl[2][c + 4] : 0.f;
+    const f4 bc = *(const f4 *)(b + c);
+    f4 dprev = z4;
+    if constexpr ((MODE & SM_READ_D) != 0) dprev = *(const f4 *)(dvec + c);
+    f4 xo, dno;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float xzm = q > 0 ? xc[q - 1] : xzlo;
+        const float xzp = q < 3 ? xc[q + 1] : xzhi;
+        const float czp = q < 3 ? cz[q + 1] : czhi;
+        float off = fmaf(cxc[q], xm[q], cxp[q] * xp[q]);
+        if (ylo) off = fmaf(cym[q], xym[q], off);
+        if (yhi) off = fmaf(cyp[q], xyp[q], off);
+        if (q > 0 || zl) off = fmaf(cz[q], xzm, off);
+        if (q < 3 || zh) off = fmaf(czp, xzp, off);
+        const float x0 = xc[q];
+        const float r = bc[q] - fmaf(dg[q], x0, -off);
+        const float dp = (MODE & SM_XFIRST) != 0 ? x0 : dprev[q];
+        const float dn = fmaf(cd, dp, cr * rcp_or_zero(dg[q]) * r);
+        xo[q] = x0 + dn;
+        dno[q] = dn;
+    }
+    *(f4 *)(xout + c) = xo;
+    if constexpr ((MODE & SM_WRITE_D) != 0) *(f4 *)(dvec + c) = dno;
 }
 
 // Two passes over the grid in one (temporal blocking). Step A is a
@@ -906,9 +977,41 @@ void launch_smooth_first(hipStream_t st, Dims d, Op32 A, const double *r64, floa
     SFP_LAUNCH_CHECK();
 }
 
+// The flat step where it applies: no fused first step or prolongation, z rows
+// of whole 4-cell groups, 16-byte aligned arrays (on MI355X 12 % faster than
+// x-plane tiles at 256^3, 17 % at 128^3).
+static bool smooth_flat(Dims d, int mode, const double *r64, std::initializer_list<const float *> arrays) {
+    if ((mode & (SM_FIRST | SM_PROLONG)) || r64 || d.nz % 4 || d.cells() >= (size_t(1) << 32))
+        return false;
+    for (const float *a : arrays)
+        if ((uintptr_t)a % 16) return false;
+    return true;
+}
+
 void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, float *b,
                    const float *xin, float *xout, float *dvec, float c0, float cd, float cr,
                    const float *ecoarse, const double *gate) {
+    if (smooth_flat(d, mode, r64, {A.diag, A.cpl[0], A.cpl[1], A.cpl[2], b, xin, xout, dvec})) {
+        const unsigned nb = (unsigned)((d.cells() + kSmoothFlatCells - 1) / kSmoothFlatCells);
+#define SFP_FLAT_CASE(M)                                                                        \
+    case M:                                                                                     \
+        hipLaunchKernelGGL(k_smooth_flat<M>, dim3(nb), dim3(kBlock), 0, st, d, A, b, xin, xout, \
+                           dvec, cd, cr, gate);                                                 \
+        break;
+        switch (mode) {
+            SFP_FLAT_CASE(SM_XFIRST)
+            SFP_FLAT_CASE(SM_XFIRST | SM_WRITE_D)
+            SFP_FLAT_CASE(SM_READ_D)
+            SFP_FLAT_CASE(SM_READ_D | SM_WRITE_D)
+            SFP_FLAT_CASE(0)
+            SFP_FLAT_CASE(SM_WRITE_D)
+            default:
+                throw ArgError("unsupported smoother mode");
+        }
+#undef SFP_FLAT_CASE
+        SFP_LAUNCH_CHECK();
+        return;
+    }
     // 8 planes per block measured best for the shuffle-based step (256^3:
     // 122 us vs 124 us at 16 and 123 us at 4)
     Launch L = cell_launch(d, 0, 8);

@@ -222,7 +222,7 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
     };
     // the last step and the restriction run as one pass where the grid pays
     // for the pass's tile overlap
-    const bool fuse_restrict = deg >= 3 && pair_pays(L.d);
+    const bool fuse_restrict = deg >= 3 && ws.fused_pairs && pair_pays(L.d);
     if (deg == 1) {
         launch_smooth_first(st, L.d, A, r64, L.b.p, cur, cb.cr[0], ws.gate);
     } else {
@@ -260,7 +260,7 @@ float *System::post_smooth(Workspace &ws, const Smoothing &sm, int l, float *cur
     // the last two steps run as one pass where the grid is large enough to
     // pay for the pass's tile overlap (the direction they leave is not stored:
     // nothing reads it before the next cycle rewrites it)
-    const int fused = (deg >= 3 && pair_pays(L.d)) ? 2 : 0;
+    const int fused = (deg >= 3 && ws.fused_pairs && pair_pays(L.d)) ? 2 : 0;
     // on large grids the correction is added by its own pointwise pass and
     // step 0 is a plain step (cheaper than gathering the coarse values at
     // every neighbour); equal values either way

@@ -78,6 +78,9 @@ struct Workspace {
     const double *gate = nullptr;
     // run the V-cycle's small levels (<= kTailMaxCells cells) in one launch
     bool vcycle_tail = true;
+    // run step pairs (the last pre-smoothing step with the restriction, the
+    // last two post-smoothing steps) as one pass on large grids
+    bool fused_pairs = true;
     // level whose x0 holds the first Chebyshev iterate of its b (stored by
     // the FCG update for the next cycle), or -1
     int x0_level = -1;

@@ -42,7 +42,8 @@ EXPORTS = {
                    "sfp_test_export_level32", "sfp_test_apply64", "sfp_test_precondition",
                    "sfp_test_residual", "sfp_test_solve",
                    "sfp_test_restrict_smoothed", "sfp_test_smooth_pair",
-                   "sfp_test_smooth_restrict", "sfp_test_set_vcycle_tail"],
+                   "sfp_test_smooth_restrict", "sfp_test_set_vcycle_tail",
+                   "sfp_test_set_fused_pairs"],
     "sfp_learned.h": ["sfp_unet_param_count", "sfp_set_network", "sfp_learned_features",
                       "sfp_unet_create", "sfp_unet_destroy", "sfp_unet_set_params",
                       "sfp_unet_get_params", "sfp_unet_init_params", "sfp_unet_forward",
@@ -150,6 +151,8 @@ def lib():
         L.sfp_test_restrict_smoothed.argtypes = [vp, C.c_int, _dp, _fp]
         L.sfp_test_set_vcycle_tail.restype = C.c_int
         L.sfp_test_set_vcycle_tail.argtypes = [vp, C.c_int]
+        L.sfp_test_set_fused_pairs.restype = C.c_int
+        L.sfp_test_set_fused_pairs.argtypes = [vp, C.c_int]
         L.sfp_test_smooth_restrict.restype = C.c_int
         L.sfp_test_smooth_restrict.argtypes = [vp, C.c_int, C.c_int, _fp, _fp, _fp, C.c_float,
                                                C.c_float, _fp, _fp, _fp, _fp]
@@ -471,6 +474,11 @@ class Context:
     def set_vcycle_tail(self, enable):
         """Small V-cycle levels in one launch (True, the default) or level by level."""
         self._check(lib().sfp_test_set_vcycle_tail(self._h, 1 if enable else 0))
+
+    def set_fused_pairs(self, enable):
+        """Step pairs as one pass on large grids (True, the default) or as
+        separate launches."""
+        self._check(lib().sfp_test_set_fused_pairs(self._h, 1 if enable else 0))
 
     def smooth_restrict(self, which, level, b, x1, d1, cd, cr):
         """The last pre-smoothing step and the restriction on a level, from
