@@ -34,6 +34,9 @@ void launch_smooth(hipStream_t s, int mode, Dims d, Op32 A, const double *r64, f
 // pair_pays: grids on which it beats the two launches on MI355X (its 64 x 16
 // tiles overlap by one cell per side; measured equal at 128^3, faster above).
 inline bool pair_pays(Dims d) { return d.nz >= 64 && d.cells() > ((size_t)1 << 21); }
+// Grids on which launch_smooth runs its steps (all but the fused first step)
+// over the flat cell index: nz a multiple of 4, fewer than 2^32 cells.
+bool flat_step(Dims d);
 void launch_smooth_pair(hipStream_t s, Dims d, Op32 A, const float *b, const float *x1, float *x3,
                         const float *d1, float cdA, float crA, float cdB, float crB,
                         const double *gate = nullptr);

@@ -190,22 +190,28 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
     }
 }
 
-// The same step (modes without SM_FIRST and SM_PROLONG) over the flat cell
-// index, four consecutive cells of one z row per lane (grids with nz a
-// multiple of 4): 16-byte loads of every array at the cell, its x and y
-// neighbours and the upper x and y couplings, one more iterate value and z
-// coupling at each end of the group. Every cell's arithmetic is k_smooth's
-// (same operands, same order), so the two give the same bits. A wavefront
-// whose 256 cells are all inactive writes nothing.
+// The same step (modes without SM_FIRST) over the flat cell index, four
+// consecutive cells of one z row per lane (grids with nz a multiple of 4):
+// 16-byte loads of every array at the cell, its x and y neighbours and the
+// upper x and y couplings, one more iterate value and z coupling at each end
+// of the group. With SM_PROLONG the coarse correction (two coarse cells per
+// group and row) is added to every iterate value read and, on active cells,
+// to the centre, as k_prolong_add followed by a plain step would. Every cell's
+// arithmetic is k_smooth's (same operands, same order), so the two give the
+// same bits. A wavefront whose 256 cells are all inactive writes nothing.
 constexpr unsigned kSmoothFlatCells = 4 * kBlock;
 
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_smooth_flat(Dims d, Op32 A, const float *__restrict__ b,
                                                         const float *__restrict__ xin,
                                                         float *__restrict__ xout, float *dvec,
-                                                        float cd, float cr, const double *gate) {
-    static_assert((MODE & (SM_FIRST | SM_PROLONG)) == 0, "flat step: plain input iterate only");
+                                                        float cd, float cr,
+                                                        const float *__restrict__ e, Dims dc,
+                                                        const double *gate) {
+    static_assert((MODE & SM_FIRST) == 0, "flat step: no fused first step");
+    constexpr bool kProlong = (MODE & SM_PROLONG) != 0;
     typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef float f2 __attribute__((ext_vector_type(2)));
     if (gated(gate)) return;
     const Strides s(d);
     const size_t c = (size_t)blockIdx.x * kSmoothFlatCells + 4 * (size_t)flat_tid();
@@ -222,22 +228,46 @@ __global__ __launch_bounds__(kBlock) void k_smooth_flat(Dims d, Op32 A, const fl
     const int i = (int)iu, j = (int)(t - iu * (unsigned)d.ny);
     const bool up = i + 1 < d.nx, ylo = j > 0, yhi = j + 1 < d.ny;
     const bool zl = k > 0, zh = k + 4 < d.nz;  // neighbours below the first, above the last
-    const f4 xc = *(const f4 *)(xin + c);
-    const f4 xm = i > 0 ? *(const f4 *)(xin + c - s.sx) : z4;
-    const f4 xp = up ? *(const f4 *)(xin + c + s.sx) : z4;
+    f4 xc = *(const f4 *)(xin + c);
+    f4 xm = i > 0 ? *(const f4 *)(xin + c - s.sx) : z4;
+    f4 xp = up ? *(const f4 *)(xin + c + s.sx) : z4;
     const f4 cxc = *(const f4 *)(A.cpl[0] + c);
     const f4 cxp = up ? *(const f4 *)(A.cpl[0] + c + s.sx) : z4;
     const f4 cym = ylo ? *(const f4 *)(A.cpl[1] + c) : z4;
-    const f4 xym = ylo ? *(const f4 *)(xin + c - s.sy) : z4;
+    f4 xym = ylo ? *(const f4 *)(xin + c - s.sy) : z4;
     const f4 cyp = yhi ? *(const f4 *)(A.cpl[1] + c + s.sy) : z4;
-    const f4 xyp = yhi ? *(const f4 *)(xin + c + s.sy) : z4;
+    f4 xyp = yhi ? *(const f4 *)(xin + c + s.sy) : z4;
     const f4 cz = *(const f4 *)(A.cpl[2] + c);
-    const float xzlo = zl ? xin[c - 1] : 0.f;
-    const float xzhi = zh ? xin[c + 4] : 0.f;
+    float xzlo = zl ? xin[c - 1] : 0.f;
+    float xzhi = zh ? xin[c + 4] : 0.f;
     const float czhi = zh ? A.cpl[2][c + 4] : 0.f;
     const f4 bc = *(const f4 *)(b + c);
     f4 dprev = z4;
     if constexpr ((MODE & SM_READ_D) != 0) dprev = *(const f4 *)(dvec + c);
+    const f4 xin_c = xc;
+    if constexpr (kProlong) {
+        // coarse values of the group's two coarse cells in this row and in the
+        // neighbouring rows and planes (16-byte aligned groups: k >> 1 even)
+        const int kc = k >> 1;
+        auto e2 = [&](int ii, int jj) { return *(const f2 *)(e + dc.idx(ii >> 1, jj >> 1, kc)); };
+        const f2 ec = e2(i, j);
+        const f2 e_xm = i > 0 ? e2(i - 1, j) : f2{0.f, 0.f};
+        const f2 e_xp = up ? e2(i + 1, j) : f2{0.f, 0.f};
+        const f2 e_ym = ylo ? e2(i, j - 1) : f2{0.f, 0.f};
+        const f2 e_yp = yhi ? e2(i, j + 1) : f2{0.f, 0.f};
+        const float ezlo = zl ? e[dc.idx(i >> 1, j >> 1, kc - 1)] : 0.f;
+        const float ezhi = zh ? e[dc.idx(i >> 1, j >> 1, kc + 2)] : 0.f;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            xc[q] = xc[q] + ec[q >> 1];
+            xm[q] = xm[q] + e_xm[q >> 1];
+            xp[q] = xp[q] + e_xp[q >> 1];
+            xym[q] = xym[q] + e_ym[q >> 1];
+            xyp[q] = xyp[q] + e_yp[q >> 1];
+        }
+        xzlo = xzlo + ezlo;
+        xzhi = xzhi + ezhi;
+    }
     f4 xo, dno;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -249,7 +279,8 @@ __global__ __launch_bounds__(kBlock) void k_smooth_flat(Dims d, Op32 A, const fl
         if (yhi) off = fmaf(cyp[q], xyp[q], off);
         if (q > 0 || zl) off = fmaf(cz[q], xzm, off);
         if (q < 3 || zh) off = fmaf(czp, xzp, off);
-        const float x0 = xc[q];
+        // centre iterate: the coarse correction only lands on active cells
+        const float x0 = (kProlong && !(dg[q] > 0.f)) ? xin_c[q] : xc[q];
         const float r = bc[q] - fmaf(dg[q], x0, -off);
         const float dp = (MODE & SM_XFIRST) != 0 ? x0 : dprev[q];
         const float dn = fmaf(cd, dp, cr * rcp_or_zero(dg[q]) * r);
@@ -977,12 +1008,15 @@ void launch_smooth_first(hipStream_t st, Dims d, Op32 A, const double *r64, floa
     SFP_LAUNCH_CHECK();
 }
 
-// The flat step where it applies: no fused first step or prolongation, z rows
-// of whole 4-cell groups, 16-byte aligned arrays (on MI355X 12 % faster than
-// x-plane tiles at 256^3, 17 % at 128^3).
-static bool smooth_flat(Dims d, int mode, const double *r64, std::initializer_list<const float *> arrays) {
-    if ((mode & (SM_FIRST | SM_PROLONG)) || r64 || d.nz % 4 || d.cells() >= (size_t(1) << 32))
-        return false;
+bool flat_step(Dims d) { return d.nz % 4 == 0 && d.cells() < (size_t(1) << 32); }
+
+// The flat step where it applies: no fused first step, z rows of whole
+// 4-cell groups, 16-byte aligned arrays (8-byte aligned coarse values). On
+// MI355X 12 % faster than x-plane tiles at 256^3, 17 % at 128^3.
+static bool smooth_flat(Dims d, int mode, const double *r64, const float *ecoarse,
+                        std::initializer_list<const float *> arrays) {
+    if ((mode & SM_FIRST) || r64 || !flat_step(d)) return false;
+    if ((mode & SM_PROLONG) && (uintptr_t)ecoarse % 8) return false;
     for (const float *a : arrays)
         if ((uintptr_t)a % 16) return false;
     return true;
@@ -991,14 +1025,17 @@ static bool smooth_flat(Dims d, int mode, const double *r64, std::initializer_li
 void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, float *b,
                    const float *xin, float *xout, float *dvec, float c0, float cd, float cr,
                    const float *ecoarse, const double *gate) {
-    if (smooth_flat(d, mode, r64, {A.diag, A.cpl[0], A.cpl[1], A.cpl[2], b, xin, xout, dvec})) {
+    if (smooth_flat(d, mode, r64, ecoarse, {A.diag, A.cpl[0], A.cpl[1], A.cpl[2], b, xin, xout, dvec})) {
         const unsigned nb = (unsigned)((d.cells() + kSmoothFlatCells - 1) / kSmoothFlatCells);
+        const Dims dc = d.coarse();
 #define SFP_FLAT_CASE(M)                                                                        \
     case M:                                                                                     \
         hipLaunchKernelGGL(k_smooth_flat<M>, dim3(nb), dim3(kBlock), 0, st, d, A, b, xin, xout, \
-                           dvec, cd, cr, gate);                                                 \
+                           dvec, cd, cr, ecoarse, dc, gate);                                    \
         break;
         switch (mode) {
+            SFP_FLAT_CASE(SM_PROLONG)
+            SFP_FLAT_CASE(SM_PROLONG | SM_WRITE_D)
             SFP_FLAT_CASE(SM_XFIRST)
             SFP_FLAT_CASE(SM_XFIRST | SM_WRITE_D)
             SFP_FLAT_CASE(SM_READ_D)

@@ -261,10 +261,10 @@ float *System::post_smooth(Workspace &ws, const Smoothing &sm, int l, float *cur
     // pay for the pass's tile overlap (the direction they leave is not stored:
     // nothing reads it before the next cycle rewrites it)
     const int fused = (deg >= 3 && ws.fused_pairs && pair_pays(L.d)) ? 2 : 0;
-    // on large grids the correction is added by its own pointwise pass and
-    // step 0 is a plain step (cheaper than gathering the coarse values at
-    // every neighbour); equal values either way
-    const bool add_first = pair_pays(L.d);
+    // step 0 adds the coarse correction: in the flat step's gathers, else on
+    // large grids by its own pointwise pass before a plain step (cheaper than
+    // the x-plane tiles' gathers at every neighbour); equal values every way
+    const bool add_first = !flat_step(L.d) && pair_pays(L.d);
     if (add_first) launch_prolong_add(st, L.d, L.diag32.p, e, cur, ws.gate);
     for (int s = 0; s < deg - fused; s++) {
         // step 0 adds the prolonged correction; the fine-level steps that read
