@@ -75,15 +75,18 @@ HOT_KERNELS = {
     "smooth_restrict": ("k_smooth_pair<true>", 32.5,
                         "fine-level last pre-smoothing step and the residual restriction in one "
                         "pass, FP32: reads x, b, d, diag, 3 couplings, writes x and the coarse b"),
-    "smooth": ("k_smooth<6>", 36.0,
-               "fine-level Chebyshev step with the direction in and out, FP32 7-point stencil"),
+    "smooth": ("k_smooth_flat", 32.0,
+               "fine-level single Chebyshev steps over the flat cell index (the first two "
+               "pre-smoothing steps from the stored iterate, the last one, the first "
+               "post-smoothing step with the coarse correction), FP32 7-point stencil: reads x, "
+               "b, diag, 3 couplings (and d), writes x (and d)"),
     "matvec64": ("k_fcg_matvec_flat", 60.0,
                  "flexible-CG w = A z with the FP64 operator and its 3 inner products "
                  "(two cells of a z row per lane)"),
     "update64": ("k_fcg_update", 88.0,
                  "flexible-CG update of p, q, x, r (FP64) with r.r, the FP32 copy b of r and "
                  "the next cycle's first iterate c0 D^-1 b"),
-    "restrict": ("k_restrict", 24.5, "fine-level FP32 residual summed over 2x2x2 children"),
+    "restrict": ("k_restrict_flat", 24.5, "fine-level FP32 residual summed over 2x2x2 children"),
 }
 
 
@@ -332,7 +335,8 @@ def main():
         # each system after the timed steps (event pairs around every launch
         # would perturb the timed steps and keep FCG from batching iterations)
         # Only systems whose every cell is active count: pressure solves skip
-        # wavefronts of air cells, which the 36 B/cell figure would overcount.
+        # wavefronts of air cells, which the per-cell byte figures would
+        # overcount.
         refill_ring()
         ctx.profile(True)
         prof = [i for i, (_, chi) in enumerate(systems) if chi is not None] or list(range(len(systems)))

@@ -13,7 +13,7 @@ extern "C" {
 #endif
 
 typedef enum sfp_kernel_id {
-    SFP_KERNEL_SMOOTH = 0,   /* fine-level Chebyshev step with direction in and out, FP32 */
+    SFP_KERNEL_SMOOTH = 0,   /* fine-level single Chebyshev steps (all but the fused first), FP32 */
     SFP_KERNEL_MATVEC64 = 1, /* flexible-CG matvec fused with its inner products, FP64 */
     SFP_KERNEL_UPDATE64 = 2, /* flexible-CG vector update, FP64 */
     SFP_KERNEL_RESTRICT = 3, /* fine-level residual + restriction, FP32 */

@@ -299,8 +299,11 @@ sfp_status sfp_test_set_vcycle_tail(sfp_context ctx, int enable) {
     return run(ctx, [&] { ctx->ws.vcycle_tail = enable != 0; });
 }
 
-sfp_status sfp_test_set_fused_pairs(sfp_context ctx, int enable) {
-    return run(ctx, [&] { ctx->ws.fused_pairs = enable != 0; });
+sfp_status sfp_test_set_fused_pairs(sfp_context ctx, int mask) {
+    return run(ctx, [&] {
+        if (mask < 0 || mask > 3) throw ArgError("fused pairs: mask is 0..3");
+        ctx->ws.fused_pairs = mask;
+    });
 }
 
 }  // extern "C"

@@ -584,6 +584,87 @@ __global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 
     }
 }
 
+// The same restriction over the flat cell index (grids with nz a multiple of
+// 4): a wavefront covers 64 cells of a z row in two rows and two planes, four
+// cells per lane (lanes 0-15: row j, plane i; 16-31: row j+1; 32-63: plane
+// i+1), so a coarse cell's children are added in k_restrict's order -- the z
+// pair in the lane, the y pair across lanes 16 apart, the x pair across lanes
+// 32 apart, onto a zero -- and the first 16 lanes store two coarse values
+// each. Each active cell's residual is k_restrict's arithmetic, so the two
+// give the same values. A wavefront whose 256 cells are all inactive only
+// stores zeros.
+constexpr int kRestrictFlatZ = 64;  // z cells per wavefront
+
+__global__ __launch_bounds__(kBlock) void k_restrict_flat(Dims df, Op32 A, const float *__restrict__ b,
+                                                          const float *__restrict__ x, Dims dc,
+                                                          float *__restrict__ bc, const double *gate) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    if (gated(gate)) return;
+    const int lane = threadIdx.x & 63;
+    const unsigned wave = (unsigned)(blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const unsigned zch = (unsigned)(df.nz + kRestrictFlatZ - 1) / kRestrictFlatZ;
+    const unsigned yp = (unsigned)(df.ny + 1) / 2;
+    const unsigned kz = wave % zch, t = wave / zch;
+    const unsigned jp = t % yp, ip = t / yp;
+    if (ip >= (unsigned)(df.nx + 1) / 2) return;  // uniform over the wavefront (no barriers follow)
+    const int i = 2 * (int)ip + (lane >> 5), j = 2 * (int)jp + ((lane >> 4) & 1);
+    const int k = kRestrictFlatZ * (int)kz + 4 * (lane & 15);
+    const bool on = i < df.nx && j < df.ny && k < df.nz;
+    const Strides s(df);
+    const size_t c = on ? df.idx(i, j, k) : 0;
+    const f4 z4 = {0.f, 0.f, 0.f, 0.f};
+    const f4 dg = on ? *(const f4 *)(A.diag + c) : z4;
+    f4 r = z4;
+    if (__any(dg.x > 0.f || dg.y > 0.f || dg.z > 0.f || dg.w > 0.f) && on) {
+        const bool up = i + 1 < df.nx, ylo = j > 0, yhi = j + 1 < df.ny;
+        const bool zl = k > 0, zh = k + 4 < df.nz;
+        const f4 xc = *(const f4 *)(x + c);
+        const f4 xm = i > 0 ? *(const f4 *)(x + c - s.sx) : z4;
+        const f4 xp = up ? *(const f4 *)(x + c + s.sx) : z4;
+        const f4 cxc = *(const f4 *)(A.cpl[0] + c);
+        const f4 cxp = up ? *(const f4 *)(A.cpl[0] + c + s.sx) : z4;
+        const f4 cym = ylo ? *(const f4 *)(A.cpl[1] + c) : z4;
+        const f4 xym = ylo ? *(const f4 *)(x + c - s.sy) : z4;
+        const f4 cyp = yhi ? *(const f4 *)(A.cpl[1] + c + s.sy) : z4;
+        const f4 xyp = yhi ? *(const f4 *)(x + c + s.sy) : z4;
+        const f4 cz = *(const f4 *)(A.cpl[2] + c);
+        const float xzlo = zl ? x[c - 1] : 0.f;
+        const float xzhi = zh ? x[c + 4] : 0.f;
+        const float czhi = zh ? A.cpl[2][c + 4] : 0.f;
+        const f4 bv = *(const f4 *)(b + c);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (!(dg[q] > 0.f)) continue;
+            const float xzm = q > 0 ? xc[q - 1] : xzlo;
+            const float xzp = q < 3 ? xc[q + 1] : xzhi;
+            const float czp = q < 3 ? cz[q + 1] : czhi;
+            float off = 0.f;
+            if (i > 0) off = fmaf(cxc[q], xm[q], off);
+            if (up) off = fmaf(cxp[q], xp[q], off);
+            if (ylo) off = fmaf(cym[q], xym[q], off);
+            if (yhi) off = fmaf(cyp[q], xyp[q], off);
+            if (q > 0 || zl) off = fmaf(cz[q], xzm, off);
+            if (q < 3 || zh) off = fmaf(czp, xzp, off);
+            r[q] = bv[q] - fmaf(dg[q], xc[q], -off);
+        }
+    }
+    float s0 = r.x + r.y, s1 = r.z + r.w;  // z pairs
+    s0 += __shfl_xor(s0, 16);              // y pairs
+    s1 += __shfl_xor(s1, 16);
+    const float o0 = __shfl_xor(s0, 32), o1 = __shfl_xor(s1, 32);  // the other plane's
+    if (lane < 16 && on) {
+        f2 v;
+        v.x = 0.f;
+        v.x += s0;
+        v.x += o0;
+        v.y = 0.f;
+        v.y += s1;
+        v.y += o1;
+        *(f2 *)(bc + dc.idx((int)ip, (int)jp, k >> 1)) = v;
+    }
+}
+
 // Galerkin coarsening over the fine grid (the mapping of k_restrict): each
 // fine cell forms its contributions to its aggregate -- the active cell's
 // row sum diag - (its 6 couplings), its lower-face couplings where it lies on
@@ -1118,6 +1199,16 @@ void launch_prolong_add(hipStream_t st, Dims d, const float *diag32, const float
 void launch_restrict(hipStream_t st, Dims df, Op32 A, const float *b, const float *x, float *bc,
                      const double *gate) {
     Dims dc = df.coarse();
+    bool flat = flat_step(df) && (uintptr_t)bc % 8 == 0;
+    for (const float *a : {A.diag, A.cpl[0], A.cpl[1], A.cpl[2], b, x}) flat = flat && (uintptr_t)a % 16 == 0;
+    if (flat) {
+        const size_t waves = (size_t)((df.nz + kRestrictFlatZ - 1) / kRestrictFlatZ) *
+                             ((df.ny + 1) / 2) * ((df.nx + 1) / 2);
+        const unsigned nb = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_restrict_flat, dim3(nb), dim3(kBlock), 0, st, df, A, b, x, dc, bc, gate);
+        SFP_LAUNCH_CHECK();
+        return;
+    }
     Launch L = cell_launch(df, 0, 8);
     if (L.cg.planes & 1) L = cell_launch(df, L.cg.planes + 1);  // plane pairs stay in one block
     hipLaunchKernelGGL(k_restrict, L.grid, L.block, 0, st, L.cg, df, A, b, x, dc, bc, gate);

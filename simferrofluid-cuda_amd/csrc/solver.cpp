@@ -194,6 +194,17 @@ float *System::vcycle_tail(Workspace &ws, const Smoothing &sm, int l) {
     return tail_result(t);
 }
 
+// Algorithmic bytes per cell of one single Chebyshev step: x, b, diag and 3
+// couplings in and x out, plus the direction in and out by mode and the
+// coarse correction (an eighth of a value per cell).
+static double step_bytes(int mode) {
+    double b = 4.0 * 7;
+    if (mode & SM_READ_D) b += 4.0;
+    if (mode & SM_WRITE_D) b += 4.0;
+    if (mode & SM_PROLONG) b += 0.5;
+    return b;
+}
+
 float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r64) {
     if (!r64) {
         if (float *z = vcycle_tail(ws, sm, l)) return z;
@@ -212,17 +223,16 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
     float *cur = L.x0.p, *oth = L.x1.p;
     const int deg = cb.degree;
     auto smooth = [&](int mode, float cd, float cr, const double *r, const float *e) {
-        // timed: the fine-level step that reads and writes the direction,
-        // 36 algorithmic bytes per cell (x, b, d, diag, 3 couplings, x', d')
-        const bool track = fine && ws.profiling && mode == (SM_READ_D | SM_WRITE_D);
-        if (track) ws.timers[K_SMOOTH].begin(st, 36.0 * cells);
+        // timed: the fine-level single steps but the fused first step
+        const bool track = fine && ws.profiling && !(mode & SM_FIRST);
+        if (track) ws.timers[K_SMOOTH].begin(st, step_bytes(mode) * cells);
         launch_smooth(st, mode, L.d, A, r, L.b.p, cur, oth, L.dv.p, cb.cr[0], cd, cr, e, ws.gate);
         if (track) ws.timers[K_SMOOTH].end(st);
         std::swap(cur, oth);
     };
     // the last step and the restriction run as one pass where the grid pays
     // for the pass's tile overlap
-    const bool fuse_restrict = deg >= 3 && ws.fused_pairs && pair_pays(L.d);
+    const bool fuse_restrict = deg >= 3 && (ws.fused_pairs & 1) && pair_pays(L.d);
     if (deg == 1) {
         launch_smooth_first(st, L.d, A, r64, L.b.p, cur, cb.cr[0], ws.gate);
     } else {
@@ -260,18 +270,17 @@ float *System::post_smooth(Workspace &ws, const Smoothing &sm, int l, float *cur
     // the last two steps run as one pass where the grid is large enough to
     // pay for the pass's tile overlap (the direction they leave is not stored:
     // nothing reads it before the next cycle rewrites it)
-    const int fused = (deg >= 3 && ws.fused_pairs && pair_pays(L.d)) ? 2 : 0;
+    const int fused = (deg >= 3 && (ws.fused_pairs & 2) && pair_pays(L.d)) ? 2 : 0;
     // step 0 adds the coarse correction: in the flat step's gathers, else on
     // large grids by its own pointwise pass before a plain step (cheaper than
     // the x-plane tiles' gathers at every neighbour); equal values every way
     const bool add_first = !flat_step(L.d) && pair_pays(L.d);
     if (add_first) launch_prolong_add(st, L.d, L.diag32.p, e, cur, ws.gate);
     for (int s = 0; s < deg - fused; s++) {
-        // step 0 adds the prolonged correction; the fine-level steps that read
-        // and write the direction (36 algorithmic bytes per cell) are timed
+        // step 0 adds the prolonged correction; the fine-level steps are timed
         const int mode = (s == 0 ? (add_first ? 0 : SM_PROLONG) : SM_READ_D) | (s < deg - 1 ? SM_WRITE_D : 0);
-        const bool t = track && mode == (SM_READ_D | SM_WRITE_D);
-        if (t) ws.timers[K_SMOOTH].begin(st, 36.0 * (double)L.d.cells());
+        const bool t = track;
+        if (t) ws.timers[K_SMOOTH].begin(st, step_bytes(mode) * (double)L.d.cells());
         launch_smooth(st, mode, L.d, A, nullptr, L.b.p, cur, oth, L.dv.p, cb.cr[0], s == 0 ? 0.f : cb.cd[s],
                       s == 0 ? cb.cr[0] : cb.cr[s], s == 0 ? e : nullptr, ws.gate);
         if (t) ws.timers[K_SMOOTH].end(st);
