@@ -94,6 +94,8 @@ void launch_vcycle_tail(hipStream_t s, const TailArgs &t, const double *gate = n
 // The buffer that holds the tail's result for its first level.
 inline float *tail_result(const TailArgs &t) { return t.lv[0].x1; }
 void launch_coarse_invert(hipStream_t s, Dims d, Op64 A, float *ainv);
+// v[0..3][c] = 0 (and w[c] = 0 when w is given) at the cells with diag32[c] == 0.
+void launch_clear_inactive(hipStream_t s, Dims d, const float *diag32, float *const v[4], double *w);
 void launch_coarse_solve(hipStream_t s, int n, const float *ainv, const float *b, float *x,
                          const double *gate = nullptr);
 
@@ -103,8 +105,12 @@ constexpr int kReducePlanes = 0;  // automatic
 // Doubles of partial-sum scratch a reduction over grid d needs.
 size_t reduction_partials(Dims d);
 // r = b - A x (x may be null = 0); scalars->rr = r.r, scalars->bb = b.b.
+// With b32, also b32 = r rounded to FP32 and, with x0, x0 = c0 D32^-1 b32
+// (diag32: the FP32 diagonal): the right-hand side and first Chebyshev
+// iterate of a V-cycle on r, as the FCG update stores them.
 void launch_residual64(hipStream_t s, Dims d, Op64 A, const double *b, const double *x,
-                       double *r, double *partials, KrylovScalars *sc);
+                       double *r, double *partials, KrylovScalars *sc, float *b32 = nullptr,
+                       float *x0 = nullptr, const float *diag32 = nullptr, float c0 = 0.f);
 // w = A z; dots z.w, z.q_prev, z.r; the last block forms beta, p.q and alpha.
 // active: the FP32 diagonal (0 = inactive cell) to skip all-inactive
 // wavefronts, where z must be 0; null = every cell (any z).

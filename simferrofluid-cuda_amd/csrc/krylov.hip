@@ -39,7 +39,11 @@ __global__ __launch_bounds__(kBlock) void k_residual64(CellGrid cg, Dims d, Op64
                                                        const double *__restrict__ b,
                                                        const double *__restrict__ x,
                                                        double *__restrict__ r,
-                                                       double *__restrict__ partials) {
+                                                       double *__restrict__ partials,
+                                                       float *__restrict__ b32,
+                                                       float *__restrict__ x0,
+                                                       const float *__restrict__ diag32,
+                                                       float c0) {
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     const Strides s(d);
@@ -49,11 +53,19 @@ __global__ __launch_bounds__(kBlock) void k_residual64(CellGrid cg, Dims d, Op64
     const int ie = min(i0 + cg.planes, d.nx);
     double v[2] = {0.0, 0.0};  // r.r, b.b
     size_t c = on ? d.idx(i0, j, k) : 0;
+    // the FP32 copy of r and the cycle's first iterate (the FCG update's arithmetic)
+    auto store32 = [&](size_t q, double rq) {
+        if (!b32) return;
+        const float bf = (float)rq;
+        b32[q] = bf;
+        if (x0) x0[q] = c0 * rcp_or_zero(diag32[q]) * bf;
+    };
     if (!x) {
         if (on)
             for (int i = i0; i < ie; i++, c += s.sx) {
                 const double bc = b[c];
                 r[c] = bc;
+                store32(c, bc);
                 v[0] += bc * bc;
                 v[1] += bc * bc;
             }
@@ -87,6 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_residual64(CellGrid cg, Dims d, Op64
             const double bc = b[c];
             const double rc = bc - (A.diag[c] * xc - off);
             r[c] = rc;
+            store32(c, rc);
             v[0] += rc * rc;
             v[1] += bc * bc;
         }
@@ -469,9 +482,12 @@ void launch_count_active(hipStream_t st, Dims d, const float *diag32, double *pa
 }
 
 void launch_residual64(hipStream_t st, Dims d, Op64 A, const double *b, const double *x,
-                       double *r, double *partials, KrylovScalars *sc) {
+                       double *r, double *partials, KrylovScalars *sc, float *b32, float *x0,
+                       const float *diag32, float c0) {
+    if (x0 && (!b32 || !diag32)) throw ArgError("launch_residual64: x0 needs b32 and the diagonal");
     Launch L = cell_launch(d, kReducePlanes);
-    hipLaunchKernelGGL(k_residual64, L.grid, L.block, 0, st, L.cg, d, A, b, x, r, partials);
+    hipLaunchKernelGGL(k_residual64, L.grid, L.block, 0, st, L.cg, d, A, b, x, r, partials, b32, x0,
+                       diag32, c0);
     SFP_LAUNCH_CHECK();
     finalize(st, FIN_RESIDUAL, L.blocks(), 0, partials, sc);
 }

@@ -665,6 +665,23 @@ __global__ __launch_bounds__(kBlock) void k_restrict_flat(Dims df, Op32 A, const
     }
 }
 
+// Zeros the level vectors (and, if given, one FP64 vector) at the inactive
+// cells (FP32 diagonal 0), one cell per lane: a new operator may make inactive
+// cells that were unknowns before, and the values left there would survive in
+// the groups of inactive cells the stencil kernels skip. Active cells are
+// written before they are read in every cycle, so they need no clearing.
+__global__ __launch_bounds__(kBlock) void k_clear_inactive(size_t n, const float *__restrict__ diag32,
+                                                           float *v0, float *v1, float *v2, float *v3,
+                                                           double *w) {
+    const size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n || diag32[c] > 0.f) return;
+    v0[c] = 0.f;
+    v1[c] = 0.f;
+    v2[c] = 0.f;
+    v3[c] = 0.f;
+    if (w) w[c] = 0.0;
+}
+
 // Galerkin coarsening over the fine grid (the mapping of k_restrict): each
 // fine cell forms its contributions to its aggregate -- the active cell's
 // row sum diag - (its 6 couplings), its lower-face couplings where it lies on
@@ -1193,6 +1210,13 @@ void launch_prolong_add(hipStream_t st, Dims d, const float *diag32, const float
                         const double *gate) {
     Launch L = cell_launch(d, 0, 8);
     hipLaunchKernelGGL(k_prolong_add, L.grid, L.block, 0, st, L.cg, d, diag32, ecoarse, d.coarse(), x, gate);
+    SFP_LAUNCH_CHECK();
+}
+
+void launch_clear_inactive(hipStream_t st, Dims d, const float *diag32, float *const v[4], double *w) {
+    const size_t n = d.cells();
+    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_clear_inactive, dim3(nb), dim3(kBlock), 0, st, n, diag32, v[0], v[1], v[2], v[3], w);
     SFP_LAUNCH_CHECK();
 }
 

@@ -159,15 +159,17 @@ void System::build_hierarchy(Workspace &ws, const SolverConfig &cfg) {
     // Every kernel skips groups of inactive cells (rows, tiles, pairs of
     // rows) and writes 0 at the inactive cells of the groups it processes
     // only if their inputs are 0 there; the groups differ between kernels.
-    // A new operator may have a new active set, so the vectors start each
-    // solve at 0 (values left on cells that were active before would
-    // otherwise leak into z, w, p, q and the inner products).
-    for (Level &L : lv_) {
-        const size_t n = L.d.cells();
-        for (auto *v : {&L.x0, &L.x1, &L.dv, &L.b})
-            SFP_HIP(hipMemsetAsync(v->p, 0, n * sizeof(float), ws.stream));
+    // A new operator may have a new active set, so the vectors are cleared at
+    // the inactive cells (values left on cells that were active before would
+    // otherwise leak into z, w, q and the inner products). Active cells are
+    // written before they are read in every cycle; p and q are written at
+    // every cell of the update's groups by the first iteration and read in
+    // those groups only.
+    for (size_t l = 0; l < lv_.size(); l++) {
+        Level &L = lv_[l];
+        float *const v[4] = {L.x0.p, L.x1.p, L.dv.p, L.b.p};
+        launch_clear_inactive(ws.stream, L.d, L.diag32.p, v, l == 0 ? w_.p : nullptr);
     }
-    for (auto *v : {&p_, &q_, &w_}) SFP_HIP(hipMemsetAsync(v->p, 0, d_.cells() * sizeof(double), ws.stream));
 }
 
 float *System::vcycle_tail(Workspace &ws, const Smoothing &sm, int l) {
@@ -385,8 +387,15 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     const bool fine = level == 0;
     const bool profiling = fine && ws.profiling;
     if (!warm) SFP_HIP(hipMemsetAsync(x_.p, 0, n * sizeof(double), st));
+    // The V-cycle preconditioners start from the residual rounded to FP32 in
+    // the level's b, with the cycle's first Chebyshev iterate in the level's
+    // x0: the residual kernels and the FCG update store both, so no cycle
+    // reads the FP64 residual (the Jacobi preconditioner reads r).
+    float *b32 = cfg.precond != 2 ? lv_[level].b.p : nullptr;
+    float *x0 = b32 ? lv_[level].x0.p : nullptr;
+    const float c0 = Smoothing::make(cfg).at(level).cr[0];
     launch_residual64(st, d, A, rhs_.p, warm ? x_.p : nullptr, r_.p, ws.partials.p,
-                      ws.scalars.p);
+                      ws.scalars.p, b32, x0, lv_[level].diag32.p, c0);
     ws.read_scalars();
     const double bb = ws.host_sc->bb;
     if (bb == 0.0) {  // zero right-hand side: x = 0
@@ -403,12 +412,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     // than stall_ratio; then the multigrid V-cycle
     bool learned = fine && cfg.precond == 1 && learned_ok(ws);
     if (learned) prepare_learned(ws);
-    // The V-cycle preconditioners start from the residual rounded to FP32 in
-    // the level's b: the update stores it, so the cycle after an update reads
-    // b instead of the FP64 residual (the Jacobi preconditioner reads r).
-    float *b32 = cfg.precond != 2 ? lv_[level].b.p : nullptr;
-    bool b_ready = false;  // lv_[level].b holds r rounded (and lv_[level].x0 its first iterate)
-    const float c0 = Smoothing::make(cfg).at(level).cr[0];
+    bool b_ready = b32 != nullptr;  // lv_[level].b holds r rounded (and lv_[level].x0 its first iterate)
     info.precond_used = learned ? 1 : (cfg.precond == 1 ? 0 : cfg.precond);
     while (true) {
         if (rel <= cfg.tol) {
@@ -416,12 +420,12 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
             // recursive residual reached tol: recompute it from x in FP64 and
             // restart from the true residual if the two have drifted apart
             launch_residual64(st, d, A, rhs_.p, x_.p, r_.p, ws.partials.p,
-                              ws.scalars.p);
+                              ws.scalars.p, b32, x0, lv_[level].diag32.p, c0);
             ws.read_scalars();
             rel = std::sqrt(ws.host_sc->rr / bb);
             checked = true;
             first = true;
-            b_ready = false;
+            b_ready = b32 != nullptr;
             continue;
         }
         if (info.iterations >= cfg.max_iters) break;
