@@ -61,9 +61,9 @@ sfp_status sfp_test_smooth_restrict(sfp_context ctx, int which, int level, const
 /* Run the V-cycle's small levels (at most 4096 cells) in one launch (enable
  * != 0, the default) or level by level. Results are the same bit for bit. */
 sfp_status sfp_test_set_vcycle_tail(sfp_context ctx, int enable);
-/* On grids large enough for them, run as one pass (mask bit 0) the last
- * pre-smoothing step with the restriction and (bit 1) the last two
- * post-smoothing steps, or each as separate launches; the default is 2.
+/* Run as one pass (mask bit 0) the last pre-smoothing step with the
+ * restriction and (bit 1) the last two post-smoothing steps on large grids,
+ * (bits 2 and 3) the same on small grids, or each as separate launches.
  * Results are the same bit for bit. */
 sfp_status sfp_test_set_fused_pairs(sfp_context ctx, int mask);
 

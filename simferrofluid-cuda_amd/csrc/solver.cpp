@@ -196,6 +196,9 @@ float *System::vcycle_tail(Workspace &ws, const Smoothing &sm, int l) {
     return tail_result(t);
 }
 
+// Grids whose kernels are bound by launch latency rather than bytes.
+static bool small_grid(Dims d) { return d.cells() <= ((size_t)1 << 19); }
+
 // Algorithmic bytes per cell of one single Chebyshev step: x, b, diag and 3
 // couplings in and x out, plus the direction in and out by mode and the
 // coarse correction (an eighth of a value per cell).
@@ -234,7 +237,8 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
     };
     // the last step and the restriction run as one pass where the grid pays
     // for the pass's tile overlap
-    const bool fuse_restrict = deg >= 3 && (ws.fused_pairs & 1) && pair_pays(L.d);
+    const bool fuse_restrict =
+        deg >= 3 && (((ws.fused_pairs & 1) && pair_pays(L.d)) || ((ws.fused_pairs & 4) && small_grid(L.d)));
     if (deg == 1) {
         launch_smooth_first(st, L.d, A, r64, L.b.p, cur, cb.cr[0], ws.gate);
     } else {
@@ -272,7 +276,8 @@ float *System::post_smooth(Workspace &ws, const Smoothing &sm, int l, float *cur
     // the last two steps run as one pass where the grid is large enough to
     // pay for the pass's tile overlap (the direction they leave is not stored:
     // nothing reads it before the next cycle rewrites it)
-    const int fused = (deg >= 3 && (ws.fused_pairs & 2) && pair_pays(L.d)) ? 2 : 0;
+    const int fused = (deg >= 3 && (((ws.fused_pairs & 2) && pair_pays(L.d)) ||
+                                    ((ws.fused_pairs & 8) && small_grid(L.d)))) ? 2 : 0;
     // step 0 adds the coarse correction: in the flat step's gathers, else on
     // large grids by its own pointwise pass before a plain step (cheaper than
     // the x-plane tiles' gathers at every neighbour); equal values every way

@@ -78,9 +78,10 @@ struct Workspace {
     const double *gate = nullptr;
     // run the V-cycle's small levels (<= kTailMaxCells cells) in one launch
     bool vcycle_tail = true;
-    // on large grids, run as one pass: bit 0, the last pre-smoothing step with
-    // the restriction; bit 1, the last two post-smoothing steps
-    // (the flat step and restriction beat the first pass on MI355X)
+    // run as one pass on large grids: bit 0, the last pre-smoothing step with
+    // the restriction; bit 1, the last two post-smoothing steps (the flat step
+    // and restriction beat the first pass on MI355X); bits 2 and 3: the same
+    // on small grids, where launches cost more than the bytes they move
     int fused_pairs = 2;
     // level whose x0 holds the first Chebyshev iterate of its b (stored by
     // the FCG update for the next cycle), or -1

@@ -476,10 +476,10 @@ class Context:
         self._check(lib().sfp_test_set_vcycle_tail(self._h, 1 if enable else 0))
 
     def set_fused_pairs(self, mask):
-        """Step pairs as one pass on large grids: True or 3 for both, 1 for the
-        last pre-smoothing step with the restriction only, 2 (the default) for
-        the last two post-smoothing steps only, False or 0 for neither."""
-        mask = 3 if mask is True else 0 if mask is False else int(mask)
+        """Step pairs as one pass: bit 0, the last pre-smoothing step with the
+        restriction, and bit 1, the last two post-smoothing steps, on large
+        grids; bits 2 and 3, the same on small grids. True is 15, False 0."""
+        mask = 15 if mask is True else 0 if mask is False else int(mask)
         self._check(lib().sfp_test_set_fused_pairs(self._h, mask))
 
     def smooth_restrict(self, which, level, b, x1, d1, cd, cr):

@@ -224,18 +224,18 @@ def test_vcycle_tail_matches_level_by_level(kind, n, degree, coarse):
 @pytest.mark.parametrize("kind", ["pressure", "magnetic"])
 def test_fused_pairs_match_separate_launches(kind):
     """Solves with the step pairs as one pass and as separate launches (flat
-    single steps and the restriction) give the same solution and iteration
-    count, bit for bit, on a grid large enough for the pairs."""
+    single steps and the restriction), on the large and on the small levels,
+    give the same solution and iteration count, bit for bit."""
     print(f"[case] fused pairs on/off {kind} 160^3", flush=True)
     sc = scenes.pressure_scene(160, 3) if kind == "pressure" else scenes.magnetic_scene(160, seed=3)
     with sfp.Context.for_scene(sc) as ctx:
         out = {}
-        for mask in (3, 0, 1, 2):
+        for mask in (3, 0, 1, 2, 14):
             ctx.set_fused_pairs(mask)
             ctx.reset_warm_start()
             out[mask] = ctx.pressure_project(sc) if kind == "pressure" else ctx.magnetic_solve(sc)
     a = out[3]
-    for m in (0, 1, 2):
+    for m in (0, 1, 2, 14):
         b = out[m]
         assert a[-1]["iterations"] == b[-1]["iterations"]
         for u, v in zip(a[:-1], b[:-1]):
