@@ -86,7 +86,8 @@ HOT_KERNELS = {
     "update64": ("k_fcg_update", 88.0,
                  "flexible-CG update of p, q, x, r (FP64) with r.r, the FP32 copy b of r and "
                  "the next cycle's first iterate c0 D^-1 b"),
-    "restrict": ("k_restrict_flat", 24.5, "fine-level FP32 residual summed over 2x2x2 children"),
+    "restrict": ("k_restrict_flat", 25.5, "fine-level FP32 residual summed over 2x2x2 children, "
+                 "with the coarse level's first Chebyshev iterate"),
 }
 
 

@@ -50,9 +50,13 @@ void launch_smooth_restrict(hipStream_t s, Dims d, Op32 A, const float *b, const
 // x += P e on active cells (diag32 > 0), in place.
 void launch_prolong_add(hipStream_t s, Dims d, const float *diag32, const float *ecoarse, float *x,
                         const double *gate = nullptr);
-// bc = R (b - A x): sum of the residuals of the (up to 8) children.
+// bc = R (b - A x): sum of the residuals of the (up to 8) children. With
+// x0c, also x0c = c0 Dc^-1 bc (diag32c: the coarse FP32 diagonal), the first
+// Chebyshev iterate of the coarse level's cycle, as the FCG update stores it
+// on the fine level.
 void launch_restrict(hipStream_t s, Dims df, Op32 A, const float *b, const float *x, float *bc,
-                     const double *gate = nullptr);
+                     const double *gate = nullptr, float *x0c = nullptr,
+                     const float *diag32c = nullptr, float c0 = 0.f);
 // Galerkin coarsening by 2x2x2 aggregation, scaled: writes the coarse FP64
 // and FP32 operators.
 void launch_coarsen(hipStream_t s, Dims df, Op64 Af, const float *diag32f, Dims dc,

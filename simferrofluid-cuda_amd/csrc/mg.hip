@@ -516,7 +516,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(CellGrid cg, Dims d, con
 // is even and starts on an even plane, and tz, ty are even (or tz = 1).
 __global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 A, const float *__restrict__ b,
                                                      const float *__restrict__ x, Dims dc,
-                                                     float *__restrict__ bc, const double *gate) {
+                                                     float *__restrict__ bc, float *__restrict__ x0c,
+                                                     const float *__restrict__ diag32c, float c0,
+                                                     const double *gate) {
     if (gated(gate)) return;
     __shared__ float sh[kBlock];
     int k, j, i0;
@@ -578,7 +580,11 @@ __global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 
         __syncthreads();
         acc += r;  // x pair
         if ((i & 1) == 1 || !up) {
-            if (writer) bc[dc.idx(i >> 1, j >> 1, k >> 1)] = acc;
+            if (writer) {
+                const size_t q = dc.idx(i >> 1, j >> 1, k >> 1);
+                bc[q] = acc;
+                if (x0c) x0c[q] = c0 * rcp_or_zero(diag32c[q]) * acc;
+            }
             acc = 0.f;
         }
     }
@@ -597,7 +603,9 @@ constexpr int kRestrictFlatZ = 64;  // z cells per wavefront
 
 __global__ __launch_bounds__(kBlock) void k_restrict_flat(Dims df, Op32 A, const float *__restrict__ b,
                                                           const float *__restrict__ x, Dims dc,
-                                                          float *__restrict__ bc, const double *gate) {
+                                                          float *__restrict__ bc, float *__restrict__ x0c,
+                                                          const float *__restrict__ diag32c, float c0,
+                                                          const double *gate) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     typedef float f2 __attribute__((ext_vector_type(2)));
     if (gated(gate)) return;
@@ -661,7 +669,15 @@ __global__ __launch_bounds__(kBlock) void k_restrict_flat(Dims df, Op32 A, const
         v.y = 0.f;
         v.y += s1;
         v.y += o1;
-        *(f2 *)(bc + dc.idx((int)ip, (int)jp, k >> 1)) = v;
+        const size_t q = dc.idx((int)ip, (int)jp, k >> 1);
+        *(f2 *)(bc + q) = v;
+        if (x0c) {
+            const f2 dgc = *(const f2 *)(diag32c + q);
+            f2 x0;
+            x0.x = c0 * rcp_or_zero(dgc.x) * v.x;
+            x0.y = c0 * rcp_or_zero(dgc.y) * v.y;
+            *(f2 *)(x0c + q) = x0;
+        }
     }
 }
 
@@ -1221,21 +1237,25 @@ void launch_clear_inactive(hipStream_t st, Dims d, const float *diag32, float *c
 }
 
 void launch_restrict(hipStream_t st, Dims df, Op32 A, const float *b, const float *x, float *bc,
-                     const double *gate) {
+                     const double *gate, float *x0c, const float *diag32c, float c0) {
+    if (x0c && !diag32c) throw ArgError("launch_restrict: x0c needs the coarse diagonal");
     Dims dc = df.coarse();
-    bool flat = flat_step(df) && (uintptr_t)bc % 8 == 0;
+    bool flat = flat_step(df) && (uintptr_t)bc % 8 == 0 && (uintptr_t)x0c % 8 == 0 &&
+                (uintptr_t)diag32c % 8 == 0;
     for (const float *a : {A.diag, A.cpl[0], A.cpl[1], A.cpl[2], b, x}) flat = flat && (uintptr_t)a % 16 == 0;
     if (flat) {
         const size_t waves = (size_t)((df.nz + kRestrictFlatZ - 1) / kRestrictFlatZ) *
                              ((df.ny + 1) / 2) * ((df.nx + 1) / 2);
         const unsigned nb = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_restrict_flat, dim3(nb), dim3(kBlock), 0, st, df, A, b, x, dc, bc, gate);
+        hipLaunchKernelGGL(k_restrict_flat, dim3(nb), dim3(kBlock), 0, st, df, A, b, x, dc, bc, x0c,
+                           diag32c, c0, gate);
         SFP_LAUNCH_CHECK();
         return;
     }
     Launch L = cell_launch(df, 0, 8);
     if (L.cg.planes & 1) L = cell_launch(df, L.cg.planes + 1);  // plane pairs stay in one block
-    hipLaunchKernelGGL(k_restrict, L.grid, L.block, 0, st, L.cg, df, A, b, x, dc, bc, gate);
+    hipLaunchKernelGGL(k_restrict, L.grid, L.block, 0, st, L.cg, df, A, b, x, dc, bc, x0c, diag32c, c0,
+                       gate);
     SFP_LAUNCH_CHECK();
 }
 

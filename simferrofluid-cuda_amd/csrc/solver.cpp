@@ -210,7 +210,9 @@ static double step_bytes(int mode) {
     return b;
 }
 
-float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r64) {
+// x0_ready: the level's x0 holds c0 D^-1 b, stored by the restriction that
+// formed its b (coarse levels) or by the FCG kernels (ws.x0_level).
+float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r64, bool x0_ready) {
     if (!r64) {
         if (float *z = vcycle_tail(ws, sm, l)) return z;
     }
@@ -244,11 +246,12 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
     } else {
         // steps 0 and 1 in one pass (step 0 read from x0 when the FCG update
         // stored it); then steps 2 .. deg-1
-        const int first = (!r64 && ws.x0_level == l) ? SM_XFIRST : SM_FIRST;
+        const int first = (!r64 && (x0_ready || ws.x0_level == l)) ? SM_XFIRST : SM_FIRST;
         smooth(first | (deg > 2 ? SM_WRITE_D : 0), cb.cd[1], cb.cr[1], r64, nullptr);
         for (int s = 2; s < deg - (fuse_restrict ? 1 : 0); s++)
             smooth(SM_READ_D | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s], nullptr, nullptr);
     }
+    bool store_x0 = false;  // the restriction stores the coarse level's first iterate
     if (fuse_restrict) {
         // reads x, b, d, diag, 3 couplings; writes x and the coarse b
         if (fine && ws.profiling) ws.timers[K_SMOOTH_RESTRICT].begin(st, (32.0 + 0.5) * cells);
@@ -257,11 +260,17 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
         if (fine && ws.profiling) ws.timers[K_SMOOTH_RESTRICT].end(st);
         std::swap(cur, oth);
     } else {
-        if (fine && ws.profiling) ws.timers[K_RESTRICT].begin(st, (24.0 + 0.5) * cells);
-        launch_restrict(st, L.d, A, L.b.p, cur, lv_[l + 1].b.p, ws.gate);
+        // the coarse level's first iterate too, where its cycle's first step reads it
+        Level &C = lv_[l + 1];
+        store_x0 = l + 2 < (int)lv_.size() && sm.at(l + 1).degree >= 2;
+        // reads x, b, diag, 3 couplings; writes the coarse b (and reads the
+        // coarse diagonal, writes the coarse x0): per fine cell
+        if (fine && ws.profiling) ws.timers[K_RESTRICT].begin(st, (24.0 + 0.5 + (store_x0 ? 1.0 : 0.0)) * cells);
+        launch_restrict(st, L.d, A, L.b.p, cur, C.b.p, ws.gate, store_x0 ? C.x0.p : nullptr,
+                        C.diag32.p, sm.at(l + 1).cr[0]);
         if (fine && ws.profiling) ws.timers[K_RESTRICT].end(st);
     }
-    const float *e = vlevel(ws, sm, l + 1, nullptr);
+    const float *e = vlevel(ws, sm, l + 1, nullptr, store_x0);
     return post_smooth(ws, sm, l, cur, e);
 }
 

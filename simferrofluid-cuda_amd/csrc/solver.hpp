@@ -146,7 +146,7 @@ class System {
     float *learned_cycle(Workspace &ws, const SolverConfig &cfg, const double *r64, Cycle c);
     float *post_smooth(Workspace &ws, const Smoothing &sm, int l, float *cur, const float *e);
     SolveInfo fcg(Workspace &ws, const SolverConfig &cfg, int level, bool warm);
-    float *vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r64);
+    float *vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r64, bool x0_ready = false);
     // levels l .. coarsest in one launch (vlevel with r64 null), or null
     // where the tail does not apply
     float *vcycle_tail(Workspace &ws, const Smoothing &sm, int l);
