@@ -99,8 +99,11 @@ __global__ __launch_bounds__(kBlock) void k_pressure_assemble(CellGrid cg,
             diag32[c] = 0.f;
         } else {
             const int cell[3] = {i, j, k};
-            double diag = 0;
-            double div = 0;
+            // every operand of the six faces is loaded before any is used:
+            // face by face, each face's loads would wait on the one before
+            double fr[6], pn[6], uv[6], usv[6], jv[6];
+            bool nbv[6];
+#pragma unroll
             for (int ord = 0; ord < 6; ord++) {
                 const int axis = ord >> 1, side = (ord & 1) ? 1 : -1;
                 int nb[3] = {cell[0], cell[1], cell[2]};
@@ -108,25 +111,36 @@ __global__ __launch_bounds__(kBlock) void k_pressure_assemble(CellGrid cg,
                 int face[3] = {cell[0], cell[1], cell[2]};
                 face[axis] += (ord & 1);
                 const size_t fidx = d.face(axis).idx(face[0], face[1], face[2]);
-                const double weight = 1 - frac.a[axis][fidx];
+                nbv[ord] = valid(d, nb);
+                fr[ord] = frac.a[axis][fidx];
+                pn[ord] = nbv[ord] ? phi[d.idx(nb[0], nb[1], nb[2])] : 0.0;
+                uv[ord] = vel.a[axis][fidx];
+                usv[ord] = us.a[axis][fidx];
+                jv[ord] = has_jump ? jump.a[axis][fidx] : 0.0;
+            }
+            double diag = 0;
+            double div = 0;
+#pragma unroll
+            for (int ord = 0; ord < 6; ord++) {
+                const int axis = ord >> 1, side = (ord & 1) ? 1 : -1;
+                const double weight = 1 - fr[ord];
                 if (weight > 0) {
-                    if (!valid(d, nb)) {
+                    if (!nbv[ord]) {
                         atomicExch(err, 1);
                     } else {
-                        const double pn = phi[d.idx(nb[0], nb[1], nb[2])];
-                        if (pn <= 0) {
+                        if (pn[ord] <= 0) {
                             diag += weight;
                             if (side < 0) low[axis] = weight;
                         } else {
-                            const double theta = pc / (pc - pn);
+                            const double theta = pc / (pc - pn[ord]);
                             const double ic = 1. / max_001(theta);
                             diag += weight * ic;
-                            if (has_jump) div += jump.a[axis][fidx] * weight * ic;
+                            if (has_jump) div += jv[ord] * weight * ic;
                         }
                     }
-                    div -= (double)side * weight * vel.a[axis][fidx];
+                    div -= (double)side * weight * uv[ord];
                 }
-                if (weight < 1) div -= (double)side * (1 - weight) * us.a[axis][fidx];
+                if (weight < 1) div -= (double)side * (1 - weight) * usv[ord];
             }
             div += vol_error;
             rhs[c] = div;
@@ -256,21 +270,34 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_assemble(CellGrid cg, Dims 
             diag32[c] = 0.f;
         } else {
             const int cell[3] = {i, j, k};
-            double diag = 0.;
-            double b = 0;
+            // the six faces' operands are loaded before any is used (every
+            // face, boundary faces included, has an entry in the face grids)
+            double mv[6], hv[6];
+            bool nbv[6], pin[6];
+#pragma unroll
             for (int ord = 0; ord < 6; ord++) {
                 const int axis = ord >> 1, side = (ord & 1) ? 1 : -1;
                 int nb[3] = {cell[0], cell[1], cell[2]};
                 nb[axis] += side;
-                if (!valid(d, nb)) continue;
                 int face[3] = {cell[0], cell[1], cell[2]};
                 face[axis] += (ord & 1);
                 const size_t fidx = d.face(axis).idx(face[0], face[1], face[2]);
-                const double weight = mu.a[axis][fidx];
+                nbv[ord] = valid(d, nb);
+                pin[ord] = nb[0] == 0 && nb[1] == 0 && nb[2] == 0;
+                mv[ord] = mu.a[axis][fidx];
+                hv[ord] = hext.a[axis][fidx];
+            }
+            double diag = 0.;
+            double b = 0;
+#pragma unroll
+            for (int ord = 0; ord < 6; ord++) {
+                const int axis = ord >> 1, side = (ord & 1) ? 1 : -1;
+                if (!nbv[ord]) continue;
+                const double weight = mv[ord];
                 // column 0 is dropped from every other row (pin by elimination)
-                if (side < 0 && d.idx(nb[0], nb[1], nb[2]) != 0) low[axis] = weight;
+                if (side < 0 && !pin[ord]) low[axis] = weight;
                 diag += weight;
-                b += (weight - 1.) * hext.a[axis][fidx] * (double)(-side) * spacing;
+                b += (weight - 1.) * hv[ord] * (double)(-side) * spacing;
             }
             diag64[c] = diag;
             rhs[c] = b;
