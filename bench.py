@@ -69,9 +69,11 @@ def parse():
 # rocprofv3 symbol, algorithmic bytes per cell (each distinct array once per
 # launch, as the library counts them) and what the kernel does.
 HOT_KERNELS = {
-    "smooth_pair": ("k_smooth_pair<false>", 32.0,
-                    "fine-level last two post-smoothing steps in one pass, FP32 7-point stencil: "
-                    "reads x, b, d, diag, 3 couplings, writes x"),
+    "smooth_pair": ("k_smooth_pair<false>", 30.0,
+                    "fine-level pairs of Chebyshev steps in one pass, FP32 7-point stencil: the "
+                    "first two pre-smoothing steps (reads the stored first iterate, b, diag, 3 "
+                    "couplings, writes x: 28 B/cell) and the last two post-smoothing steps (reads "
+                    "x, b, d, diag, 3 couplings, writes x: 32 B/cell)"),
     "smooth_restrict": ("k_smooth_pair<true>", 32.5,
                         "fine-level last pre-smoothing step and the residual restriction in one "
                         "pass, FP32: reads x, b, d, diag, 3 couplings, writes x and the coarse b"),

@@ -17,7 +17,7 @@ typedef enum sfp_kernel_id {
     SFP_KERNEL_MATVEC64 = 1, /* flexible-CG matvec fused with its inner products, FP64 */
     SFP_KERNEL_UPDATE64 = 2, /* flexible-CG vector update, FP64 */
     SFP_KERNEL_RESTRICT = 3, /* fine-level residual + restriction, FP32 */
-    SFP_KERNEL_SMOOTH_PAIR = 4, /* fine-level last two post-smoothing steps in one pass, FP32 */
+    SFP_KERNEL_SMOOTH_PAIR = 4, /* fine-level pairs of Chebyshev steps in one pass, FP32 */
     SFP_KERNEL_SMOOTH_RESTRICT = 5 /* fine-level last pre-smoothing step + restriction, FP32 */
 } sfp_kernel_id;
 

@@ -63,9 +63,10 @@ sfp_status sfp_test_smooth_restrict(sfp_context ctx, int which, int level, const
 sfp_status sfp_test_set_vcycle_tail(sfp_context ctx, int enable);
 /* Run as one pass (mask bit 0) the last pre-smoothing step with the
  * restriction and (bit 1) the last two post-smoothing steps on large grids,
- * (bits 2 and 3) the same on small grids, or each as separate launches.
- * Results are the same bit for bit. */
-sfp_status sfp_test_set_fused_pairs(sfp_context ctx, int mask);
+ * (bits 2 and 3) the same on small grids, (bit 4) the first two
+ * pre-smoothing passes of a degree-3 cycle on large grids, or each as
+ * separate launches. Results are the same bit for bit. */
+sfp_status sfp_test_set_fused_pairs(sfp_context ctx, int mask); /* default 18 */
 
 #ifdef __cplusplus
 }

@@ -301,7 +301,7 @@ sfp_status sfp_test_set_vcycle_tail(sfp_context ctx, int enable) {
 
 sfp_status sfp_test_set_fused_pairs(sfp_context ctx, int mask) {
     return run(ctx, [&] {
-        if (mask < 0 || mask > 15) throw ArgError("fused pairs: mask is 0..15");
+        if (mask < 0 || mask > 31) throw ArgError("fused pairs: mask is 0..31");
         ctx->ws.fused_pairs = mask;
     });
 }

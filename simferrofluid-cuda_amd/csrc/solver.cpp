@@ -247,9 +247,22 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
         // steps 0 and 1 in one pass (step 0 read from x0 when the FCG update
         // stored it); then steps 2 .. deg-1
         const int first = (!r64 && (x0_ready || ws.x0_level == l)) ? SM_XFIRST : SM_FIRST;
-        smooth(first | (deg > 2 ? SM_WRITE_D : 0), cb.cd[1], cb.cr[1], r64, nullptr);
-        for (int s = 2; s < deg - (fuse_restrict ? 1 : 0); s++)
-            smooth(SM_READ_D | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s], nullptr, nullptr);
+        // at degree 3 the two passes from the stored first iterate can run as
+        // one: the iterate is also step 1's previous direction
+        const bool pre_pair = deg == 3 && !fuse_restrict && first == SM_XFIRST &&
+                              (ws.fused_pairs & 16) && pair_pays(L.d);
+        if (pre_pair) {
+            // reads x0, b, diag, 3 couplings; writes x
+            if (fine && ws.profiling) ws.timers[K_SMOOTH_PAIR].begin(st, 28.0 * cells);
+            launch_smooth_pair(st, L.d, A, L.b.p, cur, oth, cur, cb.cd[1], cb.cr[1], cb.cd[2], cb.cr[2],
+                               ws.gate);
+            if (fine && ws.profiling) ws.timers[K_SMOOTH_PAIR].end(st);
+            std::swap(cur, oth);
+        } else {
+            smooth(first | (deg > 2 ? SM_WRITE_D : 0), cb.cd[1], cb.cr[1], r64, nullptr);
+            for (int s = 2; s < deg - (fuse_restrict ? 1 : 0); s++)
+                smooth(SM_READ_D | (s < deg - 1 ? SM_WRITE_D : 0), cb.cd[s], cb.cr[s], nullptr, nullptr);
+        }
     }
     bool store_x0 = false;  // the restriction stores the coarse level's first iterate
     if (fuse_restrict) {

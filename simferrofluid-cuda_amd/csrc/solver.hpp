@@ -81,8 +81,10 @@ struct Workspace {
     // run as one pass on large grids: bit 0, the last pre-smoothing step with
     // the restriction; bit 1, the last two post-smoothing steps (the flat step
     // and restriction beat the first pass on MI355X); bits 2 and 3: the same
-    // on small grids, where launches cost more than the bytes they move
-    int fused_pairs = 2;
+    // on small grids, where launches cost more than the bytes they move; bit
+    // 4: the first two pre-smoothing passes (degree 3, from the stored
+    // first iterate) on large grids
+    int fused_pairs = 2 | 16;
     // level whose x0 holds the first Chebyshev iterate of its b (stored by
     // the FCG update for the next cycle), or -1
     int x0_level = -1;

@@ -478,8 +478,10 @@ class Context:
     def set_fused_pairs(self, mask):
         """Step pairs as one pass: bit 0, the last pre-smoothing step with the
         restriction, and bit 1, the last two post-smoothing steps, on large
-        grids; bits 2 and 3, the same on small grids. True is 15, False 0."""
-        mask = 15 if mask is True else 0 if mask is False else int(mask)
+        grids; bits 2 and 3, the same on small grids; bit 4, the first two
+        pre-smoothing passes of a degree-3 cycle on large grids. The default
+        is 18 (bits 1 and 4); True is 31, False 0."""
+        mask = 31 if mask is True else 0 if mask is False else int(mask)
         self._check(lib().sfp_test_set_fused_pairs(self._h, mask))
 
     def smooth_restrict(self, which, level, b, x1, d1, cd, cr):

@@ -230,12 +230,12 @@ def test_fused_pairs_match_separate_launches(kind):
     sc = scenes.pressure_scene(160, 3) if kind == "pressure" else scenes.magnetic_scene(160, seed=3)
     with sfp.Context.for_scene(sc) as ctx:
         out = {}
-        for mask in (3, 0, 1, 2, 14):
+        for mask in (3, 0, 1, 2, 14, 18, 31):
             ctx.set_fused_pairs(mask)
             ctx.reset_warm_start()
             out[mask] = ctx.pressure_project(sc) if kind == "pressure" else ctx.magnetic_solve(sc)
     a = out[3]
-    for m in (0, 1, 2, 14):
+    for m in (0, 1, 2, 14, 18, 31):
         b = out[m]
         assert a[-1]["iterations"] == b[-1]["iterations"]
         for u, v in zip(a[:-1], b[:-1]):
