@@ -327,7 +327,17 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
     __shared__ float sx2[3][kPairRows][64];
     __shared__ float scy[3][kPairRows][64];  // cpl_y of the same three planes
     __shared__ float sres[kRestrict ? kPairRows : 1][64];
-    const unsigned L = blockIdx.x, per = cg.gx * cg.gy;
+    // Blocks go to the 8 XCDs in turn (block b to XCD b % 8); renumbered so
+    // that each XCD takes a contiguous run of tiles, the rows, planes and
+    // cache lines that neighbouring tiles share are read into one L2 (the
+    // tiles overlap, and their z extents are not line-aligned). A tail of
+    // fewer than 8 blocks keeps its numbers.
+    unsigned L = blockIdx.x;
+    {
+        const unsigned q = gridDim.x / 8;
+        if (L < 8 * q) L = (L % 8) * q + L / 8;
+    }
+    const unsigned per = cg.gx * cg.gy;
     const unsigned bx = L / per, rem = L - bx * per;
     const unsigned by = rem / cg.gx, bz = rem - by * cg.gx;
     const int lane = threadIdx.x, row = threadIdx.y;
