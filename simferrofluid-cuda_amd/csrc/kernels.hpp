@@ -16,7 +16,8 @@ enum SmoothMode : int {
     SM_WRITE_D = 4,    // store the new direction
     SM_PROLONG = 8,    // input iterate is x + P e (coarse correction fused in)
     SM_XFIRST = 16     // input iterate is the zero-start first step c0 D^{-1} b, stored
-                       // in xin (by the FCG update); the previous direction equals it
+                       // in xin (by the FCG kernels on the fine level, the restriction on
+                       // the coarse ones); the previous direction equals it
 };
 
 // x = c0 * D^{-1} b from a zero start; at the fine level b is first rounded
@@ -31,6 +32,8 @@ void launch_smooth(hipStream_t s, int mode, Dims d, Op32 A, const double *r64, f
 // Two steps in one pass: x2 = x1 + d2, d2 = cdA d1 + crA D^{-1} (b - A x1), then
 // x3 = x2 + cdB d2 + crB D^{-1} (b - A x2); d2 is not stored. x3 must differ
 // from x1. Same results as the two launch_smooth calls, bit for bit.
+// With x1 == d1 (the stored zero-start iterate, which is also the previous
+// direction) the pair is the first two steps of a cycle.
 // pair_pays: grids on which it beats the two launches on MI355X (its 64 x 16
 // tiles overlap by one cell per side; measured equal at 128^3, faster above).
 inline bool pair_pays(Dims d) { return d.nz >= 64 && d.cells() > ((size_t)1 << 21); }
