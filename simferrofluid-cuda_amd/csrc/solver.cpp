@@ -440,6 +440,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     bool learned = fine && cfg.precond == 1 && learned_ok(ws);
     if (learned) prepare_learned(ws);
     bool b_ready = b32 != nullptr;  // lv_[level].b holds r rounded (and lv_[level].x0 its first iterate)
+    int next_batch = kBatch;        // iterations to enqueue before the host reads the residual
     info.precond_used = learned ? 1 : (cfg.precond == 1 ? 0 : cfg.precond);
     while (true) {
         if (rel <= cfg.tol) {
@@ -457,11 +458,13 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
         }
         if (info.iterations >= cfg.max_iters) break;
         // Up to kBatch iterations are enqueued before the host looks at the
-        // residual: the update's finalize raises `stop` on the device once
-        // ||r|| <= tol ||b||, and every kernel of the remaining iterations of
-        // the batch then returns at once. The learned path (its stall test
-        // needs every residual) and profiled solves run one at a time.
-        const int batch = (learned || profiling) ? 1 : std::min(kBatch, cfg.max_iters - info.iterations);
+        // residual (after the first batch, as many as the residual's rate of
+        // decrease predicts are left): the update's finalize raises `stop` on
+        // the device once ||r|| <= tol ||b||, and every kernel of the
+        // remaining iterations of the batch then returns at once. The learned
+        // path (its stall test needs every residual) and profiled solves run
+        // one at a time.
+        const int batch = (learned || profiling) ? 1 : std::min(next_batch, cfg.max_iters - info.iterations);
         if (batch > 1) {
             ws.host_sc->stop = 0.0;
             ws.host_sc->iters = 0.0;
@@ -502,6 +505,15 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
         const double prev = rel;
         rel = std::sqrt(ws.host_sc->rr / bb);
         checked = false;
+        // the next batch: as many iterations as the last ones' mean residual
+        // reduction predicts are left (iterations enqueued past convergence
+        // still launch every kernel, which returns at once)
+        next_batch = kBatch;
+        if (done > 0 && rel > 0.0 && rel < prev) {
+            const double rate = std::pow(rel / prev, 1.0 / done);
+            const double left = std::log(cfg.tol / rel) / std::log(rate);
+            if (left < kBatch) next_batch = std::max(1, (int)std::ceil(left));
+        }
         if (learned && rel > cfg.stall_ratio * prev) learned = false;  // fall back to multigrid
     }
     if (!checked) {
