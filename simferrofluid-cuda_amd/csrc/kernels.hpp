@@ -111,6 +111,9 @@ void launch_coarse_solve(hipStream_t s, int n, const float *ainv, const float *b
 constexpr int kReducePlanes = 0;  // automatic
 // Doubles of partial-sum scratch a reduction over grid d needs.
 size_t reduction_partials(Dims d);
+// Before a batch of gated iterations: scalars->stop = 0, ->iters = 0,
+// ->stop_rr = stop_rr.
+void launch_batch_begin(hipStream_t s, KrylovScalars *sc, double stop_rr);
 // r = b - A x (x may be null = 0); scalars->rr = r.r, scalars->bb = b.b.
 // With b32, also b32 = r rounded to FP32 and, with x0, x0 = c0 D32^-1 b32
 // (diag32: the FP32 diagonal): the right-hand side and first Chebyshev

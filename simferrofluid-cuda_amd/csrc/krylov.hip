@@ -400,6 +400,16 @@ __global__ __launch_bounds__(kBlock) void k_count_active(CellGrid cg, Dims d,
 // sums of the largest grids take few rounds of loads).
 constexpr int kFinalizeThreads = 1024;
 
+// Start of a batch of gated iterations: clear the stop flag and the iteration
+// count, set the threshold on ||r||^2 (one thread; no host copy needed).
+__global__ __launch_bounds__(64) void k_batch_begin(KrylovScalars *sc, double stop_rr) {
+    if (threadIdx.x == 0) {
+        sc->stop = 0.0;
+        sc->iters = 0.0;
+        sc->stop_rr = stop_rr;
+    }
+}
+
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int op, unsigned nb, int first,
                                                      const double *__restrict__ partials,
                                                      KrylovScalars *sc, const double *gate) {
@@ -479,6 +489,11 @@ void launch_count_active(hipStream_t st, Dims d, const float *diag32, double *pa
     hipLaunchKernelGGL(k_count_active, L.grid, L.block, 0, st, L.cg, d, diag32, partials);
     SFP_LAUNCH_CHECK();
     finalize(st, FIN_COUNT, L.blocks(), 0, partials, sc);
+}
+
+void launch_batch_begin(hipStream_t st, KrylovScalars *sc, double stop_rr) {
+    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, sc, stop_rr);
+    SFP_LAUNCH_CHECK();
 }
 
 void launch_residual64(hipStream_t st, Dims d, Op64 A, const double *b, const double *x,

@@ -78,10 +78,6 @@ void Workspace::read_scalars() {
     SFP_HIP(hipStreamSynchronize(stream));
 }
 
-void Workspace::write_scalars() {
-    SFP_HIP(hipMemcpyAsync(scalars.p, host_sc, sizeof(KrylovScalars), hipMemcpyHostToDevice, stream));
-}
-
 // ---------------------------------------------------------------- System
 // The stencil kernels skip wavefront rows whose cells are all inactive; the
 // skipped cells of the level vectors then keep the zeros written here.
@@ -466,10 +462,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
         // one at a time.
         const int batch = (learned || profiling) ? 1 : std::min(next_batch, cfg.max_iters - info.iterations);
         if (batch > 1) {
-            ws.host_sc->stop = 0.0;
-            ws.host_sc->iters = 0.0;
-            ws.host_sc->stop_rr = cfg.tol * cfg.tol * bb;
-            ws.write_scalars();
+            launch_batch_begin(st, ws.scalars.p, cfg.tol * cfg.tol * bb);
             ws.gate = &ws.scalars.p->stop;
         }
         for (int it = 0; it < batch; it++) {

@@ -89,7 +89,6 @@ struct Workspace {
     // the FCG update for the next cycle), or -1
     int x0_level = -1;
     void read_scalars();               // synchronous copy of *scalars to host_sc
-    void write_scalars();              // host_sc to *scalars (stream-ordered)
 };
 
 struct SolveInfo {
