@@ -85,9 +85,10 @@ HOT_KERNELS = {
     "matvec64": ("k_fcg_matvec_flat", 60.0,
                  "flexible-CG w = A z with the FP64 operator and its 3 inner products "
                  "(two cells of a z row per lane)"),
-    "update64": ("k_fcg_update", 88.0,
-                 "flexible-CG update of p, q, x, r (FP64) with r.r, the FP32 copy b of r and "
-                 "the next cycle's first iterate c0 D^-1 b"),
+    "update64": ("k_fcg_update", 80.0,
+                 "flexible-CG update of p, q, r (FP64) with r.r, the FP32 copy b of r and the next "
+                 "cycle's first iterate c0 D^-1 b; x read and written every other iteration "
+                 "(88 B then 72 B per cell)"),
     "restrict": ("k_restrict_flat", 25.5, "fine-level FP32 residual summed over 2x2x2 children, "
                  "with the coarse level's first Chebyshev iterate"),
 }

@@ -127,14 +127,23 @@ void launch_residual64(hipStream_t s, Dims d, Op64 A, const double *b, const dou
 void launch_fcg_matvec(hipStream_t s, Dims d, Op64 A, const float *active, const float *z,
                        const double *r, const double *qprev, double *w, int first,
                        double *partials, KrylovScalars *sc, const double *gate = nullptr);
-// p = z - beta p; q = w - beta q; x += alpha p; r -= alpha q; rr = r.r
+// The update of x in an FCG iteration: now (x += alpha p), later (left for
+// the next iteration or launch_fcg_flush), or both the previous iteration's
+// and this one's (x += alpha_prev p_prev + alpha p), so x is read and written
+// every other iteration.
+enum XUpdate : int { X_NOW = 0, X_LATER = 1, X_BOTH = 2 };
+// p = z - beta p; q = w - beta q; x per xmode; r -= alpha q; rr = r.r
 // (all-inactive wavefronts skipped when `active` is given); b32 (or null)
 // receives r rounded to FP32 and x0 (or null) the first Chebyshev iterate
 // c0 D^{-1} b32, with D the FP32 diagonal `active`.
 void launch_fcg_update(hipStream_t s, Dims d, const float *active, const float *z,
                        const double *w, double *p, double *q, double *x, double *r, float *b32,
-                       float *x0, float c0, int first, double *partials, KrylovScalars *sc,
-                       const double *gate = nullptr);
+                       float *x0, float c0, int first, int xmode, double *partials,
+                       KrylovScalars *sc, const double *gate = nullptr);
+// x += alpha p with the last iteration's alpha and direction (after an
+// X_LATER iteration), over the update's groups of cells.
+void launch_fcg_flush(hipStream_t s, Dims d, const float *active, const double *p, double *x,
+                      const KrylovScalars *sc);
 // z = D^{-1} r on active cells (Jacobi preconditioner).
 void launch_jacobi(hipStream_t s, Dims d, Op32 A, const double *r, float *z,
                    const double *gate = nullptr);

@@ -295,12 +295,12 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__
                                                        double *__restrict__ x,
                                                        double *__restrict__ r,
                                                        float *__restrict__ b32,
-                                                       float *x0, float c0, int first,
+                                                       float *x0, float c0, int first, int xmode,
                                                        double *__restrict__ partials,
                                                        const KrylovScalars *__restrict__ sc,
                                                        const double *gate) {
     if (gated(gate)) return;
-    const double alpha = sc->alpha, beta = sc->beta;
+    const double alpha = sc->alpha, beta = sc->beta, alpha_prev = sc->alpha_prev;
     double v[1] = {0.0};
     const size_t c = (size_t)blockIdx.x * kUpdateCells + 2 * (size_t)flat_tid();
     const bool pair = c + 1 < n;  // else at most the grid's last cell
@@ -316,29 +316,35 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__
             const float2v zf = *(const float2v *)(z + c);
             const double2v zc = {(double)zf.x, (double)zf.y};
             const double2v wc = ld_stream((const double2v *)(w + c));
-            const double2v xc = ld_stream((const double2v *)(x + c));
             const double2v r0 = ld_stream((const double2v *)(r + c));
-            double2v pc, qc;
+            double2v pc, qc, pp = {0.0, 0.0};
             if (first) {
                 pc = zc;
                 qc = wc;
             } else {
-                const double2v pp = ld_stream((const double2v *)(p + c));
+                pp = ld_stream((const double2v *)(p + c));
                 const double2v qq = ld_stream((const double2v *)(q + c));
                 pc.x = zc.x - beta * pp.x;
                 pc.y = zc.y - beta * pp.y;
                 qc.x = wc.x - beta * qq.x;
                 qc.y = wc.y - beta * qq.y;
             }
-            double2v xn, rc;
-            xn.x = xc.x + alpha * pc.x;
-            xn.y = xc.y + alpha * pc.y;
+            double2v rc;
             rc.x = r0.x - alpha * qc.x;
             rc.y = r0.y - alpha * qc.y;
             st_stream((double2v *)(p + c), pc);
             st_stream((double2v *)(q + c), qc);
-            st_stream((double2v *)(x + c), xn);
             st_stream((double2v *)(r + c), rc);
+            if (xmode != X_LATER) {
+                double2v xn = ld_stream((const double2v *)(x + c));
+                if (xmode == X_BOTH) {
+                    xn.x += alpha_prev * pp.x;
+                    xn.y += alpha_prev * pp.y;
+                }
+                xn.x += alpha * pc.x;
+                xn.y += alpha * pc.y;
+                st_stream((double2v *)(x + c), xn);
+            }
             if (b32) {
                 const float2v bf = {(float)rc.x, (float)rc.y};
                 *(float2v *)(b32 + c) = bf;
@@ -351,11 +357,17 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__
             v[0] += rc.y * rc.y;
         } else if (c < n) {
             const double zc = (double)z[c];
-            const double pc = first ? zc : zc - beta * p[c];
+            const double pp = first ? 0.0 : p[c];
+            const double pc = first ? zc : zc - beta * pp;
             const double qc = first ? w[c] : w[c] - beta * q[c];
             p[c] = pc;
             q[c] = qc;
-            x[c] += alpha * pc;
+            if (xmode != X_LATER) {
+                double xn = x[c];
+                if (xmode == X_BOTH) xn += alpha_prev * pp;
+                xn += alpha * pc;
+                x[c] = xn;
+            }
             const double rc = r[c] - alpha * qc;
             r[c] = rc;
             if (b32) {
@@ -366,6 +378,34 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(size_t n, const float *__
         }
     }
     block_partials<1>(v, partials);
+}
+
+// x += alpha p (the update an X_LATER iteration left pending), over the
+// update's groups of cells.
+__global__ __launch_bounds__(kBlock) void k_fcg_flush(size_t n, const float *__restrict__ active,
+                                                      const double *__restrict__ p,
+                                                      double *__restrict__ x,
+                                                      const KrylovScalars *__restrict__ sc) {
+    const double alpha = sc->alpha;
+    const size_t c = (size_t)blockIdx.x * kUpdateCells + 2 * (size_t)flat_tid();
+    const bool pair = c + 1 < n;
+    bool act = c < n;
+    if (active) {
+        float2v dg = {0.f, 0.f};
+        if (pair) dg = *(const float2v *)(active + c);
+        else if (c < n) dg.x = active[c];
+        act = dg.x > 0.f || dg.y > 0.f;
+    }
+    if (!__any(act)) return;
+    if (pair) {
+        const double2v pc = ld_stream((const double2v *)(p + c));
+        double2v xn = ld_stream((const double2v *)(x + c));
+        xn.x += alpha * pc.x;
+        xn.y += alpha * pc.y;
+        st_stream((double2v *)(x + c), xn);
+    } else if (c < n) {
+        x[c] += alpha * p[c];
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_jacobi(CellGrid cg, Dims d, Op32 A,
@@ -446,6 +486,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int op, unsigned 
         sc->zr = zr;
         sc->beta = beta;
         sc->pq = pq;
+        sc->alpha_prev = sc->alpha;
         sc->alpha = pq != 0.0 ? zr / pq : 0.0;
     } else if (op == FIN_UPDATE) {
         sc->rr = v[0];
@@ -533,8 +574,9 @@ void launch_fcg_matvec(hipStream_t st, Dims d, Op64 A, const float *active, cons
 
 void launch_fcg_update(hipStream_t st, Dims d, const float *active, const float *z,
                        const double *w, double *p, double *q, double *x, double *r, float *b32,
-                       float *x0, float c0, int first, double *partials, KrylovScalars *sc,
-                       const double *gate) {
+                       float *x0, float c0, int first, int xmode, double *partials,
+                       KrylovScalars *sc, const double *gate) {
+    if (xmode == X_BOTH && first) throw ArgError("launch_fcg_update: X_BOTH needs a previous direction");
     // two cells per lane: 16-byte (8-byte for z) aligned vectors
     for (const void *a : {(const void *)w, (const void *)p, (const void *)q, (const void *)x, (const void *)r})
         if ((uintptr_t)a % 16) throw ArgError("launch_fcg_update: FP64 vector not 16-byte aligned");
@@ -543,9 +585,16 @@ void launch_fcg_update(hipStream_t st, Dims d, const float *active, const float 
     if (x0 && (!b32 || !active)) throw ArgError("launch_fcg_update: x0 needs b32 and the diagonal");
     const unsigned nb = update_blocks(d);
     hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(kBlock), 0, st, d.cells(), active, z, w, p, q,
-                       x, r, b32, x0, c0, first, partials, sc, gate);
+                       x, r, b32, x0, c0, first, xmode, partials, sc, gate);
     SFP_LAUNCH_CHECK();
     finalize(st, FIN_UPDATE, nb, first, partials, sc, gate);
+}
+
+void launch_fcg_flush(hipStream_t st, Dims d, const float *active, const double *p, double *x,
+                      const KrylovScalars *sc) {
+    const unsigned nb = update_blocks(d);
+    hipLaunchKernelGGL(k_fcg_flush, dim3(nb), dim3(kBlock), 0, st, d.cells(), active, p, x, sc);
+    SFP_LAUNCH_CHECK();
 }
 
 void launch_jacobi(hipStream_t st, Dims d, Op32 A, const double *r, float *z, const double *gate) {

@@ -132,6 +132,7 @@ struct KrylovScalars {
     // update, and every gated kernel of later iterations then returns at once;
     // `iters` counts the updates done
     double stop, iters, stop_rr;
+    double alpha_prev;  // the previous iteration's alpha (lagged solution update)
 };
 
 // Per-kernel timing collected with HIP events on the context stream.

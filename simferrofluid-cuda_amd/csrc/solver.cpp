@@ -437,12 +437,21 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     if (learned) prepare_learned(ws);
     bool b_ready = b32 != nullptr;  // lv_[level].b holds r rounded (and lv_[level].x0 its first iterate)
     int next_batch = kBatch;        // iterations to enqueue before the host reads the residual
+    // x is updated every other iteration (X_LATER, then X_BOTH): `run` counts
+    // the updates since it was last current; an odd count leaves alpha p of
+    // the last one pending, added before anything reads x
+    int run = 0;
+    auto bring_x_up_to_date = [&]() {
+        if (run % 2) launch_fcg_flush(st, d, lv_[level].diag32.p, p_.p, x_.p, ws.scalars.p);
+        run = 0;
+    };
     info.precond_used = learned ? 1 : (cfg.precond == 1 ? 0 : cfg.precond);
     while (true) {
         if (rel <= cfg.tol) {
             if (checked) break;
             // recursive residual reached tol: recompute it from x in FP64 and
             // restart from the true residual if the two have drifted apart
+            bring_x_up_to_date();
             launch_residual64(st, d, A, rhs_.p, x_.p, r_.p, ws.partials.p,
                               ws.scalars.p, b32, x0, lv_[level].diag32.p, c0);
             ws.read_scalars();
@@ -479,12 +488,15 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
             launch_fcg_matvec(st, d, A, lv_[level].diag32.p, z, r_.p, q_.p, w_.p, first ? 1 : 0,
                               ws.partials.p, ws.scalars.p, ws.gate);
             if (profiling) ws.timers[K_MATVEC64].end(st);
-            // z, w, p, q, x, r in; p, q, x, r out (and diag in, b and x0 out)
+            // z, w, p, q, r in; p, q, r out; x in and out every other
+            // iteration (and diag in, b and x0 out)
+            const int xmode = (run + it) % 2 == 0 ? X_LATER : X_BOTH;
             if (profiling)
-                ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 5 + 8.0 * 4 + (b32 ? 4.0 + 4.0 + 4.0 : 0.0)) * cells);
+                ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 4 + 8.0 * 3 + (xmode == X_LATER ? 0.0 : 16.0) +
+                                                 (b32 ? 4.0 + 4.0 + 4.0 : 0.0)) * cells);
             ws.x0_level = -1;
             launch_fcg_update(st, d, lv_[level].diag32.p, z, w_.p, p_.p, q_.p, x_.p, r_.p, b32,
-                              b32 ? lv_[level].x0.p : nullptr, c0, first ? 1 : 0, ws.partials.p,
+                              b32 ? lv_[level].x0.p : nullptr, c0, first ? 1 : 0, xmode, ws.partials.p,
                               ws.scalars.p, ws.gate);
             if (profiling) ws.timers[K_UPDATE64].end(st);
             first = false;
@@ -493,6 +505,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
         ws.read_scalars();
         ws.gate = nullptr;
         const int done = batch > 1 ? (int)ws.host_sc->iters : 1;
+        run += done;
         info.iterations += done;
         if (!learned && cfg.precond == 1 && info.precond_used == 1) info.fallback += done;
         const double prev = rel;
@@ -509,6 +522,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
         }
         if (learned && rel > cfg.stall_ratio * prev) learned = false;  // fall back to multigrid
     }
+    bring_x_up_to_date();
     if (!checked) {
         launch_residual64(st, d, A, rhs_.p, x_.p, r_.p, ws.partials.p, ws.scalars.p);
         ws.read_scalars();
