@@ -306,4 +306,8 @@ sfp_status sfp_test_set_fused_pairs(sfp_context ctx, int mask) {
     });
 }
 
+sfp_status sfp_test_set_rowsum_diagonal(sfp_context ctx, int enable) {
+    return run(ctx, [&] { ctx->ws.rowsum_diag = enable != 0; });
+}
+
 }  // extern "C"

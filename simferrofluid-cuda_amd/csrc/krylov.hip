@@ -237,13 +237,22 @@ __global__ __launch_bounds__(kBlock) void k_fcg_matvec_flat(Dims d, Op64 A,
         const float2v zyp = yh ? *(const float2v *)(z + c + s.sy) : f0;
         const float zzm = zl ? z[c - 1] : 0.f;
         const float zzp = zh ? z[c + 2] : 0.f;
-        const double2v dg = *(const double2v *)(A.diag + c);
         const double2v cxm = xl ? *(const double2v *)(A.cpl[0] + c) : d0;
         const double2v cxp = xh ? *(const double2v *)(A.cpl[0] + c + s.sx) : d0;
         const double2v cym = yl ? *(const double2v *)(A.cpl[1] + c) : d0;
         const double2v cyp = yh ? *(const double2v *)(A.cpl[1] + c + s.sy) : d0;
         const double2v cz = *(const double2v *)(A.cpl[2] + c);  // below each cell of the pair
         const double czp = zh ? A.cpl[2][c + 2] : 0.0;          // above the second
+        // A row-sum diagonal is rebuilt from the couplings (the same sums in
+        // the same order as the assembly's, so the same bits); the pairs
+        // holding cell 0 and its upper neighbours read the stored one.
+        double2v dg;
+        if (A.pinned_rowsum && cu != 0 && cu != (unsigned)d.nz && cu != (unsigned)s.sx) {
+            dg.x = ((((cxm.x + cxp.x) + cym.x) + cyp.x) + cz.x) + cz.y;
+            dg.y = ((((cxm.y + cxp.y) + cym.y) + cyp.y) + cz.y) + czp;
+        } else {
+            dg = *(const double2v *)(A.diag + c);
+        }
         const double2v qp = first ? d0 : ld_stream((const double2v *)(qprev + c));
         const double2v rc = ld_stream((const double2v *)(r + c));
         double o0 = 0.0, o1 = 0.0;

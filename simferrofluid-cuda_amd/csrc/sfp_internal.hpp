@@ -105,6 +105,13 @@ struct DevBuf {
 struct Op64 {
     const double *diag;
     const double *cpl[3];
+    // 1: each diagonal entry equals the sum of the row's six couplings, added
+    // in the order -x, +x, -y, +y, -z, +z (0 on the boundary), bit for bit,
+    // except at cell 0 (an identity row) and its upper neighbours 1, nz and
+    // nz*ny (whose coupling to cell 0 is eliminated but stays in the
+    // diagonal). Kernels may then rebuild the diagonal from the couplings
+    // they load anyway instead of reading it.
+    int pinned_rowsum = 0;
 };
 struct Op32 {
     const float *diag;  // diagonal for active cells, 0 marks an inactive cell
@@ -118,7 +125,10 @@ struct Level {
     DevBuf<double> diag64, cpl64[3];
     DevBuf<float> diag32, cpl32[3];
     DevBuf<float> x0, x1, dv, b;  // ping-pong iterates, Chebyshev direction, rhs
-    Op64 op64() const { return Op64{diag64.p, {cpl64[0].p, cpl64[1].p, cpl64[2].p}}; }
+    bool pinned_rowsum = false;  // the FP64 operator has Op64::pinned_rowsum's structure
+    Op64 op64() const {
+        return Op64{diag64.p, {cpl64[0].p, cpl64[1].p, cpl64[2].p}, pinned_rowsum ? 1 : 0};
+    }
     Op32 op32() const { return Op32{diag32.p, {cpl32[0].p, cpl32[1].p, cpl32[2].p}}; }
 };
 

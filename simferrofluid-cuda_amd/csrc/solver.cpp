@@ -406,7 +406,8 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     const hipStream_t st = ws.stream;
     const Dims d = lv_[level].d;
     const size_t n = d.cells();
-    const Op64 A = lv_[level].op64();
+    Op64 A = lv_[level].op64();
+    if (!ws.rowsum_diag) A.pinned_rowsum = 0;
     const bool fine = level == 0;
     const bool profiling = fine && ws.profiling;
     if (!warm) SFP_HIP(hipMemsetAsync(x_.p, 0, n * sizeof(double), st));
@@ -484,7 +485,11 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
                 z = precondition(ws, cfg, r64);
             else
                 z = vlevel(ws, Smoothing::make(cfg), level, r64);
-            if (profiling) ws.timers[K_MATVEC64].begin(st, (4.0 + 32.0 + 8.0 + 8.0 + 8.0) * cells);
+            // z, diag32, the couplings (and the diagonal unless rebuilt from
+            // them), r, q_prev (after the first iteration) in; w out
+            if (profiling)
+                ws.timers[K_MATVEC64].begin(st, (4.0 + 4.0 + (A.pinned_rowsum ? 24.0 : 32.0) + 8.0 +
+                                                 (first ? 0.0 : 8.0) + 8.0) * cells);
             launch_fcg_matvec(st, d, A, lv_[level].diag32.p, z, r_.p, q_.p, w_.p, first ? 1 : 0,
                               ws.partials.p, ws.scalars.p, ws.gate);
             if (profiling) ws.timers[K_MATVEC64].end(st);

@@ -43,7 +43,7 @@ EXPORTS = {
                    "sfp_test_residual", "sfp_test_solve",
                    "sfp_test_restrict_smoothed", "sfp_test_smooth_pair",
                    "sfp_test_smooth_restrict", "sfp_test_set_vcycle_tail",
-                   "sfp_test_set_fused_pairs"],
+                   "sfp_test_set_fused_pairs", "sfp_test_set_rowsum_diagonal"],
     "sfp_learned.h": ["sfp_unet_param_count", "sfp_set_network", "sfp_learned_features",
                       "sfp_unet_create", "sfp_unet_destroy", "sfp_unet_set_params",
                       "sfp_unet_get_params", "sfp_unet_init_params", "sfp_unet_forward",
@@ -153,6 +153,8 @@ def lib():
         L.sfp_test_set_vcycle_tail.argtypes = [vp, C.c_int]
         L.sfp_test_set_fused_pairs.restype = C.c_int
         L.sfp_test_set_fused_pairs.argtypes = [vp, C.c_int]
+        L.sfp_test_set_rowsum_diagonal.restype = C.c_int
+        L.sfp_test_set_rowsum_diagonal.argtypes = [vp, C.c_int]
         L.sfp_test_smooth_restrict.restype = C.c_int
         L.sfp_test_smooth_restrict.argtypes = [vp, C.c_int, C.c_int, _fp, _fp, _fp, C.c_float,
                                                C.c_float, _fp, _fp, _fp, _fp]
@@ -483,6 +485,12 @@ class Context:
         is 18 (bits 1 and 4); True is 31, False 0."""
         mask = 31 if mask is True else 0 if mask is False else int(mask)
         self._check(lib().sfp_test_set_fused_pairs(self._h, mask))
+
+    def set_rowsum_diagonal(self, enable):
+        """Let the FCG matvec of the magnetic system rebuild its diagonal
+        from the couplings (True, the default) or read it (False); the results
+        are the same bit for bit."""
+        self._check(lib().sfp_test_set_rowsum_diagonal(self._h, 1 if enable else 0))
 
     def smooth_restrict(self, which, level, b, x1, d1, cd, cr):
         """The last pre-smoothing step and the restriction on a level, from

@@ -302,6 +302,33 @@ def test_magnetic_invariants(oracle, name):
     assert abs(b.sum() + row0) <= 1e-12 * max(1.0, np.abs(b).sum())
 
 
+@pytest.mark.parametrize("name", list(MSCENES))
+def test_magnetic_diagonal_is_ordered_coupling_sum(oracle, name):
+    """Away from cell 0 and its upper neighbours, each diagonal entry equals,
+    bit for bit, the row's off-diagonal magnitudes added in the order -x, +x,
+    -y, +y, -z, +z (what the FCG matvec rebuilds instead of reading the
+    diagonal); at those four cells it does not."""
+    sc = MSCENES[name]
+    A = oracle.magnetic_system(sc)["A"].tocoo()
+    nx, ny, nz = sc["geo"].n
+    n = nx * ny * nz
+    off = {}
+    for r, c, v in zip(A.row, A.col, A.data):
+        if r != c:
+            off[(int(r), int(c) - int(r))] = -float(v)
+    diag = A.diagonal()
+    strides = (ny * nz, nz, 1)
+    special = {0, 1, nz, ny * nz}
+    for r in range(n):
+        acc = 0.0
+        for s in strides:
+            acc = (acc + off.get((r, -s), 0.0)) + off.get((r, s), 0.0)
+        if r in special:
+            assert acc != diag[r]
+        else:
+            assert acc == diag[r], r
+
+
 def test_k1_planar_interface_gives_exact_flux(oracle):
     n = 12
     geo = scenes.Geometry(n, 1.0 / n)
