@@ -69,19 +69,21 @@ def parse():
 # rocprofv3 symbol, algorithmic bytes per cell (each distinct array once per
 # launch, as the library counts them) and what the kernel does.
 HOT_KERNELS = {
-    "smooth_pair": ("k_smooth_pair<false>", 30.0,
-                    "fine-level pairs of Chebyshev steps in one pass, FP32 7-point stencil: the "
-                    "first two pre-smoothing steps (reads the stored first iterate, b, diag, 3 "
-                    "couplings, writes x: 28 B/cell) and the last two post-smoothing steps (reads "
-                    "x, b, d, diag, 3 couplings, writes x: 32 B/cell)"),
-    "smooth_restrict": ("k_smooth_pair<true>", 32.5,
+    "smooth_pair": ("k_smooth_pair<false, true>", 26.0,
+                    "fine-level pairs of Chebyshev steps in one pass, FP32 7-point stencil, the "
+                    "magnetic diagonal rebuilt from the couplings: the first two pre-smoothing "
+                    "steps (reads the stored first iterate, b, 3 couplings, writes x: 24 B/cell) "
+                    "and the last two post-smoothing steps (reads x, b, d, 3 couplings, writes x: "
+                    "28 B/cell)"),
+    "smooth_restrict": ("k_smooth_pair<true, true>", 28.5,
                         "fine-level last pre-smoothing step and the residual restriction in one "
                         "pass, FP32: reads x, b, d, diag, 3 couplings, writes x and the coarse b"),
-    "smooth": ("k_smooth_flat", 32.0,
+    "smooth": ("k_smooth_flat", 28.0,
                "fine-level single Chebyshev steps over the flat cell index (the first two "
                "pre-smoothing steps from the stored iterate, the last one, the first "
                "post-smoothing step with the coarse correction), FP32 7-point stencil: reads x, "
-               "b, diag, 3 couplings (and d), writes x (and d)"),
+               "b, 3 couplings (the magnetic diagonal rebuilt from them; and d), writes x (and "
+               "d)"),
     "matvec64": ("k_fcg_matvec_flat", 56.0,
                  "flexible-CG w = A z with the FP64 operator and its 3 inner products "
                  "(two cells of a z row per lane): reads z, diag32, 3 FP64 couplings (the "
@@ -90,7 +92,7 @@ HOT_KERNELS = {
                  "flexible-CG update of p, q, r (FP64) with r.r, the FP32 copy b of r and the next "
                  "cycle's first iterate c0 D^-1 b; x read and written every other iteration "
                  "(88 B then 72 B per cell)"),
-    "restrict": ("k_restrict_flat", 25.5, "fine-level FP32 residual summed over 2x2x2 children, "
+    "restrict": ("k_restrict_flat", 21.5, "fine-level FP32 residual summed over 2x2x2 children, "
                  "with the coarse level's first Chebyshev iterate"),
 }
 

@@ -67,10 +67,11 @@ sfp_status sfp_test_set_vcycle_tail(sfp_context ctx, int enable);
  * pre-smoothing passes of a degree-3 cycle on large grids, or each as
  * separate launches. Results are the same bit for bit. */
 sfp_status sfp_test_set_fused_pairs(sfp_context ctx, int mask); /* default 18 */
-/* Let the FCG matvec of the magnetic system rebuild its diagonal from the
- * couplings (each row's diagonal is their sum, bit for bit, away from the
- * pinned cell) instead of reading it, or read it. Results are the same bit
- * for bit. */
+/* Let the stencil kernels of the magnetic system (FCG matvec, smoothing
+ * steps, restriction) rebuild the FP64 and FP32 diagonals from the couplings
+ * (each row's diagonal is their sum, bit for bit, away from the pinned cell)
+ * instead of reading them, or read them. Results are the same bit for bit.
+ * Takes effect at the next magnetic solve. */
 sfp_status sfp_test_set_rowsum_diagonal(sfp_context ctx, int enable); /* default 1 */
 
 #ifdef __cplusplus

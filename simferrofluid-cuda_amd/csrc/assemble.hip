@@ -294,6 +294,13 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_assemble(CellGrid cg, Dims 
             }
             double diag = 0.;
             double b = 0;
+            // FP32 diagonal: the sum of the FP32 couplings in the same order
+            // (in FP64, rounded once), which the stencil kernels can rebuild
+            // from the couplings (next to the pinned cell the eliminated
+            // coupling stays in the diagonal, which is then the FP64 one
+            // rounded)
+            double diag_f = 0.;
+            bool next_to_pin = false;
 #pragma unroll
             for (int ord = 0; ord < 6; ord++) {
                 const int axis = ord >> 1, side = (ord & 1) ? 1 : -1;
@@ -302,11 +309,13 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_assemble(CellGrid cg, Dims 
                 // column 0 is dropped from every other row (pin by elimination)
                 if (side < 0 && !pin[ord]) low[axis] = weight;
                 diag += weight;
+                diag_f += (double)(float)weight;
+                next_to_pin = next_to_pin || pin[ord];
                 b += (weight - 1.) * hv[ord] * (double)(-side) * spacing;
             }
             diag64[c] = diag;
             rhs[c] = b;
-            diag32[c] = (float)diag;
+            diag32[c] = (float)(next_to_pin ? diag : diag_f);
         }
         for (int a = 0; a < 3; a++) {
             cpl64.a[a][c] = low[a];

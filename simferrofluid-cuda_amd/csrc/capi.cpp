@@ -177,7 +177,7 @@ void magnetic_dev(sfp_context ctx, const double *phi, const double *const hext[3
     float *c32[3] = {L.cpl32[0].p, L.cpl32[1].p, L.cpl32[2].p};
     launch_magnetic_assemble(s, d, Mf, H, ctx->spacing, L.diag64.p, c64, sys.rhs().p, L.diag32.p,
                              c32);
-    L.pinned_rowsum = true;
+    L.pinned_rowsum = ctx->ws.rowsum_diag;
     sys.build_hierarchy(ctx->ws, ctx->cfg);
     SFP_HIP(hipEventRecord(ctx->ev[1], s));
     ctx->ws.phi = phi;

@@ -201,7 +201,31 @@ __global__ __launch_bounds__(kBlock) void k_smooth(CellGrid cg, Dims d, Op32 A,
 // same bits. A wavefront whose 256 cells are all inactive writes nothing.
 constexpr unsigned kSmoothFlatCells = 4 * kBlock;
 
-template <int MODE>
+// Row-sum FP32 diagonal (Op32::pinned_rowsum) of one cell from its six
+// couplings in the order -x, +x, -y, +y, -z, +z (0 where there is no
+// neighbour): summed in FP64 (exact for couplings within 2^27 of each other)
+// and rounded once, the assembly's arithmetic, so the bits are the stored
+// diagonal's.
+__device__ __forceinline__ float rowsum_diag(float xl, float xh, float yl, float yh, float zl, float zh) {
+    return (float)(((((((double)xl + (double)xh) + (double)yl) + (double)yh) + (double)zl) + (double)zh));
+}
+// The same for the four cells of a group: the lower x and y couplings (cxc,
+// cym), the upper ones (cxp, cyp: the next plane's and row's lower), the
+// lower z ones (cz) and the one above the group (czhi).
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f4v rowsum_diag4(f4v cxc, f4v cxp, f4v cym, f4v cyp, f4v cz, float czhi) {
+    f4v dg;
+#pragma unroll
+    for (int q = 0; q < 4; q++) dg[q] = rowsum_diag(cxc[q], cxp[q], cym[q], cyp[q], cz[q], q < 3 ? cz[q + 1] : czhi);
+    return dg;
+}
+// The group starting at cell c (c a multiple of 4, nz a multiple of 4) holds
+// none of the cells whose diagonal is not the row sum (1, nz, nz*ny).
+__device__ __forceinline__ bool rowsum_group(const Op32 &A, size_t c, Dims d) {
+    return A.pinned_rowsum && c != 0 && c != (size_t)d.nz && c != (size_t)d.ny * d.nz;
+}
+
+template <int MODE, bool kRowsum>
 __global__ __launch_bounds__(kBlock) void k_smooth_flat(Dims d, Op32 A, const float *__restrict__ b,
                                                         const float *__restrict__ xin,
                                                         float *__restrict__ xout, float *dvec,
@@ -217,8 +241,9 @@ __global__ __launch_bounds__(kBlock) void k_smooth_flat(Dims d, Op32 A, const fl
     const size_t c = (size_t)blockIdx.x * kSmoothFlatCells + 4 * (size_t)flat_tid();
     const bool on = c < d.cells();  // cells() is a multiple of 4
     const f4 z4 = {0.f, 0.f, 0.f, 0.f};
-    const f4 dg = on ? *(const f4 *)(A.diag + c) : z4;
-    const bool act = dg.x > 0.f || dg.y > 0.f || dg.z > 0.f || dg.w > 0.f;
+    const bool rs = kRowsum && on && rowsum_group(A, c, d);  // every group there has an active cell
+    f4 dg = (on && !rs) ? *(const f4 *)(A.diag + c) : z4;
+    const bool act = rs || dg.x > 0.f || dg.y > 0.f || dg.z > 0.f || dg.w > 0.f;
     if (!__any(act) || !on) return;
     // (i, j, k) of the first cell (the launcher keeps cells() < 2^32)
     const unsigned cu = (unsigned)c;
@@ -241,6 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth_flat(Dims d, Op32 A, const fl
     float xzlo = zl ? xin[c - 1] : 0.f;
     float xzhi = zh ? xin[c + 4] : 0.f;
     const float czhi = zh ? A.cpl[2][c + 4] : 0.f;
+    if (rs) dg = rowsum_diag4(cxc, cxp, cym, cyp, cz, czhi);
     const f4 bc = *(const f4 *)(b + c);
     f4 dprev = z4;
     if constexpr ((MODE & SM_READ_D) != 0) dprev = *(const f4 *)(dvec + c);
@@ -318,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth_flat(Dims d, Op32 A, const fl
 constexpr int kPairRows = 16;
 constexpr int kPairZ = 62, kPairY = kPairRows - 2;  // interior extents (tile strides)
 
-template <bool kRestrict>
+template <bool kRestrict, bool kRowsum>
 __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
     CellGrid cg, Dims d, Op32 A, const float *__restrict__ b, const float *__restrict__ x1,
     float *__restrict__ xo, const float *__restrict__ d1, float cdA, float crA, float cdB,
@@ -367,8 +393,10 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
         Plane P{};
         if (q >= d.nx) return P;
         const size_t c = c0 + q * s.sx;
-        const float dg = on ? A.diag[c] : 0.f;
-        P.live = __any(on && dg > 0.f);
+        // a row-sum diagonal is rebuilt below (cells 1, nz and nz*ny read it)
+        const bool rs = kRowsum && on && A.pinned_rowsum && c != 1 && c != (size_t)d.nz && c != s.sx;
+        float dg = (on && !rs) ? A.diag[c] : 0.f;
+        P.live = __any(on && (rs || dg > 0.f));
         if (!P.live) {
             primed = false;
             return P;
@@ -397,6 +425,7 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
             xzp = (on && zhi) ? x1[c + 1] : 0.f;
             czp = (on && zhi) ? A.cpl[2][c + 1] : 0.f;
         }
+        if (rs) dg = rowsum_diag(cxc, cxp, cym, cyp, czm, czp);
         if (on) {
             float off = fmaf(cxc, xm, cxp * xp);
             if (j > 0) off = fmaf(cym, xym, off);
@@ -611,6 +640,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(CellGrid cg, Dims df, Op32 
 // stores zeros.
 constexpr int kRestrictFlatZ = 64;  // z cells per wavefront
 
+template <bool kRowsum>
 __global__ __launch_bounds__(kBlock) void k_restrict_flat(Dims df, Op32 A, const float *__restrict__ b,
                                                           const float *__restrict__ x, Dims dc,
                                                           float *__restrict__ bc, float *__restrict__ x0c,
@@ -632,9 +662,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict_flat(Dims df, Op32 A, const
     const Strides s(df);
     const size_t c = on ? df.idx(i, j, k) : 0;
     const f4 z4 = {0.f, 0.f, 0.f, 0.f};
-    const f4 dg = on ? *(const f4 *)(A.diag + c) : z4;
+    const bool rs = kRowsum && on && rowsum_group(A, c, df);  // every group there has an active cell
+    f4 dg = (on && !rs) ? *(const f4 *)(A.diag + c) : z4;
     f4 r = z4;
-    if (__any(dg.x > 0.f || dg.y > 0.f || dg.z > 0.f || dg.w > 0.f) && on) {
+    if (__any(rs || dg.x > 0.f || dg.y > 0.f || dg.z > 0.f || dg.w > 0.f) && on) {
         const bool up = i + 1 < df.nx, ylo = j > 0, yhi = j + 1 < df.ny;
         const bool zl = k > 0, zh = k + 4 < df.nz;
         const f4 xc = *(const f4 *)(x + c);
@@ -650,6 +681,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict_flat(Dims df, Op32 A, const
         const float xzlo = zl ? x[c - 1] : 0.f;
         const float xzhi = zh ? x[c + 4] : 0.f;
         const float czhi = zh ? A.cpl[2][c + 4] : 0.f;
+        if (rs) dg = rowsum_diag4(cxc, cxp, cym, cyp, cz, czhi);
         const f4 bv = *(const f4 *)(b + c);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -1152,10 +1184,14 @@ void launch_smooth(hipStream_t st, int mode, Dims d, Op32 A, const double *r64, 
     if (smooth_flat(d, mode, r64, ecoarse, {A.diag, A.cpl[0], A.cpl[1], A.cpl[2], b, xin, xout, dvec})) {
         const unsigned nb = (unsigned)((d.cells() + kSmoothFlatCells - 1) / kSmoothFlatCells);
         const Dims dc = d.coarse();
-#define SFP_FLAT_CASE(M)                                                                        \
-    case M:                                                                                     \
-        hipLaunchKernelGGL(k_smooth_flat<M>, dim3(nb), dim3(kBlock), 0, st, d, A, b, xin, xout, \
-                           dvec, cd, cr, ecoarse, dc, gate);                                    \
+#define SFP_FLAT_CASE(M)                                                                           \
+    case M:                                                                                        \
+        if (A.pinned_rowsum)                                                                       \
+            hipLaunchKernelGGL((k_smooth_flat<M, true>), dim3(nb), dim3(kBlock), 0, st, d, A, b,   \
+                               xin, xout, dvec, cd, cr, ecoarse, dc, gate);                        \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_smooth_flat<M, false>), dim3(nb), dim3(kBlock), 0, st, d, A, b,  \
+                               xin, xout, dvec, cd, cr, ecoarse, dc, gate);                        \
         break;
         switch (mode) {
             SFP_FLAT_CASE(SM_PROLONG)
@@ -1217,8 +1253,12 @@ void launch_smooth_pair(hipStream_t st, Dims d, Op32 A, const float *b, const fl
                         const double *gate) {
     if (x1 == x3) throw ArgError("launch_smooth_pair: x3 aliases x1");
     const Launch L = pair_launch(d, false);
-    hipLaunchKernelGGL(k_smooth_pair<false>, L.grid, L.block, 0, st, L.cg, d, A, b, x1, x3, d1, cdA,
-                       crA, cdB, crB, Dims{}, nullptr, gate);
+    if (A.pinned_rowsum)
+        hipLaunchKernelGGL((k_smooth_pair<false, true>), L.grid, L.block, 0, st, L.cg, d, A, b, x1, x3,
+                           d1, cdA, crA, cdB, crB, Dims{}, nullptr, gate);
+    else
+        hipLaunchKernelGGL((k_smooth_pair<false, false>), L.grid, L.block, 0, st, L.cg, d, A, b, x1, x3,
+                           d1, cdA, crA, cdB, crB, Dims{}, nullptr, gate);
     SFP_LAUNCH_CHECK();
 }
 
@@ -1227,8 +1267,12 @@ void launch_smooth_restrict(hipStream_t st, Dims d, Op32 A, const float *b, cons
                             const double *gate) {
     if (x1 == x2) throw ArgError("launch_smooth_restrict: x2 aliases x1");
     const Launch L = pair_launch(d, true);
-    hipLaunchKernelGGL(k_smooth_pair<true>, L.grid, L.block, 0, st, L.cg, d, A, b, x1, x2, d1, cd,
-                       cr, 0.f, 0.f, d.coarse(), bc, gate);
+    if (A.pinned_rowsum)
+        hipLaunchKernelGGL((k_smooth_pair<true, true>), L.grid, L.block, 0, st, L.cg, d, A, b, x1, x2,
+                           d1, cd, cr, 0.f, 0.f, d.coarse(), bc, gate);
+    else
+        hipLaunchKernelGGL((k_smooth_pair<true, false>), L.grid, L.block, 0, st, L.cg, d, A, b, x1, x2,
+                           d1, cd, cr, 0.f, 0.f, d.coarse(), bc, gate);
     SFP_LAUNCH_CHECK();
 }
 
@@ -1257,8 +1301,12 @@ void launch_restrict(hipStream_t st, Dims df, Op32 A, const float *b, const floa
         const size_t waves = (size_t)((df.nz + kRestrictFlatZ - 1) / kRestrictFlatZ) *
                              ((df.ny + 1) / 2) * ((df.nx + 1) / 2);
         const unsigned nb = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_restrict_flat, dim3(nb), dim3(kBlock), 0, st, df, A, b, x, dc, bc, x0c,
-                           diag32c, c0, gate);
+        if (A.pinned_rowsum)
+            hipLaunchKernelGGL(k_restrict_flat<true>, dim3(nb), dim3(kBlock), 0, st, df, A, b, x, dc,
+                               bc, x0c, diag32c, c0, gate);
+        else
+            hipLaunchKernelGGL(k_restrict_flat<false>, dim3(nb), dim3(kBlock), 0, st, df, A, b, x, dc,
+                               bc, x0c, diag32c, c0, gate);
         SFP_LAUNCH_CHECK();
         return;
     }

@@ -116,6 +116,9 @@ struct Op64 {
 struct Op32 {
     const float *diag;  // diagonal for active cells, 0 marks an inactive cell
     const float *cpl[3];
+    // 1: as Op64::pinned_rowsum, with the FP32 sums (cell 0's rebuilt
+    // diagonal is 0, an inactive cell, as stored; cells 1, nz and nz*ny differ)
+    int pinned_rowsum = 0;
 };
 
 // One multigrid level: FP64 operator (setup, and the fine level's outer
@@ -125,11 +128,15 @@ struct Level {
     DevBuf<double> diag64, cpl64[3];
     DevBuf<float> diag32, cpl32[3];
     DevBuf<float> x0, x1, dv, b;  // ping-pong iterates, Chebyshev direction, rhs
-    bool pinned_rowsum = false;  // the FP64 operator has Op64::pinned_rowsum's structure
+    // both operators have the row-sum structure of Op64::pinned_rowsum and
+    // Op32::pinned_rowsum, and kernels may rebuild the diagonals
+    bool pinned_rowsum = false;
     Op64 op64() const {
         return Op64{diag64.p, {cpl64[0].p, cpl64[1].p, cpl64[2].p}, pinned_rowsum ? 1 : 0};
     }
-    Op32 op32() const { return Op32{diag32.p, {cpl32[0].p, cpl32[1].p, cpl32[2].p}}; }
+    Op32 op32() const {
+        return Op32{diag32.p, {cpl32[0].p, cpl32[1].p, cpl32[2].p}, pinned_rowsum ? 1 : 0};
+    }
 };
 
 // Scalars of the flexible-CG recurrences, resident on the device.

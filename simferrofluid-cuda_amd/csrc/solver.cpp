@@ -197,9 +197,10 @@ static bool small_grid(Dims d) { return d.cells() <= ((size_t)1 << 19); }
 
 // Algorithmic bytes per cell of one single Chebyshev step: x, b, diag and 3
 // couplings in and x out, plus the direction in and out by mode and the
-// coarse correction (an eighth of a value per cell).
-static double step_bytes(int mode) {
-    double b = 4.0 * 7;
+// coarse correction (an eighth of a value per cell). rowsum: the flat
+// kernels rebuild the diagonal instead of reading it.
+static double step_bytes(int mode, bool rowsum) {
+    double b = 4.0 * (rowsum ? 6 : 7);
     if (mode & SM_READ_D) b += 4.0;
     if (mode & SM_WRITE_D) b += 4.0;
     if (mode & SM_PROLONG) b += 0.5;
@@ -223,12 +224,16 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
         return L.x0.p;
     }
     const Op32 A = L.op32();
+    // the flat kernels (nz a multiple of 4) and the pair kernels rebuild a
+    // row-sum diagonal instead of reading it
+    const bool diag_rebuilt = L.pinned_rowsum && flat_step(L.d);
+    const double diag_bytes = diag_rebuilt ? 0.0 : 4.0, pair_diag_bytes = L.pinned_rowsum ? 0.0 : 4.0;
     float *cur = L.x0.p, *oth = L.x1.p;
     const int deg = cb.degree;
     auto smooth = [&](int mode, float cd, float cr, const double *r, const float *e) {
         // timed: the fine-level single steps but the fused first step
         const bool track = fine && ws.profiling && !(mode & SM_FIRST);
-        if (track) ws.timers[K_SMOOTH].begin(st, step_bytes(mode) * cells);
+        if (track) ws.timers[K_SMOOTH].begin(st, step_bytes(mode, diag_rebuilt) * cells);
         launch_smooth(st, mode, L.d, A, r, L.b.p, cur, oth, L.dv.p, cb.cr[0], cd, cr, e, ws.gate);
         if (track) ws.timers[K_SMOOTH].end(st);
         std::swap(cur, oth);
@@ -249,7 +254,7 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
                               (ws.fused_pairs & 16) && pair_pays(L.d);
         if (pre_pair) {
             // reads x0, b, diag, 3 couplings; writes x
-            if (fine && ws.profiling) ws.timers[K_SMOOTH_PAIR].begin(st, 28.0 * cells);
+            if (fine && ws.profiling) ws.timers[K_SMOOTH_PAIR].begin(st, (24.0 + pair_diag_bytes) * cells);
             launch_smooth_pair(st, L.d, A, L.b.p, cur, oth, cur, cb.cd[1], cb.cr[1], cb.cd[2], cb.cr[2],
                                ws.gate);
             if (fine && ws.profiling) ws.timers[K_SMOOTH_PAIR].end(st);
@@ -263,7 +268,7 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
     bool store_x0 = false;  // the restriction stores the coarse level's first iterate
     if (fuse_restrict) {
         // reads x, b, d, diag, 3 couplings; writes x and the coarse b
-        if (fine && ws.profiling) ws.timers[K_SMOOTH_RESTRICT].begin(st, (32.0 + 0.5) * cells);
+        if (fine && ws.profiling) ws.timers[K_SMOOTH_RESTRICT].begin(st, (28.0 + pair_diag_bytes + 0.5) * cells);
         launch_smooth_restrict(st, L.d, A, L.b.p, cur, oth, L.dv.p, cb.cd[deg - 1], cb.cr[deg - 1],
                                lv_[l + 1].b.p, ws.gate);
         if (fine && ws.profiling) ws.timers[K_SMOOTH_RESTRICT].end(st);
@@ -274,7 +279,7 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
         store_x0 = l + 2 < (int)lv_.size() && sm.at(l + 1).degree >= 2;
         // reads x, b, diag, 3 couplings; writes the coarse b (and reads the
         // coarse diagonal, writes the coarse x0): per fine cell
-        if (fine && ws.profiling) ws.timers[K_RESTRICT].begin(st, (24.0 + 0.5 + (store_x0 ? 1.0 : 0.0)) * cells);
+        if (fine && ws.profiling) ws.timers[K_RESTRICT].begin(st, (20.0 + diag_bytes + 0.5 + (store_x0 ? 1.0 : 0.0)) * cells);
         launch_restrict(st, L.d, A, L.b.p, cur, C.b.p, ws.gate, store_x0 ? C.x0.p : nullptr,
                         C.diag32.p, sm.at(l + 1).cr[0]);
         if (fine && ws.profiling) ws.timers[K_RESTRICT].end(st);
@@ -305,15 +310,15 @@ float *System::post_smooth(Workspace &ws, const Smoothing &sm, int l, float *cur
         // step 0 adds the prolonged correction; the fine-level steps are timed
         const int mode = (s == 0 ? (add_first ? 0 : SM_PROLONG) : SM_READ_D) | (s < deg - 1 ? SM_WRITE_D : 0);
         const bool t = track;
-        if (t) ws.timers[K_SMOOTH].begin(st, step_bytes(mode) * (double)L.d.cells());
+        if (t) ws.timers[K_SMOOTH].begin(st, step_bytes(mode, L.pinned_rowsum && flat_step(L.d)) * (double)L.d.cells());
         launch_smooth(st, mode, L.d, A, nullptr, L.b.p, cur, oth, L.dv.p, cb.cr[0], s == 0 ? 0.f : cb.cd[s],
                       s == 0 ? cb.cr[0] : cb.cr[s], s == 0 ? e : nullptr, ws.gate);
         if (t) ws.timers[K_SMOOTH].end(st);
         std::swap(cur, oth);
     }
     if (fused) {
-        // reads x, b, d, diag, 3 couplings and writes x: 32 algorithmic bytes per cell
-        if (track) ws.timers[K_SMOOTH_PAIR].begin(st, 32.0 * (double)L.d.cells());
+        // reads x, b, d, diag (unless rebuilt), 3 couplings and writes x
+        if (track) ws.timers[K_SMOOTH_PAIR].begin(st, (L.pinned_rowsum ? 28.0 : 32.0) * (double)L.d.cells());
         launch_smooth_pair(st, L.d, A, L.b.p, cur, oth, L.dv.p, cb.cd[deg - 2], cb.cr[deg - 2],
                            cb.cd[deg - 1], cb.cr[deg - 1], ws.gate);
         if (track) ws.timers[K_SMOOTH_PAIR].end(st);
@@ -406,8 +411,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     const hipStream_t st = ws.stream;
     const Dims d = lv_[level].d;
     const size_t n = d.cells();
-    Op64 A = lv_[level].op64();
-    if (!ws.rowsum_diag) A.pinned_rowsum = 0;
+    const Op64 A = lv_[level].op64();
     const bool fine = level == 0;
     const bool profiling = fine && ws.profiling;
     if (!warm) SFP_HIP(hipMemsetAsync(x_.p, 0, n * sizeof(double), st));

@@ -85,8 +85,8 @@ struct Workspace {
     // 4: the first two pre-smoothing passes (degree 3, from the stored
     // first iterate) on large grids
     int fused_pairs = 2 | 16;
-    // the FCG matvec rebuilds a row-sum diagonal (Op64::pinned_rowsum) from
-    // the couplings instead of reading it
+    // the stencil kernels rebuild a row-sum diagonal (Level::pinned_rowsum)
+    // from the couplings instead of reading it
     bool rowsum_diag = true;
     // level whose x0 holds the first Chebyshev iterate of its b (stored by
     // the FCG update for the next cycle), or -1

@@ -487,9 +487,10 @@ class Context:
         self._check(lib().sfp_test_set_fused_pairs(self._h, mask))
 
     def set_rowsum_diagonal(self, enable):
-        """Let the FCG matvec of the magnetic system rebuild its diagonal
-        from the couplings (True, the default) or read it (False); the results
-        are the same bit for bit."""
+        """Let the stencil kernels of the magnetic system rebuild the FP64 and
+        FP32 diagonals from the couplings (True, the default) or read them
+        (False), from the next magnetic solve on; the results are the same bit
+        for bit."""
         self._check(lib().sfp_test_set_rowsum_diagonal(self._h, 1 if enable else 0))
 
     def smooth_restrict(self, which, level, b, x1, d1, cd, cr):
