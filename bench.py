@@ -84,10 +84,10 @@ HOT_KERNELS = {
                "post-smoothing step with the coarse correction), FP32 7-point stencil: reads x, "
                "b, 3 couplings (the magnetic diagonal rebuilt from them; and d), writes x (and "
                "d)"),
-    "matvec64": ("k_fcg_matvec_flat", 56.0,
+    "matvec64": ("k_fcg_matvec_flat", 52.0,
                  "flexible-CG w = A z with the FP64 operator and its 3 inner products "
-                 "(two cells of a z row per lane): reads z, diag32, 3 FP64 couplings (the "
-                 "magnetic diagonal rebuilt from them), r, q_prev, writes w"),
+                 "(two cells of a z row per lane): reads z, 3 FP64 couplings (the magnetic "
+                 "diagonal rebuilt from them, no activity flags), r, q_prev, writes w"),
     "update64": ("k_fcg_update", 80.0,
                  "flexible-CG update of p, q, r (FP64) with r.r, the FP32 copy b of r and the next "
                  "cycle's first iterate c0 D^-1 b; x read and written every other iteration "

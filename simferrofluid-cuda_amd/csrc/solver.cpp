@@ -489,13 +489,15 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
                 z = precondition(ws, cfg, r64);
             else
                 z = vlevel(ws, Smoothing::make(cfg), level, r64);
-            // z, diag32, the couplings (and the diagonal unless rebuilt from
-            // them), r, q_prev (after the first iteration) in; w out
+            // z, the couplings, r, q_prev (after the first iteration) in; w
+            // out; and the diagonal and the FP32 one (skips inactive cells)
+            // unless the operator has the row-sum structure, where every cell
+            // but cell 0 is active and the diagonal is rebuilt
             if (profiling)
-                ws.timers[K_MATVEC64].begin(st, (4.0 + 4.0 + (A.pinned_rowsum ? 24.0 : 32.0) + 8.0 +
+                ws.timers[K_MATVEC64].begin(st, (4.0 + (A.pinned_rowsum ? 24.0 : 4.0 + 32.0) + 8.0 +
                                                  (first ? 0.0 : 8.0) + 8.0) * cells);
-            launch_fcg_matvec(st, d, A, lv_[level].diag32.p, z, r_.p, q_.p, w_.p, first ? 1 : 0,
-                              ws.partials.p, ws.scalars.p, ws.gate);
+            launch_fcg_matvec(st, d, A, A.pinned_rowsum ? nullptr : lv_[level].diag32.p, z, r_.p, q_.p,
+                              w_.p, first ? 1 : 0, ws.partials.p, ws.scalars.p, ws.gate);
             if (profiling) ws.timers[K_MATVEC64].end(st);
             // z, w, p, q, r in; p, q, r out; x in and out every other
             // iteration (and diag in, b and x0 out)
