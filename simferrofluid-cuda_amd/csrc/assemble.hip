@@ -50,6 +50,7 @@ __device__ __forceinline__ bool valid(Dims d, const int c[3]) {
 }
 
 // TriLerp::Interpolate with clamped reads (core/TriLerp.h:22-36,54-61).
+template <bool kSkipZero = false>
 __device__ double trilerp(Dims g, const double org[3], double spacing, double inv,
                           const double *__restrict__ data, const double pos[3]) {
     int lo[3];
@@ -63,6 +64,9 @@ __device__ double trilerp(Dims g, const double org[3], double spacing, double in
         w0[a] = 1. - fr;
         w1[a] = fr;
     }
+    // kSkipZero: a corner of weight exactly 0 adds a zero (the data are
+    // finite), which leaves the sum unchanged but for the sign of a zero sum;
+    // it is not loaded (at face centres of the cell grid, 6 of the 8 corners).
     double val = 0;
 #pragma unroll
     for (int p = 0; p < 8; p++) {
@@ -70,7 +74,7 @@ __device__ double trilerp(Dims g, const double org[3], double spacing, double in
         int c[3] = {lo[0] + ox, lo[1] + oy, lo[2] + oz};
         for (int a = 0; a < 3; a++) c[a] = min(max(c[a], 0), g.n(a) - 1);
         const double wt = (ox ? w1[0] : w0[0]) * (oy ? w1[1] : w0[1]) * (oz ? w1[2] : w0[2]);
-        val += data[g.idx(c[0], c[1], c[2])] * wt;
+        if (!kSkipZero || wt != 0.) val += data[g.idx(c[0], c[1], c[2])] * wt;
     }
     return val;
 }
@@ -249,7 +253,9 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_mu(CellGrid cg, Dims d, int
         }
         double pos[3];
         for (int a = 0; a < 3; a++) pos[a] = g.face_origin[axis][a] + (double)f[a] * g.spacing;
-        const double ph = trilerp(d, g.ls_origin, g.spacing, g.inv, phi, pos);
+        // (the Heaviside of +0 and -0 is the same, so trilerp's zero sign
+        // does not matter)
+        const double ph = trilerp<true>(d, g.ls_origin, g.spacing, g.inv, phi, pos);
         mu[fidx] = 1. + (1. - heaviside(ph, band)) * chi;
     }
 }
