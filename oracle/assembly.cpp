@@ -9,6 +9,21 @@
 #include <utility>
 #include <vector>
 
+// Planted mistakes for the oracle's own closed-form tests (orc_set_mutation):
+// each one changes a single expression of the restatement so that a
+// closed-form check must fail. 0, the default, is the faithful restatement.
+enum Mutation {
+    MUT_NONE = 0,
+    MUT_HEAVISIDE_NO_SIN = 1,   // band interior without the sin term
+    MUT_HEAVISIDE_HALF_BAND = 2,  // band of half the width
+    MUT_JUMP_SIGN = 3,          // jump subtracted in the rhs and in the update (consistently)
+    MUT_JUMP_NO_COEF = 4,       // rhs jump term without 1/max(theta, .001)
+    MUT_NO_CLAMP = 5,           // 1/theta at both ghost-fluid sites
+    MUT_TRILERP_SWAP = 6,       // trilinear weights (1 - f, f) swapped
+    MUT_FACE_ORIGIN = 7,        // face grids of the field placed at the cell centres
+};
+static int g_mutation = MUT_NONE;
+
 namespace {
 
 struct Dims {
@@ -32,7 +47,10 @@ inline int axis_of(int ord) { return ord >> 1; }
 inline int side_of(int ord) { return ord & 1 ? 1 : -1; }
 
 // std::max(theta, .001) with std::max's exact semantics (NaN passes through)
-inline double max_001(double t) { return (t < .001) ? .001 : t; }
+inline double max_001(double t) {
+    if (g_mutation == MUT_NO_CLAMP) return t;
+    return (t < .001) ? .001 : t;
+}
 
 void coord_of(Dims const &d, int64_t idx, int c[3]) {  // core/Grid.h:31
     c[0] = (int)(idx / d.n[2] / d.n[1]);
@@ -54,6 +72,7 @@ double trilerp(Dims const &g, const double org[3], double spacing, double inv,
         double fr = ((pos[a] - org[a]) - (double)lo[a] * spacing) * inv;
         w[0][a] = 1. - fr;
         w[1][a] = fr;
+        if (g_mutation == MUT_TRILERP_SWAP) std::swap(w[0][a], w[1][a]);
     }
     static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
                                   {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
@@ -73,8 +92,10 @@ double trilerp(Dims const &g, const double org[3], double spacing, double inv,
 // MagneticField::Heaviside (core/MagneticField.h:44-54)
 double heaviside(double phi, double band) {
     const double pi = 3.141592653589793238462643383279502884;
+    if (g_mutation == MUT_HEAVISIDE_HALF_BAND) band *= .5;
     if (phi < -band) return 0.;
     if (phi > band) return 1.;
+    if (g_mutation == MUT_HEAVISIDE_NO_SIN) return .5 * (1. + phi / band);
     return .5 * (1. + phi / band + std::sin(pi * phi / band) / pi);
 }
 
@@ -94,13 +115,16 @@ ExtGeometry ext_geometry(Dims const &c, double spacing, const double sim_origin[
     }
     for (int f = 0; f < 3; f++)
         for (int a = 0; a < 3; a++)
-            g.face_origin[f][a] = g.origin[a] + (a == f ? 0.0 : 1.0) * spacing / 2;
+            g.face_origin[f][a] =
+                g.origin[a] + (a == f && g_mutation != MUT_FACE_ORIGIN ? 0.0 : 1.0) * spacing / 2;
     return g;
 }
 
 }  // namespace
 
 extern "C" {
+
+void orc_set_mutation(int m) { g_mutation = m; }
 
 int orc_pressure_unknowns(int nx, int ny, int nz, const double *phi, int *grid2mat,
                           int *mat2grid) {
@@ -153,7 +177,14 @@ int64_t orc_pressure_matrix(int nx, int ny, int nz, const double *phi,
                     double theta = pc / (pc - phi[nidx]);
                     double ic = 1. / max_001(theta);
                     diag += weight * ic;
-                    if (jump) div += jump[axis][fidx] * weight * ic;
+                    if (jump) {
+                        if (g_mutation == MUT_JUMP_SIGN)
+                            div -= jump[axis][fidx] * weight * ic;
+                        else if (g_mutation == MUT_JUMP_NO_COEF)
+                            div += jump[axis][fidx] * weight;
+                        else
+                            div += jump[axis][fidx] * weight * ic;
+                    }
                 }
                 div -= side * weight * vel[axis][fidx];
             }
@@ -228,6 +259,7 @@ int orc_pressure_apply(int nx, int ny, int nz, const double *phi, const double *
                         double theta = phi0 / (phi0 - phi1);
                         double pin = id0 >= 0 ? p[id0] : p[id1];
                         double pintf = jump ? jump[axis][fidx] : 0.;
+                        if (g_mutation == MUT_JUMP_SIGN) pintf = -pintf;
                         double ic = 1. / max_001(id0 >= 0 ? theta : 1 - theta);
                         vel[axis][fidx] -= (id0 >= 0 ? +1 : -1) * (pintf - pin) * ic * weight;
                     }
@@ -330,6 +362,46 @@ void orc_magnetic_rhs(int nx, int ny, int nz, double spacing, const double *cons
             rhs[r] += (weight - 1.) * hext[axis][fidx] * (-side) * spacing;
         }
     }
+}
+
+// ||rhs - A x||^2 of MagneticField::BuildMatrix's system without storing A
+// (rows as in orc_magnetic_matrix: row 0 the identity, column 0 dropped),
+// for grids whose CSR would not fit; *bb receives ||rhs||^2.
+double orc_magnetic_residual(int nx, int ny, int nz, const double *const mu[3], const double *rhs,
+                             const double *x, double *bb) {
+    Dims d{{nx, ny, nz}};
+    Dims fd[3] = {face_dims(d, 0), face_dims(d, 1), face_dims(d, 2)};
+    int64_t total = (int64_t)nx * ny * nz;
+    double rr = 0.0, b2 = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rr, b2)
+    for (int64_t r = 0; r < total; r++) {
+        double ax;
+        if (r == 0) {
+            ax = x[0];
+        } else {
+            int cell[3];
+            coord_of(d, r, cell);
+            double diag = 0., off = 0.;
+            for (int i = 0; i < 6; i++) {
+                int axis = axis_of(i), side = side_of(i);
+                int nb[3] = {cell[0], cell[1], cell[2]};
+                nb[axis] += side;
+                if (!d.valid(nb[0], nb[1], nb[2])) continue;
+                int face[3] = {cell[0], cell[1], cell[2]};
+                face[axis] += (i & 1);
+                double weight = mu[axis][fd[axis].index(face[0], face[1], face[2])];
+                int64_t c = d.index(nb[0], nb[1], nb[2]);
+                if (c != 0) off += weight * x[c];
+                diag += weight;
+            }
+            ax = diag * x[r] - off;
+        }
+        double e = rhs[r] - ax;
+        rr += e * e;
+        b2 += rhs[r] * rhs[r];
+    }
+    *bb = b2;
+    return rr;
 }
 
 void orc_magnetic_total_field(int nx, int ny, int nz, double spacing, const double *potential,

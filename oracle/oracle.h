@@ -22,6 +22,11 @@
 extern "C" {
 #endif
 
+/* Planted mistakes for the oracle's closed-form tests: m selects one changed
+ * expression (see the Mutation enum in assembly.cpp), 0 restores the
+ * faithful restatement. Process-wide; the tests reset it after use. */
+void orc_set_mutation(int m);
+
 /* Pressure::SetUnKnowns (core/Pressure.cpp:104-123): liquid <=> phi <= 0,
  * rows numbered in linear-index order. grid2mat[cell] = row or -1.
  * Returns the number of unknowns. */
@@ -66,6 +71,12 @@ void orc_magnetic_rhs(int nx, int ny, int nz, double spacing, const double *cons
                       const double *const hext[3], double *rhs);
 
 /* MagneticField::AssignTotalField (core/MagneticField.cpp:125-143). */
+/* ||rhs - A x||_2^2 of the BuildMatrix system (core/MagneticField.cpp:62-90)
+ * computed row by row without the CSR (grids whose matrix would not fit);
+ * *bb receives ||rhs||_2^2. */
+double orc_magnetic_residual(int nx, int ny, int nz, const double *const mu[3],
+                             const double *rhs, const double *x, double *bb);
+
 void orc_magnetic_total_field(int nx, int ny, int nz, double spacing, const double *potential,
                               const double *const hext[3], double *const htotal[3]);
 

@@ -45,6 +45,8 @@ def lib():
         L.orc_magnetic_matrix.argtypes = [C.c_int] * 3 + [P3, _ip, _ip, _dp]
         L.orc_magnetic_rhs.restype = None
         L.orc_magnetic_rhs.argtypes = [C.c_int] * 3 + [C.c_double, P3, P3, _dp]
+        L.orc_magnetic_residual.restype = C.c_double
+        L.orc_magnetic_residual.argtypes = [C.c_int] * 3 + [P3, _dp, _dp, _dp]
         L.orc_magnetic_total_field.restype = None
         L.orc_magnetic_total_field.argtypes = [C.c_int] * 3 + [C.c_double, _dp, P3, P3]
         L.orc_magnetic_vertex_pressures.restype = None
@@ -73,8 +75,31 @@ def lib():
         L.orc_collider_enforce.restype = None
         L.orc_collider_enforce.argtypes = [C.c_int] * 3 + [C.c_double, _dp, P3, P3, P3, P3]
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_mutation.restype = None
+        L.orc_set_mutation.argtypes = [C.c_int]
         _LIB = L
     return _LIB
+
+
+# Planted mistakes (orc_set_mutation in assembly.cpp): each changes one
+# expression of the restatement so that a closed-form test must fail.
+MUTATIONS = {"heaviside_no_sin": 1, "heaviside_half_band": 2, "jump_sign": 3,
+             "jump_no_coef": 4, "no_clamp": 5, "trilerp_swap": 6, "face_origin": 7}
+
+
+class mutated:
+    """``with oracle.mutated("jump_sign"): ...`` runs the block on an oracle
+    with that planted mistake (None: the faithful one)."""
+
+    def __init__(self, name):
+        self.code = 0 if name is None else MUTATIONS[name]
+
+    def __enter__(self):
+        lib().orc_set_mutation(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_mutation(0)
 
 
 def _d(a: np.ndarray):
@@ -199,6 +224,31 @@ def magnetic_system(sc, mu=None):
     hp, k2 = _p3(sc["hext"])
     lib().orc_magnetic_rhs(nx, ny, nz, geo.spacing, mp, hp, _d(rhs))
     return dict(mu=mu, A=A, rhs=rhs, n=n)
+
+
+def magnetic_rhs(sc, mu):
+    """MagneticField::AssembleRHS alone (N^3 doubles), for grids whose CSR
+    matrix would not fit in memory."""
+    geo = sc["geo"]
+    nx, ny, nz = geo.n
+    rhs = np.zeros(geo.ncells)
+    mp, k1 = _p3(mu)
+    hp, k2 = _p3(sc["hext"])
+    lib().orc_magnetic_rhs(nx, ny, nz, geo.spacing, mp, hp, _d(rhs))
+    return rhs
+
+
+def magnetic_rel_residual(sc, mu, rhs, x):
+    """||rhs - A x||_2 / ||rhs||_2 of the magnetic system, row by row without
+    the CSR (any grid size)."""
+    geo = sc["geo"]
+    nx, ny, nz = geo.n
+    mp, k1 = _p3(mu)
+    bb = C.c_double(0)
+    rr = lib().orc_magnetic_residual(nx, ny, nz, mp, _d(np.ascontiguousarray(rhs)),
+                                     _d(np.ascontiguousarray(np.asarray(x, np.float64).ravel())),
+                                     C.byref(bb))
+    return float(np.sqrt(rr / bb.value)) if bb.value > 0 else float(np.sqrt(rr))
 
 
 def magnetic_total_field(sc, potential):

@@ -47,9 +47,13 @@ void Pressure::Reproject(SGridData<double> &velocity, GridData<double> const &le
 void Pressure::Solve(SGridData<double> &velocity, GridData<double> const &levelSet,
                      Collider const &collider, double volError, bool reproject) {
     // The callback is host code: evaluate it once per ghost-fluid face (open,
-    // exactly one liquid neighbour), at theta = phi0 / (phi0 - phi1), which is
-    // the value both reference call sites pass for that face
-    // (core/Pressure.cpp:61-64,154-159).
+    // exactly one liquid neighbour), at theta = phi0 / (phi0 - phi1). The
+    // reference calls it twice per such face: ApplyProjection passes this
+    // theta (core/Pressure.cpp:154-159); BuildProjectionMatrix passes it for a
+    // face on the cell's + side and 1 - phi1 / (phi1 - phi0) for one on its -
+    // side (:61-64), equal in exact arithmetic but not always in the last bit.
+    // A callback that ignores theta, like the simulation's own
+    // (core/Simulation.cpp:216-235), gets identical values either way.
     bool const useJump = static_cast<bool>(m_PressureJump);
     if (useJump) {
         for (int axis = 0; axis < 3; axis++) {

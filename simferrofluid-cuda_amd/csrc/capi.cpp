@@ -114,20 +114,27 @@ void pressure_dev(sfp_context ctx, double *const vel[3], const double *phi,
     Level &L = sys.fine();
     SFP_HIP(hipEventRecord(ctx->ev[0], s));
     if (!reproject) {
+        // the assembly overwrites the liquid mask and the fine operator: until
+        // the new system is complete (checked, counted, coarsened) a
+        // reprojection has nothing consistent to reuse
+        ctx->pressure_ready = false;
+        ctx->pressure_unknowns = 0;
         double *c64[3] = {L.cpl64[0].p, L.cpl64[1].p, L.cpl64[2].p};
         float *c32[3] = {L.cpl32[0].p, L.cpl32[1].p, L.cpl32[2].p};
         launch_pressure_assemble(s, d, phi, F, V, U, jump ? &J : nullptr, vol_error, L.diag64.p, c64,
                                  sys.rhs().p, L.diag32.p, c32, ctx->liquid.p, ctx->ws.err.p);
         check_err(ctx, "open (fraction < 1) face on the grid boundary next to a liquid cell");
-        ctx->pressure_unknowns = sys.count_active(ctx->ws);
-        ctx->pressure_ready = true;
-        if (ctx->pressure_unknowns == 0) {  // Pressure::Project returns early
+        const long long unknowns = sys.count_active(ctx->ws);
+        if (unknowns == 0) {  // Pressure::Project returns early
+            ctx->pressure_ready = true;
             if (pressure) SFP_HIP(hipMemsetAsync(pressure, 0, d.cells() * sizeof(double), s));
             SFP_HIP(hipStreamSynchronize(s));
             fill_stats(st, SolveInfo{0, 0.0, true, 0, ctx->cfg.precond}, 0, 0.f, 0.f, 0.f);
             return;
         }
         sys.build_hierarchy(ctx->ws, ctx->cfg);
+        ctx->pressure_unknowns = unknowns;
+        ctx->pressure_ready = true;
     } else {
         if (!ctx->pressure_ready) throw StateError("reproject before project");
         if (ctx->pressure_unknowns == 0) {

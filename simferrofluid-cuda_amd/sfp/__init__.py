@@ -263,6 +263,33 @@ class Context:
         if st != OK:
             raise SfpError(st, lib().sfp_last_error(self._h).decode())
 
+    # The library copies cells() doubles per cell array and (n + e_a) n^2 per
+    # face array of axis a, whatever the caller's buffer holds: host arrays
+    # are checked against the context's grid before any library call.
+    def _face_shape(self, a):
+        return tuple(v + (1 if i == a else 0) for i, v in enumerate(self.n))
+
+    def _cells(self, a, what):
+        if a is not None and tuple(np.shape(a)) != self.n:
+            raise ValueError(f"{what}: shape {tuple(np.shape(a))}, the grid is {self.n}")
+        return a
+
+    def _faces(self, arrs, what):
+        if arrs is None:
+            return arrs
+        if len(arrs) != 3:
+            raise ValueError(f"{what}: need 3 face arrays, got {len(arrs)}")
+        for a, v in enumerate(arrs):
+            if tuple(np.shape(v)) != self._face_shape(a):
+                raise ValueError(f"{what}[{a}]: shape {tuple(np.shape(v))}, face grid {a} is "
+                                 f"{self._face_shape(a)}")
+        return arrs
+
+    def _vertices(self, a, what):
+        if np.ndim(a) != 2 or np.shape(a)[1] != 3:
+            raise ValueError(f"{what}: need an (nverts, 3) array, got shape {tuple(np.shape(a))}")
+        return a
+
     @property
     def handle(self):
         return self._h
@@ -287,6 +314,11 @@ class Context:
     def pressure_project(self, sc, reproject=False, want_pressure=True):
         """Pressure::Project on copies of sc's velocities. Returns
         (velocity[3], pressure N^3 or None, stats dict)."""
+        self._faces(sc["velocity"], "velocity")
+        self._cells(sc["phi"], "phi")
+        self._faces(sc["fraction"], "fraction")
+        self._faces(sc["solid_velocity"], "solid_velocity")
+        self._faces(sc.get("jump"), "jump")
         vel = [np.array(v, dtype=np.float64, copy=True, order="C") for v in sc["velocity"]]
         phi = np.ascontiguousarray(sc["phi"], dtype=np.float64)
         frac = [np.ascontiguousarray(a, dtype=np.float64) for a in sc["fraction"]]
@@ -303,11 +335,12 @@ class Context:
 
     def magnetic_solve(self, sc, chi=None, band=None, want_field=True):
         """MagneticField::Solve. Returns (potential N^3, total_field[3] or None, stats)."""
-        g = sc["geo"]
+        self._cells(sc["phi"], "phi")
+        self._faces(sc["hext"], "hext")
         phi = np.ascontiguousarray(sc["phi"], dtype=np.float64)
         hext = [np.ascontiguousarray(a, dtype=np.float64) for a in sc["hext"]]
         pot = np.zeros(phi.shape)
-        H = [np.zeros(g.face_shape(a)) for a in range(3)] if want_field else None
+        H = [np.zeros(self._face_shape(a)) for a in range(3)] if want_field else None
         st = Stats()
         self._check(lib().sfp_magnetic_solve(
             self._h, _d(phi), _p3(hext), float(sc["chi"] if chi is None else chi),
@@ -316,6 +349,10 @@ class Context:
         return pot, H, st.as_dict()
 
     def vertex_pressures(self, positions, normals, chi):
+        self._vertices(positions, "positions")
+        self._vertices(normals, "normals")
+        if np.shape(normals) != np.shape(positions):
+            raise ValueError(f"normals: shape {np.shape(normals)}, positions {np.shape(positions)}")
         positions = np.ascontiguousarray(positions, dtype=np.float64)
         normals = np.ascontiguousarray(normals, dtype=np.float64)
         nv = positions.shape[0]
@@ -327,6 +364,8 @@ class Context:
     # ------------------------------------------------------------ after the projection
     def extrapolate_faces(self, data, valid, clear_value=0.0, steps=6):
         """Extrapolation::Solve on the three face grids; returns new arrays."""
+        self._faces(data, "data")
+        self._faces(valid, "valid")
         out = [np.array(a, dtype=np.float64, copy=True, order="C") for a in data]
         masks = [np.ascontiguousarray(m, dtype=np.uint8) for m in valid]
         mp = (C.POINTER(C.c_ubyte) * 3)(*[m.ctypes.data_as(C.POINTER(C.c_ubyte)) for m in masks])
@@ -335,6 +374,9 @@ class Context:
 
     def extrapolate_velocity(self, vel, phi, fraction, steps=6):
         """The velocity extrapolation of Simulation::ProjectVelocity; returns new arrays."""
+        self._faces(vel, "velocity")
+        self._cells(phi, "phi")
+        self._faces(fraction, "fraction")
         out = [np.array(a, dtype=np.float64, copy=True, order="C") for a in vel]
         phi = np.ascontiguousarray(phi, dtype=np.float64)
         frac = [np.ascontiguousarray(a, dtype=np.float64) for a in fraction]
@@ -343,6 +385,10 @@ class Context:
 
     def collider_enforce(self, vel, fraction, normal, collider_velocity):
         """Collider::Enforce; returns new arrays."""
+        self._faces(vel, "velocity")
+        self._faces(fraction, "fraction")
+        self._faces(normal, "normal")
+        self._faces(collider_velocity, "collider_velocity")
         out = [np.array(a, dtype=np.float64, copy=True, order="C") for a in vel]
         arrs = [[np.ascontiguousarray(a, dtype=np.float64) for a in g]
                 for g in (fraction, normal, collider_velocity)]
@@ -386,6 +432,7 @@ class Context:
         """Network input channels 1..3 on the first coarse level of the last
         assembled system, shape level_dims(which, 1) + (3,)."""
         out = np.zeros(self.level_dims(which, 1) + (3,), np.float32)
+        self._cells(phi, "phi")
         ph = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
         self._check(lib().sfp_learned_features(self._h, int(which), _d(ph), _f(out)))
         return out
@@ -424,18 +471,21 @@ class Context:
         return dict(diag=arrs[0], cpl=arrs[1:])
 
     def apply64(self, which, z):
+        self._cells(z, "z")
         z = np.ascontiguousarray(z, dtype=np.float32)
         w = np.zeros(z.shape)
         self._check(lib().sfp_test_apply64(self._h, which, _f(z), _d(w)))
         return w
 
     def precondition(self, which, r):
+        self._cells(r, "r")
         r = np.ascontiguousarray(r, dtype=np.float64)
         z = np.zeros(r.shape, np.float32)
         self._check(lib().sfp_test_precondition(self._h, which, _d(r), _f(z)))
         return z
 
     def residual(self, which, x):
+        self._cells(x, "x")
         x = np.ascontiguousarray(x, dtype=np.float64)
         r = np.zeros(x.shape)
         rr = C.c_double(0)
@@ -446,6 +496,9 @@ class Context:
     def solve_rhs(self, which, rhs, level=0):
         """x = A_level^-1 rhs (level 0: the assembled system; from x = 0);
         returns (x, stats)."""
+        want = self.n if level == 0 else self.level_dims(which, level)
+        if tuple(np.shape(rhs)) != tuple(want) and np.size(rhs) != int(np.prod(want)):
+            raise ValueError(f"rhs: shape {tuple(np.shape(rhs))}, level {level} is {tuple(want)}")
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
         x = np.zeros(rhs.shape)
         st = Stats()
@@ -456,6 +509,7 @@ class Context:
     def restrict_smoothed(self, which, r):
         """The residual one V-cycle leaves for right-hand side r, restricted
         to the first coarse level (FP32, coarse dims)."""
+        self._cells(r, "r")
         r = np.ascontiguousarray(r, dtype=np.float64)
         rc = np.zeros(self.level_dims(which, 1), np.float32)
         self._check(lib().sfp_test_restrict_smoothed(self._h, int(which), _d(r), _f(rc)))

@@ -1,0 +1,212 @@
+"""GPU tests of the full-size workloads, through the C ABI, checked against the
+FP64 oracle or, where the oracle's CSR would not fit, against properties
+that do not depend on size:
+
+  * the 256^3 magnetic potential on the cosine-perturbed interface for
+    chi = 0.5, 2 and 8 (MagneticField::Solve, core/MagneticField.cpp:29-35,
+    120-123): relative residual on the oracle's own CSR system, L-infinity
+    distance to the oracle's FP64 CG-AMG solution, total field bit for bit;
+  * the learned preconditioner at 256^3 with the package's trained weights on
+    the pressure projection (core/Pressure.cpp:125-135) and the three
+    magnetic systems: converged on the oracle's system, equal to the plain
+    MG-PCG solution within the tolerance's reach, fall-back counts consistent;
+  * the 512^3 spike sequence, 20 solves alternating pressure projection and
+    magnetic potential on a moving interface, magnetic warm-started, with the
+    plain and the learned preconditioner: every pressure solve leaves a
+    per-cell divergence equal to volError within tol ||b|| (the discrete
+    divergence after the update is the residual, K3), every magnetic solve a
+    relative residual <= tol on the oracle's system assembled row by row.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import sfp
+from sfp import scenes
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-8
+
+
+def _trained():
+    from sfp import learn
+    if not os.path.exists(learn.WEIGHTS):
+        pytest.skip("no trained weights in the package")
+    params, meta = learn.load_weights()
+    return params, learn.smoother(meta)
+
+
+def _learned_context(sc, params, smoother, **kw):
+    ctx = sfp.Context.for_scene(sc, tol=TOL, max_iters=200, **kw)
+    ctx.set_network(params)
+    ctx.set_params(precond=sfp.PRECOND_LEARNED, **smoother)
+    return ctx
+
+
+# ------------------------------------------------------------------ 256^3 magnetic, chi sweep
+@pytest.mark.parametrize("chi", [0.5, 2.0, 8.0])
+def test_config2_magnetic_256_vs_oracle(oracle, chi):
+    print(f"[case] magnetic 256^3 chi={chi} vs oracle", flush=True)
+    sc = scenes.magnetic_scene(256, seed=0, chi=chi)
+    with sfp.Context.for_scene(sc, tol=TOL, max_iters=200) as ctx:
+        pot, H, st = ctx.magnetic_solve(sc)
+    assert st["converged"] == 1
+    ref = oracle.magnetic_system(sc)
+    rel = oracle.rel_residual(ref["A"], ref["rhs"], pot.ravel())
+    xr, it, relr, _, _ = oracle.solve(ref["A"], ref["rhs"], "cg", "amg", tol=1e-11, maxit=300)
+    assert relr <= 1e-11
+    linf = np.abs(pot.ravel() - xr).max() / np.abs(xr).max()
+    print(f"  gpu iters={st['iterations']} oracle rel={rel:.2e} (oracle CG {it} its) "
+          f"Linf rel={linf:.2e}", flush=True)
+    assert rel <= TOL * 1.01
+    assert linf < 1e-5
+    H_ref = oracle.magnetic_total_field(sc, pot)
+    for a in range(3):
+        assert np.array_equal(H[a], H_ref[a])
+
+
+# ------------------------------------------------------------------ learned FCG at 256^3
+@pytest.mark.parametrize("system", ["pressure", "m0.5", "m2", "m8"])
+def test_config3_learned_256(oracle, system):
+    params, sm = _trained()
+    print(f"[case] learned FCG 256^3 {system}", flush=True)
+    if system == "pressure":
+        sc = scenes.pressure_scene(256, seed=0)
+        with _learned_context(sc, params, sm) as ctx:
+            vel_l, p_l, st_l = ctx.pressure_project(sc)
+        with sfp.Context.for_scene(sc, tol=TOL, max_iters=200) as ctx:
+            vel_m, p_m, st_m = ctx.pressure_project(sc)
+        ref = oracle.pressure_system(sc)
+        x_l = p_l.ravel()[ref["mat2grid"]]
+        x_m = p_m.ravel()[ref["mat2grid"]]
+        rel = oracle.rel_residual(ref["A"], ref["rhs"], x_l)
+        ref_vel = oracle.pressure_apply(sc, ref["grid2mat"], x_l)
+        for a in range(3):
+            assert np.array_equal(vel_l[a], ref_vel[a])
+    else:
+        sc = scenes.magnetic_scene(256, seed=0, chi=float(system[1:]))
+        with _learned_context(sc, params, sm) as ctx:
+            pot_l, _, st_l = ctx.magnetic_solve(sc, want_field=False)
+        with sfp.Context.for_scene(sc, tol=TOL, max_iters=200) as ctx:
+            pot_m, _, st_m = ctx.magnetic_solve(sc, want_field=False)
+        ref = oracle.magnetic_system(sc)
+        x_l, x_m = pot_l.ravel(), pot_m.ravel()
+        rel = oracle.rel_residual(ref["A"], ref["rhs"], x_l)
+    linf = np.abs(x_l - x_m).max() / np.abs(x_m).max()
+    print(f"  learned: {st_l['iterations']} its ({st_l['fallback']} after fall-back, ends with "
+          f"precond {st_l['precond_used']}), oracle rel {rel:.2e}; MG-PCG {st_m['iterations']} its; "
+          f"Linf(learned - MG) rel {linf:.2e}", flush=True)
+    assert st_l["converged"] == 1 and st_m["converged"] == 1
+    assert rel <= TOL * 1.01
+    assert linf <= 1e-6
+    # fall-back bookkeeping: iterations after the switch are MG iterations,
+    # and a solve that switched ends on the V-cycle
+    assert 0 <= st_l["fallback"] <= st_l["iterations"]
+    if st_l["fallback"] > 0:
+        assert st_l["precond_used"] == sfp.PRECOND_MG
+    else:
+        assert st_l["precond_used"] == sfp.PRECOND_LEARNED
+
+
+# ------------------------------------------------------------------ 512^3 spike sequence
+def _smoothed_velocities(geo, dev, seed=0):
+    """U[-1,1] face velocities from the counter-based generator, then 3 passes
+    of the clamped 3x3x3 box mean (sfp.scenes._box_smooth) done on the GPU."""
+    import torch
+    import torch.nn.functional as F
+    out = []
+    for a in range(3):
+        shp = geo.face_shape(a)
+        u = torch.from_numpy(scenes.uniform(seed, scenes.ID_VEL + a, int(np.prod(shp))))
+        u = (u.to(dev) * 2.0 - 1.0).reshape(1, 1, *shp)
+        for _ in range(3):
+            u = F.avg_pool3d(F.pad(u, (1, 1, 1, 1, 1, 1), mode="replicate"), 3, stride=1)
+        out.append(u.reshape(shp).contiguous())
+    return out
+
+
+def _divergence_check(vel_new, vel_old, frac, liquid, vol_error):
+    """(||div u_new - volError||_2, ||div u_old - volError||_2) over liquid
+    cells, with the solid flux (1 - w) u_solid = 0 and w in {0, 1}: the first
+    is the pressure system's residual, the second its right-hand side."""
+    import torch
+
+    def div(vel):
+        out = torch.zeros(liquid.shape, dtype=torch.float64, device=liquid.device)
+        for a in range(3):
+            flux = (1.0 - frac[a]) * vel[a]
+            hi = [slice(None)] * 3
+            lo = [slice(None)] * 3
+            hi[a] = slice(1, None)
+            lo[a] = slice(0, -1)
+            out += flux[tuple(hi)] - flux[tuple(lo)]
+        return out
+
+    dn = (div(vel_new) - vol_error)[liquid]
+    do = (div(vel_old) - vol_error)[liquid]
+    return float(torch.linalg.vector_norm(dn)), float(torch.linalg.vector_norm(do))
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("leg", ["mg", "learned"])
+def test_config4_spike_512_sequence(oracle, leg):
+    import torch
+    dev = torch.device("cuda:0")
+    n = 512
+    print(f"[case] spike sequence 512^3, {leg} preconditioner", flush=True)
+    msc = scenes.spike_scene(n, seed=0, chi=4.0)
+    geo = msc["geo"]
+    vel0 = _smoothed_velocities(geo, dev)
+    frac = [torch.from_numpy(f).to(dev) for f in scenes.wall_fractions(geo)]
+    us = [torch.zeros(geo.face_shape(a), dtype=torch.float64, device=dev) for a in range(3)]
+    hext = [torch.from_numpy(h).to(dev) for h in msc["hext"]]
+    modes = scenes._modes(0, 64, 0.005, mmax=12)
+    phi_host = msc["phi"]
+    phi = torch.from_numpy(phi_host).to(dev)
+    vel = [v.clone() for v in vel0]
+    out = torch.zeros(geo.n, dtype=torch.float64, device=dev)
+    ptr = lambda ts: [t.data_ptr() for t in ts]  # noqa: E731
+    if leg == "learned":
+        params, sm = _trained()
+        ctx = _learned_context(msc, params, sm)
+    else:
+        ctx = sfp.Context.for_scene(msc, tol=TOL, max_iters=200)
+    torch.cuda.synchronize()
+    iters = []
+    try:
+        for step in range(20):
+            if step % 2 == 0 and step > 0:
+                a, mm, nn, th = modes
+                phi_host = scenes._wavy_phi(geo, 0.2, (a, mm, nn, th + 0.1 * (step // 2)))
+                phi.copy_(torch.from_numpy(phi_host))
+            if step % 2 == 0:
+                for v, v0 in zip(vel, vel0):
+                    v.copy_(v0)
+                st = ctx.pressure_project_device(ptr(vel), phi.data_ptr(), ptr(frac), ptr(us), None,
+                                                 0.0)
+                torch.cuda.synchronize()
+                r_norm, b_norm = _divergence_check(vel, vel0, frac, phi <= 0, 0.0)
+                rel = r_norm / b_norm
+                kind = "pressure"
+            else:
+                st = ctx.magnetic_solve_device(phi.data_ptr(), ptr(hext), 4.0, 1.5 * geo.spacing,
+                                               out.data_ptr())
+                torch.cuda.synchronize()
+                sc = dict(msc, phi=phi_host)
+                mu = oracle.magnetic_coeff(sc)
+                rhs = oracle.magnetic_rhs(sc, mu)
+                rel = oracle.magnetic_rel_residual(sc, mu, rhs, out.cpu().numpy())
+                del mu, rhs
+                kind = "magnetic"
+            iters.append(st["iterations"])
+            print(f"  step {step} {kind}: {st['iterations']} its (fallback {st['fallback']}), "
+                  f"library rel {st['rel_residual']:.2e}, checked rel {rel:.2e}", flush=True)
+            assert st["converged"] == 1, (step, st)
+            assert rel <= TOL * 1.01, (step, kind, rel)
+    finally:
+        ctx.close()
+    # warm starts: a magnetic solve after the first needs no more iterations
+    # than the cold one plus a margin (the interface moves by 0.1 rad a pair)
+    assert max(iters[3::2]) <= iters[1] + 2

@@ -229,6 +229,49 @@ def test_open_boundary_face_next_to_liquid_is_rejected():
         assert st["converged"] == 1
 
 
+def test_reproject_after_a_failed_project_is_a_state_error():
+    """A project that fails its boundary check has overwritten the liquid mask
+    and the fine operator: a following reprojection must not solve that
+    half-built system (it reports SFP_ERR_STATE) until a project succeeds."""
+    good = scenes.pressure_scene(12, seed=1)
+    bad = scenes.pressure_scene(12, seed=1)
+    bad["fraction"][0][0, :, :] = 0.0
+    with sfp.Context.for_scene(good) as ctx:
+        ctx.pressure_project(good)
+        ctx.pressure_project(good, reproject=True)
+        with pytest.raises(sfp.SfpError) as e:
+            ctx.pressure_project(bad)
+        assert e.value.status == sfp.ERR_INPUT
+        with pytest.raises(sfp.SfpError) as e:
+            ctx.pressure_project(good, reproject=True)
+        assert e.value.status == sfp.ERR_STATE
+        ctx.pressure_project(good)
+        vel, p, st = ctx.pressure_project(good, reproject=True)
+        assert st["converged"] == 1
+
+
+@pytest.mark.parametrize("J", [0.37, -0.37])
+def test_uniform_jump_gives_uniform_liquid_pressure_on_gpu(J):
+    """Closed form of the ghost-fluid jump (core/Pressure.cpp:64,159-163):
+    with u = 0, closed walls and the same jump J on every face, p = J on every
+    liquid cell and the update leaves u = 0; the sign of J is the sign of p."""
+    n = 48
+    print(f"[case] uniform jump J={J} n={n}", flush=True)
+    sc = scenes.pressure_scene(n, seed=3)
+    geo = sc["geo"]
+    sc["velocity"] = [np.zeros(geo.face_shape(a)) for a in range(3)]
+    sc["jump"] = [np.full(geo.face_shape(a), J) for a in range(3)]
+    with sfp.Context.for_scene(sc, tol=1e-12) as ctx:
+        vel, p, st = ctx.pressure_project(sc)
+    liq = sc["phi"] <= 0
+    assert st["converged"] == 1
+    # solved to 1e-12: p within kappa * 1e-12 of J; the update scales the
+    # difference by up to 1/.001 at ghost-fluid faces
+    assert np.abs(p[liq] - J).max() <= 1e-7 * abs(J)
+    assert np.all(p[~liq] == 0)
+    assert max(np.abs(v).max() for v in vel) <= 1e-4 * abs(J)
+
+
 def test_call_order_errors():
     sc = scenes.pressure_scene(8, seed=1)
     with sfp.Context.for_scene(sc) as ctx:

@@ -155,3 +155,45 @@ def test_apply_full_matches_csr():
     x = rng.normal(size=shape)
     A = csr_from_full_grid(diag, cpl)
     np.testing.assert_allclose(learn.apply_full(diag, cpl, x).ravel(), A @ x.ravel(), rtol=1e-14, atol=1e-14)
+
+
+def _unbound_context(n):
+    """A Context object that never reached sfp_create: any library call on it
+    fails, so a ValueError proves the check ran before the call."""
+    ctx = object.__new__(sfp.Context)
+    ctx.n = tuple(n)
+    ctx._h = C.c_void_p()
+    return ctx
+
+
+def test_wrapper_rejects_arrays_that_do_not_fit_the_grid():
+    ctx = _unbound_context((6, 5, 4))
+    good = scenes.random_pressure_scene((6, 5, 4), seed=1)
+    bad_cells = scenes.random_pressure_scene((6, 5, 5), seed=1)
+    wrong_phi = dict(good, phi=good["phi"][:, :, :3])
+    wrong_face = dict(good, velocity=[good["velocity"][0], good["velocity"][2], good["velocity"][1]])
+    for sc in (bad_cells, wrong_phi, wrong_face, dict(good, fraction=good["fraction"][:2])):
+        with pytest.raises(ValueError):
+            ctx.pressure_project(sc)
+    msc = dict(good, hext=[np.zeros(s) for s in ((7, 5, 4), (6, 6, 4), (6, 5, 4))], chi=2.0, band=0.1)
+    with pytest.raises(ValueError):
+        ctx.magnetic_solve(msc)
+    with pytest.raises(ValueError):
+        ctx.magnetic_solve(dict(msc, phi=np.zeros((6, 5, 3))))
+    with pytest.raises(ValueError):
+        ctx.vertex_pressures(np.zeros((4, 2)), np.zeros((4, 2)), 1.0)
+    with pytest.raises(ValueError):
+        ctx.vertex_pressures(np.zeros((4, 3)), np.zeros((3, 3)), 1.0)
+    faces = [np.zeros(ctx._face_shape(a)) for a in range(3)]
+    with pytest.raises(ValueError):
+        ctx.extrapolate_faces(faces, [np.zeros((6, 5, 4), np.uint8)] * 3)
+    with pytest.raises(ValueError):
+        ctx.extrapolate_velocity(faces, np.zeros((5, 5, 4)), faces)
+    with pytest.raises(ValueError):
+        ctx.collider_enforce(faces, faces, faces[::-1], faces)
+    for hook in (ctx.apply64, ctx.precondition, ctx.residual):
+        with pytest.raises(ValueError):
+            hook(0, np.zeros((6, 5, 5)))
+    # arrays that fit get past the checks and reach the (unbound) library
+    with pytest.raises(sfp.SfpError):
+        ctx.pressure_project(good)

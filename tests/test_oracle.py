@@ -329,6 +329,20 @@ def test_magnetic_diagonal_is_ordered_coupling_sum(oracle, name):
             assert acc == diag[r], r
 
 
+@pytest.mark.parametrize("name", list(MSCENES))
+def test_matrix_free_magnetic_residual_matches_csr(oracle, name):
+    """The row-by-row residual used at sizes whose CSR does not fit agrees
+    with the CSR system's, and with its right-hand side."""
+    sc = MSCENES[name]
+    ref = oracle.magnetic_system(sc)
+    assert np.array_equal(oracle.magnetic_rhs(sc, ref["mu"]), ref["rhs"])
+    rng = np.random.default_rng(5)
+    for x in (rng.standard_normal(ref["n"]), spla.spsolve(ref["A"].tocsc(), ref["rhs"])):
+        r_csr = oracle.rel_residual(ref["A"], ref["rhs"], x)
+        r_mf = oracle.magnetic_rel_residual(sc, ref["mu"], ref["rhs"], x)
+        assert abs(r_mf - r_csr) <= 1e-12 * max(r_csr, 1e-16) + 1e-15
+
+
 def test_k1_planar_interface_gives_exact_flux(oracle):
     n = 12
     geo = scenes.Geometry(n, 1.0 / n)
