@@ -14,14 +14,20 @@ velocities, so each step is a full solve.
 Prints one JSON line (see the driver contract); `value` is the mean seconds
 per solve of plain MG-PCG (the library's default). `learned_fcg_s` is the
 same solves with the learned preconditioner (the package's trained U-Net
-weights) over the same step count, and `learned_over_mg_time` their ratio.
-The `kernels` object times the hot fine-level kernels with HIP events on the
-library's stream, over one more pass of each magnetic system (all cells
-active; pressure solves skip wavefronts of air cells) after the timed steps;
-`roofline` is the one of them with the most time. The `cpu_baseline` object times the FP64 CPU oracle
-(smoothed-aggregation AMG + SPAI-0, BiCGStab for pressure, CG for magnetic,
-hierarchy rebuilt per solve) on one pressure and one magnetic solve at the
-same size on the host cores.
+weights and the fall-back threshold recorded with them) over the same step
+count, and `learned_over_mg_time` their ratio; `unet_roofline` times the
+network alone at the first coarse level (HIP events per convolution, FLOPs
+against the BF16 MFMA peak). The `kernels` object times the hot fine-level
+kernels with HIP events on the library's stream, over one more pass of each
+magnetic system (all cells active; pressure solves skip wavefronts of air
+cells) after the timed steps; `roofline` is the one of them with the most
+time. The `cpu_baseline` object times the FP64 CPU oracle (smoothed-
+aggregation AMG + SPAI-0, BiCGStab for pressure, CG for magnetic, hierarchy
+rebuilt per solve) on one pressure and one magnetic solve at the same size on
+the host cores (median of 5 after a warm-up; also at the magnetic solve's
+as-shipped 1e-6), and `checks` holds, for every system and both
+preconditioners, the GPU solution's relative residual on the oracle's own
+system and its L-infinity distance to the oracle's solution at 1e-11.
 """
 from __future__ import annotations
 
@@ -387,15 +393,18 @@ def main():
         d["iterations"].append(st["iterations"])
         d["rel_residual"].append(st["rel_residual"])
 
+    # the solution of every system, for the checks against the oracle
+    solutions = {"mg": solve_each(ctx, systems, refill_ring, run_step, pressure_out, potential_out)}
+
     learned = None
     if not args.no_learned:
-        learned = learned_leg(args, systems, total, run_step, refill_ring, geo, by_sys, log)
+        learned = learned_leg(args, systems, total, run_step, refill_ring, geo, by_sys, log,
+                              solutions, pressure_out, potential_out)
 
     cpu = None
     checks = {}
     if not args.no_cpu_baseline:
-        cpu, checks = cpu_leg(psc, msc, pressure_out, potential_out, ctx, phi_p, frac, us, vel0,
-                              phi_m, hext, geo, args, log)
+        cpu, checks = cpu_leg(psc, msc, systems, solutions, geo, args, log)
 
     out = {
         "metric": METRIC,
@@ -423,6 +432,7 @@ def main():
         "per_system": by_sys,
         "kernels": kernels,
         "roofline": roof,
+        "unet_roofline": None if learned is None else learned["unet_roofline"],
         "cpu_baseline": cpu,
         "checks": checks,
     }
@@ -501,7 +511,52 @@ def spike512(args):
     print(json.dumps(res), flush=True)
 
 
-def learned_leg(args, systems, total, run_step, refill_ring, geo, mg_by_sys, log):
+def solve_each(ctx, systems, refill_ring, run_step, pressure_out, potential_out):
+    """One more solve of each system on `ctx` (pristine velocities, cold
+    magnetic solves): {system: (host solution N^3, stats)}."""
+    import torch
+    out = {}
+    for i, (name, chi) in enumerate(systems):
+        refill_ring()
+        _, st = run_step(i, ctx)
+        torch.cuda.synchronize()
+        out[name] = ((pressure_out if chi is None else potential_out).cpu().numpy().copy(), st)
+    return out
+
+
+def unet_roofline(params, n, log):
+    """The network of the learned cycle alone on one n^3 volume (the first
+    coarse level of the 2n^3 solve), timed with HIP events per convolution:
+    FLOPs per convolution = 2 * 27 * Cin * Cout * voxels of its level."""
+    import numpy as np
+
+    import sfp
+    cin = [8, 16, 16, 32, 32, 64, 64, 32, 32, 16]    # as the network stores them: the
+    cout = [16, 16, 32, 32, 64, 64, 32, 32, 16, 16]  # input convolution has 8 (4 used)
+    level = [0, 0, 1, 1, 2, 2, 1, 1, 0, 0]
+    with sfp.UNet(1, n, n, n, training=False) as net:
+        net.set_params(params)
+        x = np.random.default_rng(0).standard_normal((1, n, n, n, 4)).astype(np.float32)
+        net.forward(x)
+        fwd_ms, conv_ms = net.profile_forward(10)
+    flops = [2.0 * 27 * cin[k] * cout[k] * (n >> level[k]) ** 3 for k in range(10)]
+    k = int(np.argmax(conv_ms))
+    rate = flops[k] / (conv_ms[k] * 1e-3) / 1e12
+    total = sum(flops) / (fwd_ms * 1e-3) / 1e12
+    log(f"network {n}^3: {fwd_ms:.3f} ms per pass ({total:.0f} TFLOP/s over its convolutions' FLOPs); "
+        f"dominant conv {k} ({cin[k]}->{cout[k]}): {conv_ms[k] * 1e3:.1f} us, {rate:.0f} TFLOP/s")
+    return {"kernel": f"k_conv3 layer {k} ({cin[k]}->{cout[k]} channels, 3x3x3, BF16 MFMA, FP32 "
+                      f"accumulation) at {n >> level[k]}^3", "bound": "mfma",
+            "achieved": round(rate, 1), "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": round(rate / 2500.0, 4), "flops_per_launch": flops[k],
+            "avg_launch_us": round(conv_ms[k] * 1e3, 2),
+            "network_ms_per_pass": round(fwd_ms, 4),
+            "network_tflops": round(total, 1),
+            "conv_us": [round(v * 1e3, 2) for v in conv_ms]}
+
+
+def learned_leg(args, systems, total, run_step, refill_ring, geo, mg_by_sys, log, solutions,
+                pressure_out, potential_out):
     """The same systems and step count with the learned preconditioner (the
     package's trained weights) in a second context."""
     import torch
@@ -512,7 +567,7 @@ def learned_leg(args, systems, total, run_step, refill_ring, geo, mg_by_sys, log
     params, meta = learn.load_weights()
     lctx = sfp.Context(geo.n, geo.spacing, geo.origin, tol=args.tol, max_iters=200)
     lctx.set_network(params)
-    lctx.set_params(precond=sfp.PRECOND_LEARNED, **learn.smoother(meta))
+    lctx.set_params(**learn.learned_params(meta))
     refill_ring()
     for i in range(args.warmup):
         name, st = run_step(i, lctx)
@@ -527,7 +582,9 @@ def learned_leg(args, systems, total, run_step, refill_ring, geo, mg_by_sys, log
         records.append(st)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+    solutions["learned"] = solve_each(lctx, systems, refill_ring, run_step, pressure_out, potential_out)
     lctx.close()
+    roof = unet_roofline(params, geo.n[0] // 2, log)
     by_sys = {}
     for st in records:
         log(f"learned {st['system']}: iters={st['iterations']} fallback={st['fallback']} "
@@ -540,60 +597,99 @@ def learned_leg(args, systems, total, run_step, refill_ring, geo, mg_by_sys, log
         d["rel_residual"].append(st["rel_residual"])
         d["mg_iterations"] = mg_by_sys.get(st["system"], {}).get("iterations")
     return {"s_per_solve": round(elapsed / args.steps, 6), "per_system": by_sys,
-            "converged_all": all(st["converged"] for st in records),
+            "converged_all": all(st["converged"] for st in records), "unet_roofline": roof,
             "weights": {"sha256": meta["sha256"], "params": meta["count"],
                         "trained_at": meta.get("n"), "steps": meta.get("steps"),
-                        "smoother": learn.smoother(meta)}}
+                        "smoother": learn.smoother(meta),
+                        "stall_ratio": learn.learned_params(meta)["stall_ratio"]}}
 
 
-def cpu_leg(psc, msc, pressure_out, potential_out, ctx, phi_p, frac, us, vel0, phi_m, hext, geo,
-            args, log):
-    """CPU oracle timing plus FP64 checks of the GPU solutions on the oracle's CSR."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(psc, msc, systems, solutions, geo, args, log):
+    """The CPU baseline (the oracle's FP64 assembly + SA-AMG/SPAI-0 setup +
+    BiCGStab for pressure, CG for magnetic, all rebuilt per call as the
+    reference does), and the FP64 checks of every GPU solution on the
+    oracle's own systems: relative residual and L-infinity distance to the
+    oracle's solution at rel. residual 1e-11."""
+    import statistics
+
     import numpy as np
-    import torch
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    ptr = lambda t: t.data_ptr()
+    def pressure_call(tol):
+        t = time.time()
+        ps = oracle.pressure_system(psc)
+        ta = time.time() - t
+        x, it, rel, ts, tv = oracle.solve(ps["A"], ps["rhs"], "bicgstab", "amg", tol=tol, maxit=300)
+        return ps, x, it, ta, ts, tv
+
+    def magnetic_call(chi, tol):
+        t = time.time()
+        ms = oracle.magnetic_system(dict(msc, chi=chi, band=1.5 * geo.spacing))
+        ta = time.time() - t
+        x, it, rel, ts, tv = oracle.solve(ms["A"], ms["rhs"], "cg", "amg", tol=tol, maxit=300)
+        return ms, x, it, ta, ts, tv
+
+    # timing: one warm-up, then the median of 5 calls of each
+    runs = {"pressure_1e-8": lambda: pressure_call(args.tol),
+            "magnetic_chi2_1e-8": lambda: magnetic_call(2.0, args.tol),
+            "magnetic_chi2_1e-6": lambda: magnetic_call(2.0, 1e-6)}
+    for f in runs.values():
+        f()
+    timing = {}
+    for name, f in runs.items():
+        rec = []
+        for _ in range(5):
+            _, _, it, ta, ts, tv = f()
+            rec.append((ta + ts + tv, ta, ts, tv, it))
+        rec.sort()
+        tot, ta, ts, tv, it = rec[2]
+        timing[name] = {"total_s": round(tot, 4), "assembly_s": round(ta, 4), "setup_s": round(ts, 4),
+                        "solve_s": round(tv, 4), "iterations": it,
+                        "all_totals_s": [round(r[0], 4) for r in rec]}
+        log(f"cpu {name}: median {tot:.2f}s (assembly {ta:.2f} setup {ts:.2f} solve {tv:.2f}, {it} its)")
+
+    # checks of every GPU solution against the oracle's system and solution
     checks = {}
-    times = []
-    # pressure: GPU solution of the pristine velocities
-    v = [a.clone() for a in vel0]
-    ctx.pressure_project_device([ptr(a) for a in v], ptr(phi_p), [ptr(a) for a in frac],
-                                [ptr(a) for a in us], None, 0.0, ptr(pressure_out))
-    p_gpu = pressure_out.cpu().numpy()
-    t = time.time()
-    ps = oracle.pressure_system(psc)
-    t_asm = time.time() - t
-    x, it, rel, ts, tv = oracle.solve(ps["A"], ps["rhs"], "bicgstab", "amg", tol=args.tol, maxit=200)
-    times.append(t_asm + ts + tv)
-    xg = p_gpu.ravel()[ps["mat2grid"]]
-    checks["pressure"] = {"oracle_rel_residual_of_gpu_x": oracle.rel_residual(ps["A"], ps["rhs"], xg),
-                          "linf_gpu_minus_cpu_over_linf_cpu": float(np.abs(xg - x).max() / np.abs(x).max()),
-                          "cpu_iters": it, "cpu_rel": rel, "cpu_assembly_s": t_asm,
-                          "cpu_setup_s": ts, "cpu_solve_s": tv}
-    log(f"cpu pressure: asm {t_asm:.2f}s setup {ts:.2f}s solve {tv:.2f}s iters {it}")
-    del ps
-    # magnetic chi = 2, cold
-    ctx.reset_warm_start()
-    ctx.magnetic_solve_device(ptr(phi_m), [ptr(a) for a in hext], 2.0, 1.5 * geo.spacing,
-                              ptr(potential_out))
-    m_gpu = potential_out.cpu().numpy().ravel()
-    t = time.time()
-    ms = oracle.magnetic_system(dict(msc, chi=2.0, band=1.5 * geo.spacing))
-    t_asm = time.time() - t
-    x, it, rel, ts, tv = oracle.solve(ms["A"], ms["rhs"], "cg", "amg", tol=args.tol, maxit=200)
-    times.append(t_asm + ts + tv)
-    checks["magnetic_chi2"] = {"oracle_rel_residual_of_gpu_x": oracle.rel_residual(ms["A"], ms["rhs"], m_gpu),
-                               "linf_gpu_minus_cpu_over_linf_cpu": float(np.abs(m_gpu - x).max() / np.abs(x).max()),
-                               "cpu_iters": it, "cpu_rel": rel, "cpu_assembly_s": t_asm,
-                               "cpu_setup_s": ts, "cpu_solve_s": tv}
-    log(f"cpu magnetic: asm {t_asm:.2f}s setup {ts:.2f}s solve {tv:.2f}s iters {it}")
-    cpu = {"value": round(sum(times) / len(times), 4), "unit": "s",
-           "cores": oracle.num_threads(), "kind": "port",
-           "sample": f"one pressure (BiCGStab) and one magnetic chi=2 (CG) solve at "
-                     f"{args.n}^3 to rel. residual {args.tol:g}: serial CSR assembly + "
-                     f"SA-AMG/SPAI-0 setup + solve per call, OpenMP threads as cores"}
+    for name, chi in systems:
+        if chi is None:
+            sysd, xr, it, _, _, _ = pressure_call(1e-11)
+            pick = lambda a: a.ravel()[sysd["mat2grid"]]  # noqa: E731
+        else:
+            sysd, xr, it, _, _, _ = magnetic_call(chi, 1e-11)
+            pick = lambda a: a.ravel()  # noqa: E731
+        scale = float(np.abs(xr).max())
+        for leg, sol in solutions.items():
+            x, st = sol[name]
+            xg = pick(x)
+            checks[f"{leg}_{name}"] = {
+                "oracle_rel_residual": oracle.rel_residual(sysd["A"], sysd["rhs"], xg),
+                "linf_over_linf_oracle": float(np.abs(xg - xr).max() / scale),
+                "iterations": st["iterations"], "fallback": st["fallback"],
+                "library_rel_residual": st["rel_residual"]}
+            c = checks[f"{leg}_{name}"]
+            log(f"check {leg} {name}: oracle rel {c['oracle_rel_residual']:.2e}, "
+                f"Linf/Linf {c['linf_over_linf_oracle']:.2e}")
+        del sysd
+    per_solve = (timing["pressure_1e-8"]["total_s"] + timing["magnetic_chi2_1e-8"]["total_s"]) / 2
+    cpu = {"value": round(per_solve, 4), "unit": "s", "cores": oracle.num_threads(), "kind": "port",
+           "cpu_model": cpu_model(),
+           "sample": f"one pressure (BiCGStab) and one magnetic chi=2 (CG) solve at {args.n}^3 to "
+                     f"rel. residual {args.tol:g}, each the median of 5 calls after a warm-up: "
+                     f"CSR assembly + SA-AMG/SPAI-0 setup + solve per call (the reference rebuilds "
+                     f"its solver each call); OpenMP threads as cores",
+           "timing": timing}
     return cpu, checks
 
 

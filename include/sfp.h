@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define SFP_ABI_VERSION 1
+#define SFP_ABI_VERSION 2  /* 2: sfp_params gained the learned cycle's smoother (net_*) */
 
 typedef enum sfp_status {
     SFP_OK = 0,
@@ -56,6 +56,14 @@ typedef struct sfp_params {
     int verbose;         /* 1: print "{iters} iters" per solve like the reference */
     int coarse_cheby_degree;   /* Chebyshev degree on the levels below the finest (1..8) */
     double coarse_cheby_ratio; /* their smoothing interval ratio */
+    /* The learned preconditioner's V-cycle smoother: the settings its network
+       was trained under (recorded with the weights). 0 keeps the plain
+       cycle's value above. After a fall-back the plain cycle runs with the
+       settings above. */
+    int net_cheby_degree;          /* 0 or 1..8 */
+    double net_cheby_ratio;        /* 0 or > 1 */
+    int net_coarse_cheby_degree;   /* 0 or 1..8 */
+    double net_coarse_cheby_ratio; /* 0 or > 1 */
 } sfp_params;
 
 typedef struct sfp_stats {
@@ -173,6 +181,20 @@ sfp_status sfp_extrapolate_velocity(sfp_context ctx, double *const velocity[3],
 sfp_status sfp_collider_enforce(sfp_context ctx, double *const velocity[3],
                                 const double *const fraction[3], const double *const normal[3],
                                 const double *const collider_velocity[3]);
+
+/* Simulation::ProjectVelocity's three velocity passes (core/Simulation.cpp:
+ * 236-241) with the velocity kept in HBM between them: sfp_pressure_project,
+ * then sfp_extrapolate_velocity with `steps` sweeps, then
+ * sfp_collider_enforce, each with its own semantics, on host arrays that are
+ * uploaded once and downloaded once (velocity, and pressure when non-NULL).
+ * Results are those of the three calls in a row, bit for bit. */
+sfp_status sfp_projection_substep(sfp_context ctx, double *const velocity[3],
+                                  const double *level_set, const double *const fraction[3],
+                                  const double *const solid_velocity[3],
+                                  const double *const jump[3], double vol_error, int steps,
+                                  const double *const normal[3],
+                                  const double *const collider_velocity[3], double *pressure,
+                                  sfp_stats *stats);
 
 /* ---- device-resident variants ------------------------------------------ */
 /* Same semantics; every array pointer is a device pointer, work is enqueued

@@ -55,6 +55,11 @@ sfp_status sfp_unet_gradients(sfp_unet net, const float *input, const float *tar
 sfp_status sfp_unet_train_step_device(sfp_unet net, const float *input, const float *target,
                                       const float *mask, float lr, double *loss);
 const char *sfp_unet_last_error(sfp_unet net);
+/* Profiling: `reps` forward passes on the network's device-resident input
+ * (that of the last sfp_unet_forward), timed with HIP events on its
+ * stream: *forward_ms the mean of a whole pass, conv_ms[k] the mean of
+ * convolution k (10 values, in the order of the parameters). */
+sfp_status sfp_unet_profile_forward(sfp_unet net, int reps, double *forward_ms, double *conv_ms);
 
 #ifdef __cplusplus
 }

@@ -44,7 +44,9 @@ sfp_status sfp_test_restrict_smoothed(sfp_context ctx, int which, const double *
  * direction d1 with right-hand side b (host FP32 arrays of that level's
  * cells; zero on its inactive cells), with Chebyshev coefficients (cdA, crA)
  * then (cdB, crB): x3_pair from the one-pass kernel, x3_steps from two single
- * steps. Either output may be NULL. */
+ * steps. d1 NULL: the first two pre-smoothing passes from a stored first
+ * iterate x1, which is also the previous direction. Either output may be
+ * NULL. */
 sfp_status sfp_test_smooth_pair(sfp_context ctx, int which, int level, const float *b,
                                 const float *x1, const float *d1, float cdA, float crA,
                                 float cdB, float crB, float *x3_pair, float *x3_steps);

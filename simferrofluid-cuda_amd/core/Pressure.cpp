@@ -44,8 +44,8 @@ void Pressure::Reproject(SGridData<double> &velocity, GridData<double> const &le
     Solve(velocity, levelSet, collider, volError, true);
 }
 
-void Pressure::Solve(SGridData<double> &velocity, GridData<double> const &levelSet,
-                     Collider const &collider, double volError, bool reproject) {
+bool Pressure::EvaluateJumps(SGridData<double> const &velocity, GridData<double> const &levelSet,
+                             Collider const &collider) {
     // The callback is host code: evaluate it once per ghost-fluid face (open,
     // exactly one liquid neighbour), at theta = phi0 / (phi0 - phi1). The
     // reference calls it twice per such face: ApplyProjection passes this
@@ -72,6 +72,12 @@ void Pressure::Solve(SGridData<double> &velocity, GridData<double> const &levelS
             });
         }
     }
+    return useJump;
+}
+
+void Pressure::Solve(SGridData<double> &velocity, GridData<double> const &levelSet,
+                     Collider const &collider, double volError, bool reproject) {
+    bool const useJump = EvaluateJumps(velocity, levelSet, collider);
     double *vel[3] = {velocity[0].GetData().data(), velocity[1].GetData().data(),
                       velocity[2].GetData().data()};
     double const *frac[3] = {collider.GetFraction()[0].GetData().data(),
@@ -87,6 +93,30 @@ void Pressure::Solve(SGridData<double> &velocity, GridData<double> const &levelS
                   : sfp_pressure_project(m_Ctx, vel, levelSet.GetData().data(), frac, us,
                                          useJump ? jump : nullptr, volError, m_Pressure.data(), &stats);
     if (st != SFP_OK) fail(m_Ctx, st, reproject ? "Pressure::Reproject" : "Pressure::Project");
+    m_LastIters = stats.iterations;
+    m_LastResidual = stats.rel_residual;
+    if (stats.unknowns > 0) std::cout << std::setw(6) << stats.iterations << " iters";
+}
+
+void Pressure::ProjectExtrapolateEnforce(SGridData<double> &velocity, GridData<double> const &levelSet,
+                                         Collider const &collider, double volError, int steps) {
+    bool const useJump = EvaluateJumps(velocity, levelSet, collider);
+    double *vel[3];
+    double const *frac[3], *us[3], *jump[3], *nrm[3];
+    for (int a = 0; a < 3; a++) {
+        vel[a] = velocity[a].GetData().data();
+        frac[a] = collider.GetFraction()[a].GetData().data();
+        us[a] = collider.Velocity[a].GetData().data();
+        jump[a] = m_Jump[a].data();
+        nrm[a] = collider.GetNormal()[a].GetData().data();
+    }
+    sfp_stats stats;
+    // Collider::Enforce's collider velocity is the same Collider::Velocity
+    // the projection's solid faces move with (core/Collider.cpp:75-86)
+    sfp_status const st = sfp_projection_substep(m_Ctx, vel, levelSet.GetData().data(), frac, us,
+                                                 useJump ? jump : nullptr, volError, steps, nrm, us,
+                                                 m_Pressure.data(), &stats);
+    if (st != SFP_OK) fail(m_Ctx, st, "Pressure::Project + Extrapolation::Solve + Collider::Enforce");
     m_LastIters = stats.iterations;
     m_LastResidual = stats.rel_residual;
     if (stats.unknowns > 0) std::cout << std::setw(6) << stats.iterations << " iters";

@@ -33,6 +33,13 @@ class Pressure {
         m_PressureJump = pressureJump;
     }
 
+    // Simulation::ProjectVelocity's Project, then Extrapolation::Solve with
+    // `steps` sweeps and its mask (fraction < 1 and a liquid cell on either
+    // side), then collider.Enforce, with the velocity kept on the GPU between
+    // them (one upload, one download). Not in the reference's interface.
+    void ProjectExtrapolateEnforce(SGridData<double> &velocity, GridData<double> const &levelSet,
+                                   Collider const &collider, double volError, int steps);
+
     // Iterations, FP64 relative residual and full-grid pressure (0 at air
     // cells) of the last solve; the reference keeps the pressure private (m_RdP).
     int LastIterations() const { return m_LastIters; }
@@ -42,6 +49,9 @@ class Pressure {
   private:
     void Solve(SGridData<double> &velocity, GridData<double> const &levelSet,
                Collider const &collider, double volError, bool reproject);
+    // the jump callback on every ghost-fluid face into m_Jump; false: no callback
+    bool EvaluateJumps(SGridData<double> const &velocity, GridData<double> const &levelSet,
+                       Collider const &collider);
 
     StaggeredGrid const &m_SGrid;
     sfp_context_s *m_Ctx = nullptr;

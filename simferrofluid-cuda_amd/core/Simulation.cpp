@@ -2,8 +2,6 @@
 // Simulation::ProjectVelocity (core/Simulation.cpp:193-243).
 #include "Simulation.h"
 
-#include "Extrapolation.h"
-
 namespace Pivot {
 
 ProjectionStep::ProjectionStep(StaggeredGrid const &sgrid, SGridData<double> &velocity,
@@ -47,17 +45,14 @@ void ProjectionStep::ProjectVelocity(double dt, double time, double volError) {
         }
         return pressure;
     });
-    m_Pressure.Project(m_Velocity, m_LevelSet, m_Collider, m_Settings.semiImplicitSTEnabled ? 0 : volError);
-    Extrapolation::Solve(m_Velocity, 0., 6, [&](int axis, Vector3i const &face) {
-        Vector3i const cell0 = StaggeredGrid::AdjCellOfFace(axis, face, 0);
-        Vector3i const cell1 = StaggeredGrid::AdjCellOfFace(axis, face, 1);
-        Grid const &cg = m_LevelSet.GetGrid();
-        // a side outside the cell grid counts as air (the reference reads
-        // outside the grid there, which its closed-box collider never allows)
-        return m_Collider.GetFraction()[axis][face] < 1 &&
-               ((cg.IsValid(cell0) && m_LevelSet[cell0] <= 0) || (cg.IsValid(cell1) && m_LevelSet[cell1] <= 0));
-    });
-    m_Collider.Enforce(m_Velocity);
+    // Pressure::Project, Extrapolation::Solve(velocity, 0, 6, fraction < 1 and
+    // a liquid cell on either side) and Collider::Enforce, as the reference
+    // calls them in a row (core/Simulation.cpp:236-241), with the velocity
+    // kept on the GPU between the three (a side outside the cell grid counts
+    // as air: the reference reads outside the grid there, which its
+    // closed-box collider never allows)
+    m_Pressure.ProjectExtrapolateEnforce(m_Velocity, m_LevelSet, m_Collider,
+                                         m_Settings.semiImplicitSTEnabled ? 0 : volError, 6);
 }
 
 }  // namespace Pivot

@@ -16,6 +16,10 @@ thread_local std::string g_create_error;
 
 size_t face_count(Dims d, int a) { return d.face(a).cells(); }
 
+}  // namespace
+
+namespace {
+
 sfp_params to_params(const SolverConfig &c) {
     sfp_params p;
     p.tol = c.tol;
@@ -29,6 +33,10 @@ sfp_params to_params(const SolverConfig &c) {
     p.verbose = c.verbose;
     p.coarse_cheby_degree = c.coarse_cheby_degree;
     p.coarse_cheby_ratio = c.coarse_cheby_ratio;
+    p.net_cheby_degree = c.net_cheby_degree;
+    p.net_cheby_ratio = c.net_cheby_ratio;
+    p.net_coarse_cheby_degree = c.net_coarse_cheby_degree;
+    p.net_coarse_cheby_ratio = c.net_coarse_cheby_ratio;
     return p;
 }
 
@@ -370,6 +378,51 @@ sfp_status sfp_collider_enforce(sfp_context ctx, double *const velocity[3],
     });
 }
 
+sfp_status sfp_projection_substep(sfp_context ctx, double *const velocity[3], const double *level_set,
+                                  const double *const fraction[3],
+                                  const double *const solid_velocity[3],
+                                  const double *const jump[3], double vol_error, int steps,
+                                  const double *const normal[3],
+                                  const double *const collider_velocity[3], double *pressure,
+                                  sfp_stats *stats) {
+    return guarded(ctx, [&] {
+        require3((const void *const *)velocity, "velocity");
+        require3((const void *const *)fraction, "fraction");
+        require3((const void *const *)solid_velocity, "solid_velocity");
+        require3((const void *const *)normal, "normal");
+        require3((const void *const *)collider_velocity, "collider_velocity");
+        if (jump) require3((const void *const *)jump, "jump");
+        if (!level_set) throw ArgError("level_set is null");
+        check_steps(steps);
+        const Dims d = ctx->d;
+        // one upload of every input, the three passes on the device, one
+        // download of the velocity (and of the pressure when asked)
+        double *dv[3];
+        const double *df[3], *du[3], *dj[3], *dn[3], *dc[3];
+        for (int a = 0; a < 3; a++) {
+            const size_t n = face_count(d, a);
+            dv[a] = const_cast<double *>(up(ctx, ctx->s_face[0][a], velocity[a], n));
+            df[a] = up(ctx, ctx->s_face[1][a], fraction[a], n);
+            du[a] = up(ctx, ctx->s_face[2][a], solid_velocity[a], n);
+            dj[a] = jump ? up(ctx, ctx->s_face[3][a], jump[a], n) : nullptr;
+            dn[a] = up(ctx, ctx->s_face[4][a], normal[a], n);
+            dc[a] = up(ctx, ctx->s_face[5][a], collider_velocity[a], n);
+        }
+        const double *dphi = up(ctx, ctx->s_phi, level_set, d.cells());
+        double *dp = nullptr;
+        if (pressure) {
+            ctx->s_cell.ensure(d.cells());
+            dp = ctx->s_cell.p;
+        }
+        pressure_dev(ctx, dv, dphi, df, du, jump ? dj : nullptr, vol_error, dp, stats, false);
+        extrapolate_velocity_dev(ctx, dv, dphi, df, steps);
+        collider_enforce_dev(ctx, dv, df, dn, dc);
+        for (int a = 0; a < 3; a++) down(ctx, velocity[a], dv[a], face_count(d, a));
+        if (pressure) down(ctx, pressure, dp, d.cells());
+        SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
+    });
+}
+
 sfp_status sfp_extrapolate_velocity_device(sfp_context ctx, double *const velocity[3],
                                            const double *level_set,
                                            const double *const fraction[3], int steps) {
@@ -466,7 +519,10 @@ sfp_status sfp_set_params(sfp_context ctx, const sfp_params *p) {
     if (!(p->tol > 0) || p->max_iters < 0 || p->precond < 0 || p->precond > 3 ||
         p->mg_coarsest < 2 || p->mg_coarsest > 4 || p->cheby_degree < 1 || p->cheby_degree > 8 ||
         !(p->cheby_ratio > 1) || !(p->coarse_scale > 0) || !(p->stall_ratio > 0) ||
-        p->coarse_cheby_degree < 1 || p->coarse_cheby_degree > 8 || !(p->coarse_cheby_ratio > 1)) {
+        p->coarse_cheby_degree < 1 || p->coarse_cheby_degree > 8 || !(p->coarse_cheby_ratio > 1) ||
+        p->net_cheby_degree < 0 || p->net_cheby_degree > 8 || p->net_coarse_cheby_degree < 0 ||
+        p->net_coarse_cheby_degree > 8 || !(p->net_cheby_ratio == 0 || p->net_cheby_ratio > 1) ||
+        !(p->net_coarse_cheby_ratio == 0 || p->net_coarse_cheby_ratio > 1)) {
         ctx->last_error = "parameter out of range";
         return SFP_ERR_ARGUMENT;
     }
@@ -486,6 +542,10 @@ sfp_status sfp_set_params(sfp_context ctx, const sfp_params *p) {
     c.verbose = p->verbose;
     c.coarse_cheby_degree = p->coarse_cheby_degree;
     c.coarse_cheby_ratio = p->coarse_cheby_ratio;
+    c.net_cheby_degree = p->net_cheby_degree;
+    c.net_cheby_ratio = p->net_cheby_ratio;
+    c.net_coarse_cheby_degree = p->net_coarse_cheby_degree;
+    c.net_coarse_cheby_ratio = p->net_coarse_cheby_ratio;
     return SFP_OK;
 }
 

@@ -216,7 +216,9 @@ sfp_status sfp_test_smooth_pair(sfp_context ctx, int which, int level, const flo
     return run(ctx, [&] {
         System &s = pick(ctx, which);
         if (level < 0 || level >= s.num_levels()) throw ArgError("bad level");
-        if (!b || !x1 || !d1) throw ArgError("null vector");
+        if (!b || !x1) throw ArgError("null vector");
+        const bool from_x1 = d1 == nullptr;  // d1 is x1 (stored first iterate)
+        if (from_x1) d1 = x1;
         Level &L = s.level(level);
         const size_t n = L.d.cells();
         const hipStream_t st = ctx->ws.stream;
@@ -235,15 +237,16 @@ sfp_status sfp_test_smooth_pair(sfp_context ctx, int which, int level, const flo
         const Op32 A = L.op32();
         if (x3_pair) {
             SFP_HIP(hipMemcpyAsync(dd.p, d1, n * sizeof(float), hipMemcpyHostToDevice, st));
-            launch_smooth_pair(st, L.d, A, db.p, dx1.p, dx3.p, dd.p, cdA, crA, cdB, crB);
+            launch_smooth_pair(st, L.d, A, db.p, dx1.p, dx3.p, from_x1 ? dx1.p : dd.p, cdA, crA, cdB, crB,
+                               nullptr, L.rowact.p);
             d2h(ctx, x3_pair, dx3.p, n);
             SFP_HIP(hipStreamSynchronize(st));
         }
         if (x3_steps) {
             SFP_HIP(hipMemcpyAsync(dd.p, d1, n * sizeof(float), hipMemcpyHostToDevice, st));
             SFP_HIP(hipMemsetAsync(dx3.p, 0, n * sizeof(float), st));
-            launch_smooth(st, SM_READ_D | SM_WRITE_D, L.d, A, nullptr, db.p, dx1.p, dx2.p, dd.p, 0.f,
-                          cdA, crA, nullptr);
+            launch_smooth(st, (from_x1 ? SM_XFIRST : SM_READ_D) | SM_WRITE_D, L.d, A, nullptr, db.p, dx1.p,
+                          dx2.p, dd.p, 0.f, cdA, crA, nullptr);
             launch_smooth(st, SM_READ_D, L.d, A, nullptr, db.p, dx2.p, dx3.p, dd.p, 0.f, cdB, crB,
                           nullptr);
             d2h(ctx, x3_steps, dx3.p, n);

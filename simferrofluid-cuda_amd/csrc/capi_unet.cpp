@@ -193,6 +193,39 @@ sfp_status sfp_unet_forward(sfp_unet n, const float *input, float *output) {
     });
 }
 
+sfp_status sfp_unet_profile_forward(sfp_unet n, int reps, double *forward_ms, double *conv_ms) {
+    return run_net(n, [&] {
+        if (reps < 1 || !forward_ms || !conv_ms) throw ArgError("reps >= 1 and non-null outputs");
+        hipEvent_t ev[2 * kUNetConvs + 2] = {};
+        for (auto &e : ev) SFP_HIP(hipEventCreate(&e));
+        struct Cleanup {
+            hipEvent_t *e;
+            UNet *net;
+            ~Cleanup() {
+                net->set_conv_events(nullptr);
+                for (int q = 0; q < 2 * kUNetConvs + 2; q++) (void)hipEventDestroy(e[q]);
+            }
+        } cleanup{ev, n->net};
+        double conv[kUNetConvs] = {0}, total = 0;
+        n->net->set_conv_events(ev);
+        for (int r = 0; r < reps; r++) {
+            SFP_HIP(hipEventRecord(ev[2 * kUNetConvs], n->stream));
+            n->net->forward(n->stream, n->net->input_buffer(), n->net->output_buffer());
+            SFP_HIP(hipEventRecord(ev[2 * kUNetConvs + 1], n->stream));
+            SFP_HIP(hipEventSynchronize(ev[2 * kUNetConvs + 1]));
+            float ms = 0.f;
+            SFP_HIP(hipEventElapsedTime(&ms, ev[2 * kUNetConvs], ev[2 * kUNetConvs + 1]));
+            total += ms;
+            for (int k = 0; k < kUNetConvs; k++) {
+                SFP_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+                conv[k] += ms;
+            }
+        }
+        *forward_ms = total / reps;
+        for (int k = 0; k < kUNetConvs; k++) conv_ms[k] = conv[k] / reps;
+    });
+}
+
 sfp_status sfp_unet_gradients(sfp_unet n, const float *input, const float *target, const float *mask,
                               double *loss, float *grads) {
     return run_net(n, [&] {

@@ -26,7 +26,9 @@ struct sfp_context_s {
     bool field_ready = false;
     // staging for the host-pointer entry points
     sfp::DevBuf<double> s_phi, s_cell;
-    sfp::DevBuf<double> s_face[5][3];   // velocity, fraction, solid velocity, jump, field
+    // velocity, fraction, solid velocity (or normal), jump (or collider
+    // velocity), field (or normal), collider velocity
+    sfp::DevBuf<double> s_face[6][3];
     sfp::DevBuf<double> s_vert[3];      // positions, normals, pressures
     // post-projection steps: velocity before Collider::Enforce, validity masks
     sfp::DevBuf<double> post_old[3];

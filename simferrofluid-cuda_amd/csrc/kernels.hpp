@@ -40,9 +40,15 @@ inline bool pair_pays(Dims d) { return d.nz >= 64 && d.cells() > ((size_t)1 << 2
 // Grids on which launch_smooth runs its steps (all but the fused first step)
 // over the flat cell index: nz a multiple of 4, fewer than 2^32 cells.
 bool flat_step(Dims d);
+// Grids with z rows of 193..256 cells in groups of four run it over whole
+// rows (k_smooth_pair_rows); rowact (null: all active) marks the z rows
+// holding an active cell (launch_row_activity), whose loads the others skip.
 void launch_smooth_pair(hipStream_t s, Dims d, Op32 A, const float *b, const float *x1, float *x3,
                         const float *d1, float cdA, float crA, float cdB, float crB,
-                        const double *gate = nullptr);
+                        const double *gate = nullptr, const unsigned char *rowact = nullptr);
+bool row_pair(Dims d);
+// rowact[i ny + j] = 1 when the z row (i, j) holds an active cell (diag32 > 0)
+void launch_row_activity(hipStream_t s, Dims d, const float *diag32, unsigned char *rowact);
 // The last pre-smoothing step and the restriction in one pass:
 // x2 = x1 + cd d1 + cr D^{-1} (b - A x1), then bc = R (b - A x2) (the coarse
 // right-hand side, as launch_restrict). Same results as launch_smooth
@@ -112,8 +118,9 @@ constexpr int kReducePlanes = 0;  // automatic
 // Doubles of partial-sum scratch a reduction over grid d needs.
 size_t reduction_partials(Dims d);
 // Before a batch of gated iterations: scalars->stop = 0, ->iters = 0,
-// ->stop_rr = stop_rr.
-void launch_batch_begin(hipStream_t s, KrylovScalars *sc, double stop_rr);
+// ->stop_rr = stop_rr, ->stall_rr = stall_rr (0: no stall test),
+// ->stalled = 0.
+void launch_batch_begin(hipStream_t s, KrylovScalars *sc, double stop_rr, double stall_rr = 0.0);
 // r = b - A x (x may be null = 0); scalars->rr = r.r, scalars->bb = b.b.
 // With b32, also b32 = r rounded to FP32 and, with x0, x0 = c0 D32^-1 b32
 // (diag32: the FP32 diagonal): the right-hand side and first Chebyshev
@@ -159,8 +166,8 @@ struct MutFaces {
 // valid[f] (face grid axis) = fraction < 1 and a liquid cell on either side.
 void launch_velocity_valid(hipStream_t s, Dims d, int axis, const double *phi, const double *frac,
                            unsigned char *valid);
-// `steps` sweeps of Extrapolation::Solve on one grid; valid is updated,
-// scratch (same size) is overwritten.
+// `steps` sweeps of Extrapolation::Solve on one grid: valid (the mask) and
+// scratch (same size) are both overwritten.
 void launch_extrapolate(hipStream_t s, Dims g, double *data, unsigned char *valid,
                         unsigned char *scratch, double clear, int steps);
 // Collider::Enforce; old is the velocity before the call (not aliasing vel).

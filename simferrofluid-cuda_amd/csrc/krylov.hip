@@ -451,11 +451,13 @@ constexpr int kFinalizeThreads = 1024;
 
 // Start of a batch of gated iterations: clear the stop flag and the iteration
 // count, set the threshold on ||r||^2 (one thread; no host copy needed).
-__global__ __launch_bounds__(64) void k_batch_begin(KrylovScalars *sc, double stop_rr) {
+__global__ __launch_bounds__(64) void k_batch_begin(KrylovScalars *sc, double stop_rr, double stall_rr) {
     if (threadIdx.x == 0) {
         sc->stop = 0.0;
         sc->iters = 0.0;
         sc->stop_rr = stop_rr;
+        sc->stall_rr = stall_rr;
+        sc->stalled = 0.0;
     }
 }
 
@@ -498,10 +500,15 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int op, unsigned 
         sc->alpha_prev = sc->alpha;
         sc->alpha = pq != 0.0 ? zr / pq : 0.0;
     } else if (op == FIN_UPDATE) {
+        const double rr_prev = sc->rr;
         sc->rr = v[0];
         sc->pq_prev = sc->pq;
         sc->iters += 1.0;
         if (gate && v[0] <= sc->stop_rr) sc->stop = 1.0;  // later gated kernels return at once
+        if (gate && sc->stall_rr > 0.0 && v[0] > sc->stall_rr * rr_prev) {
+            sc->stalled = 1.0;  // the host switches to the plain V-cycle after the batch
+            sc->stop = 1.0;
+        }
     } else {
         sc->pad[0] = v[0];
     }
@@ -541,8 +548,8 @@ void launch_count_active(hipStream_t st, Dims d, const float *diag32, double *pa
     finalize(st, FIN_COUNT, L.blocks(), 0, partials, sc);
 }
 
-void launch_batch_begin(hipStream_t st, KrylovScalars *sc, double stop_rr) {
-    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, sc, stop_rr);
+void launch_batch_begin(hipStream_t st, KrylovScalars *sc, double stop_rr, double stall_rr) {
+    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, sc, stop_rr, stall_rr);
     SFP_LAUNCH_CHECK();
 }
 

@@ -7,6 +7,9 @@
 namespace sfp {
 namespace {
 
+// true inside a batch of FCG iterations that has stopped (KrylovScalars::stop)
+__device__ __forceinline__ bool gated(const double *gate) { return gate && *gate != 0.0; }
+
 __device__ __forceinline__ uint16_t to_bf16(float f) {
     const unsigned u = __float_as_uint(f);
     if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
@@ -56,7 +59,9 @@ __global__ __launch_bounds__(kBlock) void k_features(CellGrid cg, Dims d, Op64 A
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_rms_partials(CellGrid cg, Dims d, const T *r,
-                                                         const float *diag32, double *partials) {
+                                                         const float *diag32, double *partials,
+                                                         const double *gate) {
+    if (gated(gate)) return;
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     double v[2] = {0.0, 0.0};
@@ -74,7 +79,8 @@ __global__ __launch_bounds__(kBlock) void k_rms_partials(CellGrid cg, Dims d, co
 }
 
 __global__ __launch_bounds__(kBlock) void k_rms_finalize(unsigned nb, const double *partials,
-                                                         KrylovScalars *sc) {
+                                                         KrylovScalars *sc, const double *gate) {
+    if (gated(gate)) return;
     double v[2];
     sum_partials<2>(partials, nb, v);
     if (flat_tid() == 0) sc->pad[1] = (v[0] > 0.0 && v[1] > 0.0) ? 1.0 / sqrt(v[0] / v[1]) : 0.0;
@@ -83,7 +89,8 @@ __global__ __launch_bounds__(kBlock) void k_rms_finalize(unsigned nb, const doub
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_pack_residual(CellGrid cg, Dims d, const T *r,
                                                           const float *diag32, const KrylovScalars *sc,
-                                                          uint16_t *__restrict__ x8) {
+                                                          uint16_t *__restrict__ x8, const double *gate) {
+    if (gated(gate)) return;
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     if (k >= d.nz || j >= d.ny) return;
@@ -118,7 +125,9 @@ __global__ __launch_bounds__(kBlock) void k_restrict_phi(CellGrid cg, Dims dc, D
 
 // Partials of (z.b, z.Az) over the active cells, z taken as 0 on inactive ones.
 __global__ __launch_bounds__(kBlock) void k_corr_dots(CellGrid cg, Dims d, Op32 A, const float *z,
-                                                      const float *b, double *partials) {
+                                                      const float *b, double *partials,
+                                                      const double *gate) {
+    if (gated(gate)) return;
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     double v[2] = {0.0, 0.0};
@@ -140,7 +149,8 @@ __global__ __launch_bounds__(kBlock) void k_corr_dots(CellGrid cg, Dims d, Op32 
 }
 
 __global__ __launch_bounds__(kBlock) void k_corr_finalize(unsigned nb, const double *partials,
-                                                          KrylovScalars *sc) {
+                                                          KrylovScalars *sc, const double *gate) {
+    if (gated(gate)) return;
     double v[2];
     sum_partials<2>(partials, nb, v);
     if (flat_tid() == 0) sc->pad[2] = v[1] > 0.0 ? v[0] / v[1] : 0.0;
@@ -148,7 +158,8 @@ __global__ __launch_bounds__(kBlock) void k_corr_finalize(unsigned nb, const dou
 
 __global__ __launch_bounds__(kBlock) void k_corr_scale(CellGrid cg, Dims d, const float *z,
                                                        const float *diag32, const KrylovScalars *sc,
-                                                       float *__restrict__ e) {
+                                                       float *__restrict__ e, const double *gate) {
+    if (gated(gate)) return;
     int k, j, i0;
     cell_tile(cg, k, j, i0);
     if (k >= d.nz || j >= d.ny) return;
@@ -181,25 +192,25 @@ void launch_learned_features(hipStream_t st, Dims d, Op64 A, const float *diag32
 
 template <class T>
 void pack_residual(hipStream_t st, Dims d, const T *r, const float *diag32, double *partials,
-                   KrylovScalars *sc, uint16_t *x8) {
+                   KrylovScalars *sc, uint16_t *x8, const double *gate) {
     Launch L = cell_launch(d, kReducePlanes);
-    hipLaunchKernelGGL(k_rms_partials<T>, L.grid, L.block, 0, st, L.cg, d, r, diag32, partials);
+    hipLaunchKernelGGL(k_rms_partials<T>, L.grid, L.block, 0, st, L.cg, d, r, diag32, partials, gate);
     SFP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_rms_finalize, dim3(1), dim3(kBlock), 0, st, L.blocks(), partials, sc);
+    hipLaunchKernelGGL(k_rms_finalize, dim3(1), dim3(kBlock), 0, st, L.blocks(), partials, sc, gate);
     SFP_LAUNCH_CHECK();
     Launch P = cell_launch(d);
-    hipLaunchKernelGGL(k_pack_residual<T>, P.grid, P.block, 0, st, P.cg, d, r, diag32, sc, x8);
+    hipLaunchKernelGGL(k_pack_residual<T>, P.grid, P.block, 0, st, P.cg, d, r, diag32, sc, x8, gate);
     SFP_LAUNCH_CHECK();
 }
 
 void launch_pack_residual(hipStream_t st, Dims d, const double *r, const float *diag32,
                           double *partials, KrylovScalars *sc, uint16_t *x8) {
-    pack_residual(st, d, r, diag32, partials, sc, x8);
+    pack_residual(st, d, r, diag32, partials, sc, x8, nullptr);
 }
 
 void launch_pack_residual32(hipStream_t st, Dims d, const float *r, const float *diag32,
-                            double *partials, KrylovScalars *sc, uint16_t *x8) {
-    pack_residual(st, d, r, diag32, partials, sc, x8);
+                            double *partials, KrylovScalars *sc, uint16_t *x8, const double *gate) {
+    pack_residual(st, d, r, diag32, partials, sc, x8, gate);
 }
 
 void launch_restrict_phi(hipStream_t st, Dims df, const double *phi, Dims dc, double *out) {
@@ -209,14 +220,14 @@ void launch_restrict_phi(hipStream_t st, Dims df, const double *phi, Dims dc, do
 }
 
 void launch_scaled_correction(hipStream_t st, Dims d, Op32 A, const float *z, const float *b,
-                              double *partials, KrylovScalars *sc, float *e) {
+                              double *partials, KrylovScalars *sc, float *e, const double *gate) {
     Launch L = cell_launch(d, kReducePlanes);
-    hipLaunchKernelGGL(k_corr_dots, L.grid, L.block, 0, st, L.cg, d, A, z, b, partials);
+    hipLaunchKernelGGL(k_corr_dots, L.grid, L.block, 0, st, L.cg, d, A, z, b, partials, gate);
     SFP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_corr_finalize, dim3(1), dim3(kBlock), 0, st, L.blocks(), partials, sc);
+    hipLaunchKernelGGL(k_corr_finalize, dim3(1), dim3(kBlock), 0, st, L.blocks(), partials, sc, gate);
     SFP_LAUNCH_CHECK();
     Launch P = cell_launch(d);
-    hipLaunchKernelGGL(k_corr_scale, P.grid, P.block, 0, st, P.cg, d, z, A.diag, sc, e);
+    hipLaunchKernelGGL(k_corr_scale, P.grid, P.block, 0, st, P.cg, d, z, A.diag, sc, e, gate);
     SFP_LAUNCH_CHECK();
 }
 

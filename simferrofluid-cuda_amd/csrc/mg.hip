@@ -526,6 +526,252 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
     }
 }
 
+// The same pair of steps (step A, then step B with kRestrict false) on grids
+// whose z rows hold at most 256 cells, in groups of four: a lane holds four
+// consecutive cells of a z row (16-byte accesses), a wavefront a whole row,
+// a block kRowPairRows rows of y, and the block marches over its run of x
+// planes. Tiles overlap by one row on each side in y (rows 0 and
+// kRowPairRows - 1 are step A's halo) and not at all in z. The per-cell
+// arithmetic is k_smooth_flat's (two launches: SM_READ_D | SM_WRITE_D, or
+// SM_XFIRST | SM_WRITE_D when d1 is x1 (kDx), then SM_READ_D), bit for bit.
+//
+// Each iteration of the march, for plane i:
+//   1. puts x1 and cpl_y of plane i + 2 (loaded one iteration earlier) in
+//      LDS, with the out-of-tile rows next to the tile (rows 0 and
+//      kRowPairRows - 1 load them);
+//   2. issues the loads of x1, cpl_x, cpl_y of plane i + 3 and of cpl_z, b,
+//      d1 and the diagonal of plane i + 2, so one iteration hides their
+//      latency;
+//   3. computes step A on plane i + 1 (x1 of the rows above and below and
+//      their cpl_y from LDS) and puts x2 in LDS; one barrier;
+//   4. computes step B on plane i (x2 of the three planes and of the rows
+//      above and below from LDS) and stores it on the tile's interior rows.
+// LDS rings: x1 two planes deep (written an iteration before it is read, and
+// rewritten only after the next barrier), x2 three (written before the
+// barrier that precedes its readers, which a wavefront one barrier ahead must
+// not disturb) and cpl_y four (read by step A in one iteration and by step B
+// in the next). `rowact` (null: every row active) marks the (plane, row) z
+// rows holding an active cell: the others load nothing and stay 0, as the
+// vectors hold there.
+constexpr int kRowPairRows = 12;
+constexpr int kRowPairY = kRowPairRows - 2;  // interior rows: the tiles' stride in y
+
+template <bool kRowsum, bool kDx>
+__global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
+    Dims d, Op32 A, const float *__restrict__ b, const float *__restrict__ x1,
+    float *__restrict__ xo, const float *__restrict__ d1, float cdA, float crA, float cdB,
+    float crB, const unsigned char *__restrict__ rowact, int planes, unsigned ytiles,
+    const double *gate) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    if (gated(gate)) return;
+    __shared__ f4 sx1[2][kRowPairRows + 2][64];  // x1 of rows -1 .. kRowPairRows (index row + 1)
+    __shared__ f4 scy[4][kRowPairRows + 1][64];  // cpl_y of rows 0 .. kRowPairRows
+    __shared__ f4 sx2[3][kRowPairRows][64];      // x2 of rows 0 .. kRowPairRows - 1
+    // logical tiles run y fastest; each XCD (block b goes to XCD b % 8) takes
+    // a contiguous run of them, so the rows two tiles share meet in one L2
+    unsigned L = blockIdx.x;
+    {
+        const unsigned q = gridDim.x / 8;
+        if (L < 8 * q) L = (L % 8) * q + L / 8;
+    }
+    const unsigned by = L % ytiles, bx = L / ytiles;
+    const int i0 = (int)bx * planes;
+    if (i0 >= d.nx) return;  // uniform over the block
+    const int ie = min(i0 + planes, d.nx);
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int k = 4 * lane;
+    const int j = (int)by * kRowPairY - 1 + row;
+    const bool zon = k < d.nz;
+    const bool yin = j >= 0 && j < d.ny;
+    const bool interior = row >= 1 && row <= kRowPairY;
+    const bool ylo = j > 0, yhi = j + 1 < d.ny;
+    const bool zl = k > 0, zh = k + 4 < d.nz;
+    const Strides s(d);
+    // element offsets fit 32 bits (the launcher keeps cells() < 2^32); lanes
+    // past the row's end read its last group (a cache hit) and drop it
+    const unsigned c0 = yin ? (unsigned)d.idx(0, j, min(k, d.nz - 4)) : 0u;
+    const unsigned sx = (unsigned)s.sx;
+    // the out-of-tile rows next to the tile (x1 of rows j0 - 1 and
+    // j0 + kRowPairRows - 2, cpl_y of the latter), one value per thread: thread
+    // t of the block takes z = t mod 256 of part t / 256
+    const int t = row * 64 + lane;
+    const int hpart = t >> 8, hz = t & 255;  // part 3: nothing
+    const int jt = (int)by * kRowPairY - 1;  // the tile's row 0
+    const int jh = hpart == 0 ? jt - 1 : jt + kRowPairRows;
+    const bool hon_any = hpart < 3 && jh >= 0 && jh < d.ny;  // uniform over the wavefront
+    const unsigned ch = hon_any ? (unsigned)d.idx(0, jh, min(hz, d.nz - 1)) : 0u;
+    const f4 z4 = {0.f, 0.f, 0.f, 0.f};
+    // activity of this row and of the out-of-tile row for planes i0 - 2 ..
+    // i0 + 61 (bit q - i0 + 2), one lane per plane, read once
+    auto row_mask = [&](int jj) -> unsigned long long {
+        const int q = i0 - 2 + lane;
+        const bool a = q >= 0 && q < d.nx && jj >= 0 && jj < d.ny &&
+                       (!rowact || rowact[(size_t)q * d.ny + jj] != 0);
+        return __ballot(a);
+    };
+    const unsigned long long amask = row_mask(j);
+    const unsigned long long hmask_lo = row_mask(jt - 1), hmask_hi = row_mask(jt + kRowPairRows);
+    const unsigned long long hmask = hpart == 0 ? hmask_lo : hmask_hi;
+    auto active = [&](int q, unsigned long long m) { return ((m >> (q - i0 + 2)) & 1ull) != 0; };
+    auto ld = [&](const float *p) { return zon ? *(const f4 *)p : z4; };
+
+    struct Early {
+        f4 x, cx, cy;  // x1, cpl_x, cpl_y
+        float h;       // this thread's value of the out-of-tile rows
+    };
+    struct Rest {
+        f4 cz, bv, dv, dg;
+    };
+    // loads branch on the wavefront's row activity only (uniform)
+    auto load_early = [&](int q) {
+        Early e{z4, z4, z4, 0.f};
+        if (active(q, amask)) {
+            const unsigned c = c0 + (unsigned)q * sx;
+            e.x = ld(x1 + c);
+            e.cx = ld(A.cpl[0] + c);
+            e.cy = ld(A.cpl[1] + c);
+        }
+        if (hon_any && active(q, hmask)) {
+            const unsigned h = ch + (unsigned)q * sx;
+            const float v = hpart == 2 ? A.cpl[1][h] : x1[h];
+            e.h = hz < d.nz ? v : 0.f;
+        }
+        return e;
+    };
+    auto load_rest = [&](int q) {
+        Rest r{z4, z4, z4, z4};
+        if (active(q, amask)) {
+            const unsigned c = c0 + (unsigned)q * sx;
+            r.cz = ld(A.cpl[2] + c);
+            r.bv = ld(b + c);
+            if (!kDx) r.dv = ld(d1 + c);
+            // a row-sum diagonal is rebuilt; the groups of cells 1, nz and
+            // nz*ny (on planes 0 and 1, rows 0 and 1) read it
+            if (!kRowsum || !A.pinned_rowsum || (q <= 1 && j <= 1)) r.dg = ld(A.diag + c);
+        }
+        return r;
+    };
+    auto put_early = [&](int q, const Early &e) {
+        sx1[q & 1][row + 1][lane] = e.x;
+        scy[q & 3][row][lane] = e.cy;
+        if (hpart == 0) ((float *)&sx1[q & 1][0][0])[hz] = e.h;
+        else if (hpart == 1) ((float *)&sx1[q & 1][kRowPairRows + 1][0])[hz] = e.h;
+        else if (hpart == 2) ((float *)&scy[q & 3][kRowPairRows][0])[hz] = e.h;
+    };
+    // one Chebyshev step of the four cells at plane q from iterate xc (x
+    // neighbours xm, xp; y neighbours and couplings given), direction dp
+    auto cheb = [&](f4 xc, f4 xm, f4 xp, f4 xym, f4 xyp, f4 cxc, f4 cxp, f4 cym, f4 cyp, f4 cz,
+                    f4 bv, f4 dg, f4 dp, float cd, float cr, f4 &dn_out) {
+        // every lane takes part in the shuffles: a lane outside the exec mask
+        // does not supply its value to ds_bpermute (the reader gets 0)
+        const float xzlo = __shfl_up(xc.w, 1, 64);
+        const float xzhi = __shfl_down(xc.x, 1, 64);
+        const float czdn = __shfl_down(cz.x, 1, 64);
+        const float czhi = zh ? czdn : 0.f;
+        f4 xo4;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float xzm = q > 0 ? xc[q - 1] : xzlo;
+            const float xzp = q < 3 ? xc[q + 1] : xzhi;
+            const float czp = q < 3 ? cz[q + 1] : czhi;
+            float off = fmaf(cxc[q], xm[q], cxp[q] * xp[q]);
+            if (ylo) off = fmaf(cym[q], xym[q], off);
+            if (yhi) off = fmaf(cyp[q], xyp[q], off);
+            if (q > 0 || zl) off = fmaf(cz[q], xzm, off);
+            if (q < 3 || zh) off = fmaf(czp, xzp, off);
+            const float r = bv[q] - fmaf(dg[q], xc[q], -off);
+            const float dn = fmaf(cd, dp[q], cr * rcp_or_zero(dg[q]) * r);
+            xo4[q] = xc[q] + dn;
+            dn_out[q] = dn;
+        }
+        return xo4;
+    };
+    // the diagonal at plane q: loaded, or rebuilt from the couplings
+    auto diag_at = [&](int q, f4 cxc, f4 cxp, f4 cym, f4 cyp, const Rest &r) {
+        f4 dg = r.dg;
+        if (kRowsum && A.pinned_rowsum) {
+            const unsigned c = c0 + (unsigned)q * sx;
+            const float czdn = __shfl_down(r.cz.x, 1, 64);  // all lanes (see cheb)
+            const float czhi = zh ? czdn : 0.f;
+            if (rowsum_group(A, c, d)) dg = rowsum_diag4(cxc, cxp, cym, cyp, r.cz, czhi);
+        }
+        return dg;
+    };
+    // step A at plane q: x2 (returned) and d2; x1 of this row at planes q and
+    // q + 1 is read back from LDS (this wavefront wrote it)
+    auto step_a = [&](int q, f4 xm, const Early &e, f4 cxp, const Rest &r, f4 &d2) {
+        const f4 xc = sx1[q & 1][row + 1][lane], xp = sx1[(q + 1) & 1][row + 1][lane];
+        const f4 xym = sx1[q & 1][row][lane], xyp = sx1[q & 1][row + 2][lane];
+        const f4 cym = scy[q & 3][row][lane], cyp = scy[q & 3][row + 1][lane];
+        const f4 dg = diag_at(q, e.cx, cxp, cym, cyp, r);
+        d2 = z4;
+        if (!active(q, amask)) return z4;  // uniform over the wavefront
+        f4 x2 = cheb(xc, xm, xp, xym, xyp, e.cx, cxp, cym, cyp, r.cz, r.bv, dg, kDx ? xc : r.dv, cdA,
+                     crA, d2);
+        if (!zon) x2 = z4;
+        return x2;
+    };
+
+    // ---- prologue: planes i0 - 1 and i0 through step A, plane i0 + 1 in LDS
+    const f4 xpre = load_early(i0 - 2).x;
+    Early em = load_early(i0 - 1), e0 = load_early(i0), e1 = load_early(i0 + 1);
+    Early e2 = load_early(i0 + 2);
+    Rest rm = load_rest(i0 - 1), r0 = load_rest(i0), r1 = load_rest(i0 + 1);
+    // step A at i0 - 1 reads planes i0 - 1 and i0 from the x1 ring, step A at
+    // i0 planes i0 and i0 + 1 (which takes the buffer of i0 - 1)
+    put_early(i0 - 1, em);
+    put_early(i0, e0);
+    __syncthreads();
+    f4 dtmp, d2c;
+    const f4 x2m = step_a(i0 - 1, xpre, em, e0.cx, rm, dtmp);
+    __syncthreads();
+    put_early(i0 + 1, e1);
+    __syncthreads();
+    const f4 x2c = step_a(i0, em.x, e0, e1.cx, r0, d2c);
+    sx2[(i0 + 2) % 3][row][lane] = x2m;  // (i0 - 1) mod 3
+    sx2[i0 % 3][row][lane] = x2c;
+    __syncthreads();
+    f4 xa = e0.x;     // x1 at plane i
+    f4 cxa = e0.cx;   // cpl_x at plane i
+    f4 x2prev = x2m;  // x2 at plane i - 1 (this row)
+    (void)x2c;
+    for (int i = i0; i < ie; i++) {
+        // 1. x1 and cpl_y of plane i + 2 into LDS
+        put_early(i + 2, e2);
+        // 2. step A on plane i + 1
+        f4 d2n;
+        const f4 x2n = step_a(i + 1, xa, e1, e2.cx, r1, d2n);
+        sx2[(i + 1) % 3][row][lane] = x2n;
+        __syncthreads();
+        // 3. loads for the next iteration, in flight during step B
+        const Early e3 = load_early(i + 3);
+        const Rest r2 = load_rest(i + 2);
+        // 4. step B on plane i (the tile's interior rows)
+        if (interior) {
+            const int bi = i % 3;
+            const f4 xc = sx2[bi][row][lane];
+            if (active(i, amask)) {
+                const f4 xym = sx2[bi][row - 1][lane], xyp = sx2[bi][row + 1][lane];
+                const f4 cym = scy[i & 3][row][lane], cyp = scy[i & 3][row + 1][lane];
+                const f4 dg = diag_at(i, cxa, e1.cx, cym, cyp, r0);
+                f4 dn;
+                const f4 x3 = cheb(xc, x2prev, x2n, xym, xyp, cxa, e1.cx, cym, cyp, r0.cz, r0.bv, dg, d2c,
+                                   cdB, crB, dn);
+                if (zon) *(f4 *)(xo + c0 + (unsigned)i * sx) = x3;
+            }
+            x2prev = xc;
+        }
+        // rotate
+        xa = e1.x;
+        cxa = e1.cx;
+        e1 = e2;
+        e2 = e3;
+        r0 = r1;
+        r1 = r2;
+        d2c = d2n;
+    }
+}
+
 // x += P e on active cells (the piecewise-constant coarse correction), in
 // place, before a plain first post-smoothing step: on large grids this pass
 // and the step cost less than the step that adds the correction at every
@@ -1248,10 +1494,73 @@ Launch pair_launch(Dims d, bool even_planes) {
     return L;
 }
 
+bool row_pair(Dims d) { return d.nz % 4 == 0 && d.nz > 192 && d.nz <= 256 && d.cells() < (size_t(1) << 32); }
+
+// Compute units of the current device (cached per device).
+static int device_cus() {
+    static int cus[64] = {0};
+    int dev = 0;
+    SFP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return 256;
+    if (!cus[dev]) {
+        int v = 0;
+        SFP_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+        cus[dev] = v > 0 ? v : 256;
+    }
+    return cus[dev];
+}
+
+// rowact[i ny + j] = 1 when the z row (i, j) holds an active cell: one
+// wavefront per row.
+__global__ __launch_bounds__(kBlock) void k_row_activity(Dims d, const float *__restrict__ diag32,
+                                                         unsigned char *__restrict__ act) {
+    const size_t row = ((size_t)blockIdx.x * kBlock + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    if (row >= (size_t)d.nx * d.ny) return;  // uniform over the wavefront
+    const size_t c = row * (size_t)d.nz;
+    bool a = false;
+    for (int k = lane; k < d.nz; k += 64) a = a || diag32[c + k] > 0.f;
+    const unsigned long long m = __ballot(a);
+    if (lane == 0) act[row] = m != 0ull ? 1 : 0;
+}
+
+void launch_row_activity(hipStream_t st, Dims d, const float *diag32, unsigned char *act) {
+    const size_t threads = (size_t)d.nx * d.ny * 64;
+    hipLaunchKernelGGL(k_row_activity, dim3((unsigned)((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                       d, diag32, act);
+    SFP_LAUNCH_CHECK();
+}
+
 void launch_smooth_pair(hipStream_t st, Dims d, Op32 A, const float *b, const float *x1, float *x3,
                         const float *d1, float cdA, float crA, float cdB, float crB,
-                        const double *gate) {
+                        const double *gate, const unsigned char *rowact) {
     if (x1 == x3) throw ArgError("launch_smooth_pair: x3 aliases x1");
+    bool rows = row_pair(d);
+    for (const float *a : {A.diag, A.cpl[0], A.cpl[1], A.cpl[2], b, x1, (const float *)x3, d1})
+        rows = rows && (uintptr_t)a % 16 == 0;
+    if (rows) {
+        // one block per compute unit, in one round: as many x runs per tile
+        // column as the columns leave CUs for (at most 56 planes a run, the
+        // activity masks' reach)
+        const unsigned ytiles = (unsigned)((d.ny + kRowPairY - 1) / kRowPairY);
+        const int runs_max = std::max(1, device_cus() / (int)ytiles);
+        int planes = std::min(56, std::max(1, (d.nx + runs_max - 1) / runs_max));
+        const unsigned runs = (unsigned)((d.nx + planes - 1) / planes);
+        const dim3 grid(ytiles * runs), block(64, kRowPairRows);
+        const bool dx = d1 == x1;
+        const unsigned char *ra = A.pinned_rowsum ? nullptr : rowact;
+#define SFP_ROWPAIR(RS, DX)                                                                      \
+    hipLaunchKernelGGL((k_smooth_pair_rows<RS, DX>), grid, block, 0, st, d, A, b, x1, x3, d1, cdA, \
+                       crA, cdB, crB, ra, planes, ytiles, gate)
+        if (A.pinned_rowsum) {
+            if (dx) SFP_ROWPAIR(true, true); else SFP_ROWPAIR(true, false);
+        } else {
+            if (dx) SFP_ROWPAIR(false, true); else SFP_ROWPAIR(false, false);
+        }
+#undef SFP_ROWPAIR
+        SFP_LAUNCH_CHECK();
+        return;
+    }
     const Launch L = pair_launch(d, false);
     if (A.pinned_rowsum)
         hipLaunchKernelGGL((k_smooth_pair<false, true>), L.grid, L.block, 0, st, L.cg, d, A, b, x1, x3,

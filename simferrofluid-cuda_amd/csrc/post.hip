@@ -8,6 +8,7 @@
 #include "device_util.hpp"
 #include "kernels.hpp"
 
+
 namespace sfp {
 namespace {
 
@@ -47,7 +48,9 @@ __global__ __launch_bounds__(kBlock) void k_velocity_valid(Dims d, int axis, con
 // One sweep of Extrapolation::Solve (core/Extrapolation.cpp:6-29): an invalid
 // point with valid neighbours takes their mean and becomes valid, one
 // without takes clear. Only invalid points are written and only valid ones
-// read, so the sweep is free of races.
+// read, so the sweep is free of races. Every entry of valid1 is written (a
+// point that stays invalid gets 0, what the reference's copy of the mask holds
+// there, since validity only grows), so valid1 needs no initial copy.
 __global__ __launch_bounds__(kBlock) void k_extrapolate(Dims g, double *data,
                                                         const unsigned char *__restrict__ valid,
                                                         unsigned char *__restrict__ valid1,
@@ -78,6 +81,7 @@ __global__ __launch_bounds__(kBlock) void k_extrapolate(Dims g, double *data,
         valid1[q] = 1;
     } else {
         data[q] = clear;
+        valid1[q] = 0;
     }
 }
 
@@ -154,8 +158,6 @@ void launch_velocity_valid(hipStream_t s, Dims d, int axis, const double *phi, c
 
 void launch_extrapolate(hipStream_t s, Dims g, double *data, unsigned char *valid,
                         unsigned char *scratch, double clear, int steps) {
-    // valid1 starts as a copy of valid (core/Extrapolation.cpp:5)
-    SFP_HIP(hipMemcpyAsync(scratch, valid, g.cells(), hipMemcpyDeviceToDevice, s));
     unsigned char *a = valid, *b = scratch;
     for (int it = 0; it < steps; it++) {
         hipLaunchKernelGGL(k_extrapolate, dim3(blocks_of(g.cells())), dim3(kBlock), 0, s, g, data, a, b,

@@ -128,6 +128,9 @@ struct Level {
     DevBuf<double> diag64, cpl64[3];
     DevBuf<float> diag32, cpl32[3];
     DevBuf<float> x0, x1, dv, b;  // ping-pong iterates, Chebyshev direction, rhs
+    // z rows (i, j) holding an active cell (levels whose step pairs run over
+    // whole rows; null elsewhere and where every row is active)
+    DevBuf<unsigned char> rowact;
     // both operators have the row-sum structure of Op64::pinned_rowsum and
     // Op32::pinned_rowsum, and kernels may rebuild the diagonals
     bool pinned_rowsum = false;
@@ -150,6 +153,9 @@ struct KrylovScalars {
     // `iters` counts the updates done
     double stop, iters, stop_rr;
     double alpha_prev;  // the previous iteration's alpha (lagged solution update)
+    // learned preconditioner in a batch: an update whose ||r||^2 exceeds
+    // stall_rr times the previous one raises `stalled` and `stop` (0: off)
+    double stall_rr, stalled;
 };
 
 // Per-kernel timing collected with HIP events on the context stream.

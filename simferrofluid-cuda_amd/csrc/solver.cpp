@@ -55,11 +55,17 @@ void KernelTimer::end(hipStream_t s) {
     used += 2;
 }
 void KernelTimer::harvest() {
-    for (size_t t = 0; t + 1 < used; t += 2) {
-        float ms = 0.f;
-        SFP_HIP(hipEventElapsedTime(&ms, ev[t], ev[t + 1]));
-        total_ms += ms;
-        total_bytes += bytes[t / 2];
+    // launches of a batch past its stop return at once: a duration under a
+    // fifth of the median is such a launch, not a measurement of the kernel
+    std::vector<float> ms(used / 2);
+    for (size_t t = 0; t + 1 < used; t += 2) SFP_HIP(hipEventElapsedTime(&ms[t / 2], ev[t], ev[t + 1]));
+    std::vector<float> sorted = ms;
+    std::sort(sorted.begin(), sorted.end());
+    const float med = sorted.empty() ? 0.f : sorted[sorted.size() / 2];
+    for (size_t q = 0; q < ms.size(); q++) {
+        if (ms[q] < 0.2f * med) continue;
+        total_ms += ms[q];
+        total_bytes += bytes[q];
         launches++;
     }
     used = 0;
@@ -152,6 +158,13 @@ void System::build_hierarchy(Workspace &ws, const SolverConfig &cfg) {
                        C.diag32.p, c32);
     }
     launch_coarse_invert(ws.stream, lv_.back().d, lv_.back().op64(), ainv_.p);
+    // the row-tiled step pairs skip the loads of z rows without active cells
+    for (Level &L : lv_) {
+        if (row_pair(L.d) && !L.pinned_rowsum) {
+            L.rowact.ensure((size_t)L.d.nx * L.d.ny);
+            launch_row_activity(ws.stream, L.d, L.diag32.p, L.rowact.p);
+        }
+    }
     // Every kernel skips groups of inactive cells (rows, tiles, pairs of
     // rows) and writes 0 at the inactive cells of the groups it processes
     // only if their inputs are 0 there; the groups differ between kernels.
@@ -256,7 +269,7 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
             // reads x0, b, diag, 3 couplings; writes x
             if (fine && ws.profiling) ws.timers[K_SMOOTH_PAIR].begin(st, (24.0 + pair_diag_bytes) * cells);
             launch_smooth_pair(st, L.d, A, L.b.p, cur, oth, cur, cb.cd[1], cb.cr[1], cb.cd[2], cb.cr[2],
-                               ws.gate);
+                               ws.gate, L.rowact.p);
             if (fine && ws.profiling) ws.timers[K_SMOOTH_PAIR].end(st);
             std::swap(cur, oth);
         } else {
@@ -320,7 +333,7 @@ float *System::post_smooth(Workspace &ws, const Smoothing &sm, int l, float *cur
         // reads x, b, d, diag (unless rebuilt), 3 couplings and writes x
         if (track) ws.timers[K_SMOOTH_PAIR].begin(st, (L.pinned_rowsum ? 28.0 : 32.0) * (double)L.d.cells());
         launch_smooth_pair(st, L.d, A, L.b.p, cur, oth, L.dv.p, cb.cd[deg - 2], cb.cr[deg - 2],
-                           cb.cd[deg - 1], cb.cr[deg - 1], ws.gate);
+                           cb.cd[deg - 1], cb.cr[deg - 1], ws.gate, L.rowact.p);
         if (track) ws.timers[K_SMOOTH_PAIR].end(st);
         std::swap(cur, oth);
     }
@@ -337,12 +350,13 @@ float *System::precondition(Workspace &ws, const SolverConfig &cfg, const double
 }
 
 float *System::learned_cycle(Workspace &ws, const SolverConfig &cfg, const double *r64, Cycle c) {
-    const Smoothing sm = Smoothing::make(cfg);
+    // the smoother the network was trained under
+    const Smoothing sm = Smoothing::make_learned(cfg);
     Level &L = lv_[0], &C = lv_[1];
     const hipStream_t st = ws.stream;
     float *z = vlevel(ws, sm, 0, r64);
     // residual the V-cycle leaves, restricted to the first coarse level
-    launch_restrict(st, L.d, L.op32(), L.b.p, z, C.b.p);
+    launch_restrict(st, L.d, L.op32(), L.b.p, z, C.b.p, ws.gate);
     if (c == kCycleRestrictOnly) return C.b.p;
     if (c == kCycleLearned) {
         network_correction(ws, C.x0.p);
@@ -384,10 +398,11 @@ void System::network_correction(Workspace &ws, float *x) {
     Level &C = lv_[1];
     const hipStream_t st = ws.stream;
     UNet &net = *ws.net;
-    launch_pack_residual32(st, C.d, C.b.p, C.diag32.p, ws.partials.p, ws.scalars.p, net.input_buffer());
-    net.forward(st, net.input_buffer(), net.output_buffer());
+    launch_pack_residual32(st, C.d, C.b.p, C.diag32.p, ws.partials.p, ws.scalars.p, net.input_buffer(),
+                           ws.gate);
+    net.forward(st, net.input_buffer(), net.output_buffer(), ws.gate);
     launch_scaled_correction(st, C.d, C.op32(), net.output_buffer(), C.b.p, ws.partials.p,
-                             ws.scalars.p, x);
+                             ws.scalars.p, x, ws.gate);
 }
 
 float *System::precondition_learned(Workspace &ws, const SolverConfig &cfg, const double *r64) {
@@ -421,7 +436,16 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     // reads the FP64 residual (the Jacobi preconditioner reads r).
     float *b32 = cfg.precond != 2 ? lv_[level].b.p : nullptr;
     float *x0 = b32 ? lv_[level].x0.p : nullptr;
-    const float c0 = Smoothing::make(cfg).at(level).cr[0];
+    // learned preconditioner until an iteration reduces the residual by less
+    // than stall_ratio; then the plain V-cycle with the plain smoother
+    bool learned = fine && cfg.precond == 1 && learned_ok(ws);
+    // the first Chebyshev iterate the residual and update kernels store is
+    // that of the cycle which reads it next
+    const Smoothing plain = Smoothing::make(cfg), net_sm = Smoothing::make_learned(cfg);
+    auto first_coef = [&](bool l) {
+        return ((l || (fine && cfg.precond == 3)) ? net_sm : plain).at(level).cr[0];
+    };
+    float c0 = first_coef(learned);
     launch_residual64(st, d, A, rhs_.p, warm ? x_.p : nullptr, r_.p, ws.partials.p,
                       ws.scalars.p, b32, x0, lv_[level].diag32.p, c0);
     ws.read_scalars();
@@ -436,9 +460,6 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     bool first = true;
     bool checked = true;  // rel is a true residual
     const double cells = (double)n;
-    // learned preconditioner until an iteration reduces the residual by less
-    // than stall_ratio; then the multigrid V-cycle
-    bool learned = fine && cfg.precond == 1 && learned_ok(ws);
     if (learned) prepare_learned(ws);
     bool b_ready = b32 != nullptr;  // lv_[level].b holds r rounded (and lv_[level].x0 its first iterate)
     int next_batch = kBatch;        // iterations to enqueue before the host reads the residual
@@ -470,13 +491,14 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
         // Up to kBatch iterations are enqueued before the host looks at the
         // residual (after the first batch, as many as the residual's rate of
         // decrease predicts are left): the update's finalize raises `stop` on
-        // the device once ||r|| <= tol ||b||, and every kernel of the
-        // remaining iterations of the batch then returns at once. The learned
-        // path (its stall test needs every residual) and profiled solves run
-        // one at a time.
-        const int batch = (learned || profiling) ? 1 : std::min(next_batch, cfg.max_iters - info.iterations);
+        // the device once ||r|| <= tol ||b||, or (learned preconditioner)
+        // once an iteration reduced ||r|| by less than stall_ratio, and every
+        // kernel of the remaining iterations of the batch then returns at
+        // once (profiled launches among them are dropped from the timings).
+        const int batch = std::min(next_batch, cfg.max_iters - info.iterations);
         if (batch > 1) {
-            launch_batch_begin(st, ws.scalars.p, cfg.tol * cfg.tol * bb);
+            launch_batch_begin(st, ws.scalars.p, cfg.tol * cfg.tol * bb,
+                               learned ? cfg.stall_ratio * cfg.stall_ratio : 0.0);
             ws.gate = &ws.scalars.p->stop;
         }
         for (int it = 0; it < batch; it++) {
@@ -514,6 +536,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
             b_ready = b32 != nullptr;
         }
         ws.read_scalars();
+        const bool stalled = batch > 1 && ws.host_sc->stalled != 0.0;
         ws.gate = nullptr;
         const int done = batch > 1 ? (int)ws.host_sc->iters : 1;
         run += done;
@@ -531,7 +554,20 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
             const double left = std::log(cfg.tol / rel) / std::log(rate);
             if (left < kBatch) next_batch = std::max(1, (int)std::ceil(left));
         }
-        if (learned && rel > cfg.stall_ratio * prev) learned = false;  // fall back to multigrid
+        if (learned && (stalled || (batch == 1 && rel > cfg.stall_ratio * prev))) {
+            // fall back to the plain V-cycle, restarting the recurrence from
+            // the current iterate (a direction built with the other
+            // preconditioner slows the V-cycle's iterations down); the first
+            // iterate stored by the last update belongs to the learned
+            // cycle's smoother
+            learned = false;
+            bring_x_up_to_date();
+            first = true;
+            const float c0_plain = first_coef(false);
+            if (c0_plain != c0) b_ready = false;
+            c0 = c0_plain;
+            next_batch = kBatch;
+        }
     }
     bring_x_up_to_date();
     if (!checked) {

@@ -26,6 +26,11 @@ struct SolverConfig {
     double coarse_scale = 0.5;
     double stall_ratio = 0.5;
     int verbose = 0;
+    // the learned cycle's smoother (0: the plain cycle's value)
+    int net_cheby_degree = 0;
+    double net_cheby_ratio = 0.0;
+    int net_coarse_cheby_degree = 0;
+    double net_coarse_cheby_ratio = 0.0;
 };
 
 // Chebyshev recurrence coefficients for D^{-1}A on [lmax/ratio, lmax].
@@ -41,6 +46,15 @@ struct Smoothing {
     static Smoothing make(const SolverConfig &cfg) {
         return Smoothing{Cheby::make(cfg.cheby_degree, cfg.cheby_ratio),
                          Cheby::make(cfg.coarse_cheby_degree, cfg.coarse_cheby_ratio)};
+    }
+    // the learned cycle's: the settings its network was trained under
+    static Smoothing make_learned(const SolverConfig &cfg) {
+        auto pick = [](double v, double plain) { return v > 0 ? v : plain; };
+        return Smoothing{
+            Cheby::make((int)pick(cfg.net_cheby_degree, cfg.cheby_degree),
+                        pick(cfg.net_cheby_ratio, cfg.cheby_ratio)),
+            Cheby::make((int)pick(cfg.net_coarse_cheby_degree, cfg.coarse_cheby_degree),
+                        pick(cfg.net_coarse_cheby_ratio, cfg.coarse_cheby_ratio))};
     }
 };
 

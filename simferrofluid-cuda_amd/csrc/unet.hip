@@ -39,6 +39,7 @@ struct ConvArgs {
     int B, X, Y, Z;
     int relu;
     int up;  // 1: `in` is [B][X/2][Y/2][Z/2][CIN], read nearest-up-sampled
+    const double *gate;  // nonzero *gate: return at once (a stopped batch of FCG iterations)
 };
 
 template <int CIN>
@@ -69,6 +70,7 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
     constexpr int NB = COUT / 16;
     constexpr int RB = TX;  // 16-row blocks per wavefront
     __shared__ __attribute__((aligned(16))) uint16_t tile[HX * HY * HZ * CIN];
+    if (a.gate && *a.gate != 0.0) return;
     const int tid = threadIdx.x;
     const int tx_n = (a.X + TX - 1) / TX, ty_n = (a.Y + 3) / 4, tz_n = (a.Z + 15) / 16;
     unsigned t = blockIdx.x;
@@ -156,39 +158,61 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
         }
 }
 
-// Weight gradient partials: for tap blockIdx.y and voxel chunk blockIdx.x,
-// part[chunk][tap][cout][cin] = sum_v g[v][cout] x[v + tap][cin]; the centre-tap
-// blocks also store bias partials bpart[chunk][cout] = sum_v g[v][cout].
+// Weight gradient partials as an implicit GEMM on MFMA: for tap blockIdx.y
+// and voxel chunk blockIdx.x, part[chunk][tap][cout][cin] =
+// sum_v g[v][cout] x[v + tap][cin] (M = output channels, N = input channels,
+// K = the chunk's voxels; BF16 products, exact in FP32, FP32 accumulation);
+// the centre-tap blocks also store bias partials bpart[chunk][cout] =
+// sum_v g[v][cout]. Every K step stages 128 voxels of g and of the shifted
+// x in LDS, transposed (channel-major, 8 consecutive voxels per 16-byte
+// fragment); wavefront w takes the step's voxels 32 w .. 32 w + 31 for every
+// output tile, and the four partial sets are added in a fixed order at the
+// end, so the result does not depend on timing. Input channels below 16 are
+// padded with zeros.
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_conv_dw(const uint16_t *__restrict__ x,
                                                  const uint16_t *__restrict__ g, int B, int X,
                                                  int Y, int Z, int chunk_len,
                                                  float *__restrict__ part,
                                                  float *__restrict__ bpart) {
-    constexpr int V = 32;  // voxels staged per round
+    constexpr int KV = 128;                       // voxels per K step (4 x 32)
+    constexpr int CINP = CIN < 16 ? 16 : CIN;     // padded to the MFMA's 16 columns
+    constexpr int MB = COUT / 16, NB = CINP / 16, T = MB * NB;
     constexpr int PAIRS = CIN * COUT;
-    constexpr int PER = (PAIRS + 255) / 256;
-    __shared__ float sg[V][COUT];
-    __shared__ float sx[V][CIN];
+    __shared__ __attribute__((aligned(16))) uint16_t sg[COUT][KV];
+    __shared__ __attribute__((aligned(16))) uint16_t sx[CINP][KV];
+    __shared__ float red[4][T][4][64];            // each wavefront's partial tiles
     const int tap = blockIdx.y;
     const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
     const size_t nv = (size_t)B * X * Y * Z;
     const size_t v0 = (size_t)blockIdx.x * chunk_len;
     const size_t v1 = v0 + chunk_len < nv ? v0 + chunk_len : nv;
-    float acc[PER];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int gq = lane >> 4, r16 = lane & 15;
+    v4f acc[T];
 #pragma unroll
-    for (int p = 0; p < PER; p++) acc[p] = 0.f;
+    for (int t = 0; t < T; t++) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
     float bacc = 0.f;
-    for (size_t vb = v0; vb < v1; vb += V) {
-        for (int e = threadIdx.x; e < V * COUT; e += 256) {
-            const int i = e / COUT, c = e - i * COUT;
-            const size_t v = vb + i;
-            sg[i][c] = v < v1 ? bf2f(g[v * COUT + c]) : 0.f;
+    if (CINP > CIN) {  // the padding columns stay zero
+        for (int e = tid; e < (CINP - CIN) * KV; e += 256) sx[CIN + e / KV][e % KV] = 0;
+    }
+    for (size_t vb = v0; vb < v1; vb += KV) {
+        __syncthreads();  // the previous step's fragments are read
+        // g: 8 channels (16 bytes) of one voxel per thread and round
+        for (int e = tid; e < KV * (COUT / 8); e += 256) {
+            const int vi = e / (COUT / 8), q = e - vi * (COUT / 8);
+            const size_t v = vb + vi;
+            uint4 u = make_uint4(0, 0, 0, 0);
+            if (v < v1) u = *reinterpret_cast<const uint4 *>(g + v * COUT + 8 * q);
+            const uint16_t *h = reinterpret_cast<const uint16_t *>(&u);
+#pragma unroll
+            for (int c = 0; c < 8; c++) sg[8 * q + c][vi] = h[c];
         }
-        for (int e = threadIdx.x; e < V * CIN; e += 256) {
-            const int i = e / CIN, c = e - i * CIN;
-            const size_t v = vb + i;
-            float val = 0.f;
+        // x at the tapped voxel (zero outside the volume)
+        for (int e = tid; e < KV * (CIN / 8); e += 256) {
+            const int vi = e / (CIN / 8), q = e - vi * (CIN / 8);
+            const size_t v = vb + vi;
+            uint4 u = make_uint4(0, 0, 0, 0);
             if (v < v1) {
                 size_t r = v;
                 const int z = (int)(r % Z);
@@ -196,34 +220,50 @@ __global__ __launch_bounds__(256) void k_conv_dw(const uint16_t *__restrict__ x,
                 const int y = (int)(r % Y);
                 r /= Y;
                 const int xx = (int)(r % X);
-                const size_t b = r / X;
+                const size_t bb = r / X;
                 const int jx = xx + dx, jy = y + dy, jz = z + dz;
                 if (jx >= 0 && jy >= 0 && jz >= 0 && jx < X && jy < Y && jz < Z)
-                    val = bf2f(x[((((b * X + jx) * Y + jy) * Z + jz)) * CIN + c]);
+                    u = *reinterpret_cast<const uint4 *>(x + (((bb * X + jx) * Y + jy) * Z + jz) * CIN + 8 * q);
             }
-            sx[i][c] = val;
+            const uint16_t *h = reinterpret_cast<const uint16_t *>(&u);
+#pragma unroll
+            for (int c = 0; c < 8; c++) sx[8 * q + c][vi] = h[c];
         }
         __syncthreads();
+        // this wavefront's 32 voxels of the step: k = 32 wave + 8 gq .. + 7
+        const int k0 = 32 * wave + 8 * gq;
+        v8bf af[MB], bf[NB];
 #pragma unroll
-        for (int p = 0; p < PER; p++) {
-            const int pair = threadIdx.x + 256 * p;
-            if (pair < PAIRS) {
-                const int co = pair / CIN, ci = pair - co * CIN;
-                float s = acc[p];
-                for (int i = 0; i < V; i++) s += sg[i][co] * sx[i][ci];
-                acc[p] = s;
-            }
+        for (int mb = 0; mb < MB; mb++)
+            af[mb] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4 *>(&sg[16 * mb + r16][k0]));
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++)
+            bf[nb] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4 *>(&sx[16 * nb + r16][k0]));
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++)
+                acc[mb * NB + nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb], bf[nb], acc[mb * NB + nb], 0, 0, 0);
+        if (tap == 13 && tid < COUT) {
+            float sb = 0.f;
+            for (int i = 0; i < KV; i++) sb += bf2f(sg[tid][i]);
+            bacc += sb;
         }
-        if (tap == 13 && threadIdx.x < COUT)
-            for (int i = 0; i < V; i++) bacc += sg[i][threadIdx.x];
-        __syncthreads();
     }
+    // add the four wavefronts' tiles in a fixed order
 #pragma unroll
-    for (int p = 0; p < PER; p++) {
-        const int pair = threadIdx.x + 256 * p;
-        if (pair < PAIRS) part[((size_t)blockIdx.x * 27 + tap) * PAIRS + pair] = acc[p];
+    for (int t = 0; t < T; t++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) red[wave][t][j][lane] = acc[t][j];
+    __syncthreads();
+    for (int e = tid; e < T * 4 * 64; e += 256) {
+        const int t = e / 256, j = (e / 64) & 3, l = e & 63;
+        const float sum = ((red[0][t][j][l] + red[1][t][j][l]) + red[2][t][j][l]) + red[3][t][j][l];
+        // C[row][col] of tile t: row = 4 (l >> 4) + j (output channel), col = l & 15 (input channel)
+        const int co = 16 * (t / NB) + 4 * (l >> 4) + j, ci = 16 * (t % NB) + (l & 15);
+        if (ci < CIN) part[((size_t)blockIdx.x * 27 + tap) * PAIRS + co * CIN + ci] = sum;
     }
-    if (tap == 13 && threadIdx.x < COUT) bpart[(size_t)blockIdx.x * COUT + threadIdx.x] = bacc;
+    if (tap == 13 && tid < COUT) bpart[(size_t)blockIdx.x * COUT + tid] = bacc;
 }
 
 // out[i] += sum over chunks of part[chunk][i] (fixed order).
@@ -237,7 +277,9 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(const float *__restrict__
 }
 
 __global__ __launch_bounds__(256) void k_pool2(const uint16_t *__restrict__ in, int B, int X, int Y,
-                                               int Z, int C, uint16_t *__restrict__ out) {
+                                               int Z, int C, uint16_t *__restrict__ out,
+                                               const double *gate) {
+    if (gate && *gate != 0.0) return;
     const int Xo = X / 2, Yo = Y / 2, Zo = Z / 2;
     const size_t n = (size_t)B * Xo * Yo * Zo * C;
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -334,7 +376,8 @@ __global__ __launch_bounds__(256) void k_relu_mask(const float *__restrict__ g,
 
 // z[v] = bo + sum_c wo[c] d0[v][c]
 __global__ __launch_bounds__(256) void k_head_fwd(const uint16_t *__restrict__ d0, const float *__restrict__ wb,
-                                                  size_t nv, float *__restrict__ z) {
+                                                  size_t nv, float *__restrict__ z, const double *gate) {
+    if (gate && *gate != 0.0) return;
     const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (v >= nv) return;
     float s = wb[16];
@@ -471,9 +514,10 @@ void dw_dispatch(hipStream_t st, const uint16_t *x, const uint16_t *g, int B, in
 
 void unet_conv(hipStream_t st, int cin, int cout, const uint16_t *in, const uint16_t *w,
                const float *bias, const uint16_t *skip, uint16_t *out, float *out32,
-               uint16_t *relu_copy, int B, int X, int Y, int Z, bool relu, bool up_input) {
+               uint16_t *relu_copy, int B, int X, int Y, int Z, bool relu, bool up_input,
+               const double *gate) {
     if (up_input && (X % 2 || Y % 2 || Z % 2)) throw ArgError("up-sampled input needs even extents");
-    ConvArgs a{in, w, bias, skip, out, out32, relu_copy, B, X, Y, Z, relu ? 1 : 0, up_input ? 1 : 0};
+    ConvArgs a{in, w, bias, skip, out, out32, relu_copy, B, X, Y, Z, relu ? 1 : 0, up_input ? 1 : 0, gate};
 #define SFP_CONV(CI, CO) \
     if (cin == CI && cout == CO) return conv_dispatch<CI, CO>(st, a);
     SFP_CONV(8, 16)
@@ -513,9 +557,10 @@ void unet_conv_dw(hipStream_t st, int cin, int cout, const uint16_t *x, const ui
     SFP_LAUNCH_CHECK();
 }
 
-void unet_pool2(hipStream_t st, const uint16_t *in, int B, int X, int Y, int Z, int C, uint16_t *out) {
+void unet_pool2(hipStream_t st, const uint16_t *in, int B, int X, int Y, int Z, int C, uint16_t *out,
+                const double *gate) {
     const size_t n = (size_t)B * (X / 2) * (Y / 2) * (Z / 2) * C;
-    hipLaunchKernelGGL(k_pool2, dim3(blocks_for(n)), dim3(256), 0, st, in, B, X, Y, Z, C, out);
+    hipLaunchKernelGGL(k_pool2, dim3(blocks_for(n)), dim3(256), 0, st, in, B, X, Y, Z, C, out, gate);
     SFP_LAUNCH_CHECK();
 }
 
@@ -543,8 +588,9 @@ void unet_relu_mask(hipStream_t st, const float *g, const uint16_t *act, size_t 
     SFP_LAUNCH_CHECK();
 }
 
-void unet_head_fwd(hipStream_t st, const uint16_t *d0, const float *wb, size_t nv, float *z) {
-    hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(nv)), dim3(256), 0, st, d0, wb, nv, z);
+void unet_head_fwd(hipStream_t st, const uint16_t *d0, const float *wb, size_t nv, float *z,
+                   const double *gate) {
+    hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(nv)), dim3(256), 0, st, d0, wb, nv, z, gate);
     SFP_LAUNCH_CHECK();
 }
 

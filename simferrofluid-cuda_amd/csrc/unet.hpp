@@ -31,19 +31,24 @@ long long unet_param_count();
 
 // launchers (unet.hip)
 // up_input: `in` holds the half-resolution tensor, read nearest-up-sampled
-// (the up-sampling fused into the staging)
+// (the up-sampling fused into the staging). gate (the forward's kernels): the
+// launch returns at once while *gate is nonzero (a stopped batch of FCG
+// iterations).
 void unet_conv(hipStream_t st, int cin, int cout, const uint16_t *in, const uint16_t *w,
                const float *bias, const uint16_t *skip, uint16_t *out, float *out32,
-               uint16_t *relu_copy, int B, int X, int Y, int Z, bool relu, bool up_input = false);
+               uint16_t *relu_copy, int B, int X, int Y, int Z, bool relu, bool up_input = false,
+               const double *gate = nullptr);
 void unet_conv_dw(hipStream_t st, int cin, int cout, const uint16_t *x, const uint16_t *g, int B,
                   int X, int Y, int Z, float *part, float *dw, float *db);
-void unet_pool2(hipStream_t st, const uint16_t *in, int B, int X, int Y, int Z, int C, uint16_t *out);
+void unet_pool2(hipStream_t st, const uint16_t *in, int B, int X, int Y, int Z, int C, uint16_t *out,
+                const double *gate = nullptr);
 void unet_pool2_bwd(hipStream_t st, const float *g, const float *add, int B, int X, int Y, int Z,
                     int C, float *gin);
 void unet_up2(hipStream_t st, const uint16_t *in, int B, int Xc, int Yc, int Zc, int C, uint16_t *out);
 void unet_up2_bwd(hipStream_t st, const float *g, int B, int Xc, int Yc, int Zc, int C, float *gc);
 void unet_relu_mask(hipStream_t st, const float *g, const uint16_t *act, size_t n, uint16_t *g16);
-void unet_head_fwd(hipStream_t st, const uint16_t *d0, const float *wb, size_t nv, float *z);
+void unet_head_fwd(hipStream_t st, const uint16_t *d0, const float *wb, size_t nv, float *z,
+                   const double *gate = nullptr);
 void unet_head_bwd(hipStream_t st, const uint16_t *d0, const float *wb, const float *dz, size_t nv,
                    float *gd0, double *part, double *sums, float *gwb);
 void unet_mse(hipStream_t st, const float *z, const float *t, const float *m, size_t nv,
@@ -70,8 +75,12 @@ class UNet {
     // activations for volumes of this shape (each axis a multiple of 4)
     void ensure(const UNetShape &s, bool training);
     const UNetShape &shape() const { return shape_; }
-    // x8: BF16 [B][X][Y][Z][8]; z: FP32 [B][X][Y][Z]
-    void forward(hipStream_t st, const uint16_t *x8, float *z);
+    // x8: BF16 [B][X][Y][Z][8]; z: FP32 [B][X][Y][Z]; every launch returns at
+    // once while *gate is nonzero (gate null: always runs)
+    void forward(hipStream_t st, const uint16_t *x8, float *z, const double *gate = nullptr);
+    // profiling: with events (2 per convolution, kUNetConvs pairs), every
+    // convolution of the forward is bracketed by its pair; null: none
+    void set_conv_events(hipEvent_t *events) { conv_events_ = events; }
     // masked MSE against t (FP32 [voxels]); returns the loss on the host
     double loss_and_backward(hipStream_t st, const uint16_t *x8, const float *t, const float *m,
                              double *host_sums);
@@ -83,8 +92,9 @@ class UNet {
   private:
     void cast_weights(hipStream_t st);
     void conv(hipStream_t st, int k, int level, const uint16_t *in, const uint16_t *skip,
-              uint16_t *out, uint16_t *relu_copy, bool up_input = false);
+              uint16_t *out, uint16_t *relu_copy, bool up_input = false, const double *gate = nullptr);
     int dims(int level, int axis) const;
+    hipEvent_t *conv_events_ = nullptr;
     long long off_w_[kUNetConvs], off_b_[kUNetConvs], off_head_;
     DevBuf<float> w_, grad_, m_, v_;
     DevBuf<uint16_t> wf_[kUNetConvs], wt_[kUNetConvs];

@@ -114,39 +114,41 @@ void UNet::ensure(const UNetShape &s, bool training) {
 }
 
 void UNet::conv(hipStream_t st, int k, int level, const uint16_t *in, const uint16_t *skip,
-                uint16_t *out, uint16_t *relu_copy, bool up_input) {
+                uint16_t *out, uint16_t *relu_copy, bool up_input, const double *gate) {
+    if (conv_events_) SFP_HIP(hipEventRecord(conv_events_[2 * k], st));
     unet_conv(st, kConvCin[k], kConvCout[k], in, wf_[k].p, w_.p + off_b_[k], skip, out, nullptr, relu_copy,
-              shape_.B, dims(level, 0), dims(level, 1), dims(level, 2), true, up_input);
+              shape_.B, dims(level, 0), dims(level, 1), dims(level, 2), true, up_input, gate);
+    if (conv_events_) SFP_HIP(hipEventRecord(conv_events_[2 * k + 1], st));
 }
 
-void UNet::forward(hipStream_t st, const uint16_t *x8, float *z) {
+void UNet::forward(hipStream_t st, const uint16_t *x8, float *z, const double *gate) {
     const int B = shape_.B;
     uint16_t *keep8 = training_ ? r8_.p : nullptr, *keep6 = training_ ? r6_.p : nullptr;
-    conv(st, 0, 0, x8, nullptr, a0_.p, nullptr);
-    conv(st, 1, 0, a0_.p, nullptr, e0_.p, nullptr);
-    unet_pool2(st, e0_.p, B, dims(0, 0), dims(0, 1), dims(0, 2), 16, p0_.p);
-    conv(st, 2, 1, p0_.p, nullptr, a1_.p, nullptr);
-    conv(st, 3, 1, a1_.p, nullptr, e1_.p, nullptr);
-    unet_pool2(st, e1_.p, B, dims(1, 0), dims(1, 1), dims(1, 2), 32, p1_.p);
-    conv(st, 4, 2, p1_.p, nullptr, a2_.p, nullptr);
-    conv(st, 5, 2, a2_.p, nullptr, e2_.p, nullptr);
+    conv(st, 0, 0, x8, nullptr, a0_.p, nullptr, false, gate);
+    conv(st, 1, 0, a0_.p, nullptr, e0_.p, nullptr, false, gate);
+    unet_pool2(st, e0_.p, B, dims(0, 0), dims(0, 1), dims(0, 2), 16, p0_.p, gate);
+    conv(st, 2, 1, p0_.p, nullptr, a1_.p, nullptr, false, gate);
+    conv(st, 3, 1, a1_.p, nullptr, e1_.p, nullptr, false, gate);
+    unet_pool2(st, e1_.p, B, dims(1, 0), dims(1, 1), dims(1, 2), 32, p1_.p, gate);
+    conv(st, 4, 2, p1_.p, nullptr, a2_.p, nullptr, false, gate);
+    conv(st, 5, 2, a2_.p, nullptr, e2_.p, nullptr, false, gate);
     // training keeps the up-sampled tensors (the weight gradients read them);
     // inference reads the half-resolution ones through the convolution
     if (training_) {
         unet_up2(st, e2_.p, B, dims(2, 0), dims(2, 1), dims(2, 2), 64, u1_.p);
-        conv(st, 6, 1, u1_.p, e1_.p, s1_.p, keep6);
+        conv(st, 6, 1, u1_.p, e1_.p, s1_.p, keep6, false, gate);
     } else {
-        conv(st, 6, 1, e2_.p, e1_.p, s1_.p, keep6, true);
+        conv(st, 6, 1, e2_.p, e1_.p, s1_.p, keep6, true, gate);
     }
-    conv(st, 7, 1, s1_.p, nullptr, d1_.p, nullptr);
+    conv(st, 7, 1, s1_.p, nullptr, d1_.p, nullptr, false, gate);
     if (training_) {
         unet_up2(st, d1_.p, B, dims(1, 0), dims(1, 1), dims(1, 2), 32, u0_.p);
-        conv(st, 8, 0, u0_.p, e0_.p, s0_.p, keep8);
+        conv(st, 8, 0, u0_.p, e0_.p, s0_.p, keep8, false, gate);
     } else {
-        conv(st, 8, 0, d1_.p, e0_.p, s0_.p, keep8, true);
+        conv(st, 8, 0, d1_.p, e0_.p, s0_.p, keep8, true, gate);
     }
-    conv(st, 9, 0, s0_.p, nullptr, d0_.p, nullptr);
-    unet_head_fwd(st, d0_.p, w_.p + off_head_, shape_.voxels(0), z);
+    conv(st, 9, 0, s0_.p, nullptr, d0_.p, nullptr, false, gate);
+    unet_head_fwd(st, d0_.p, w_.p + off_head_, shape_.voxels(0), z, gate);
 }
 
 double UNet::loss_and_backward(hipStream_t st, const uint16_t *x8, const float *t, const float *m,

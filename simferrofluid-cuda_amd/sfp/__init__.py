@@ -36,7 +36,8 @@ EXPORTS = {
               "sfp_pressure_project_device", "sfp_pressure_reproject_device",
               "sfp_magnetic_solve_device", "sfp_extrapolate_faces", "sfp_extrapolate_velocity",
               "sfp_collider_enforce", "sfp_extrapolate_velocity_device",
-              "sfp_collider_enforce_device"],
+              "sfp_collider_enforce_device",
+              "sfp_projection_substep"],
     "sfp_profile.h": ["sfp_profile_enable", "sfp_profile_query"],
     "sfp_test.h": ["sfp_test_num_levels", "sfp_test_level_dims", "sfp_test_export_level",
                    "sfp_test_export_level32", "sfp_test_apply64", "sfp_test_precondition",
@@ -47,7 +48,8 @@ EXPORTS = {
     "sfp_learned.h": ["sfp_unet_param_count", "sfp_set_network", "sfp_learned_features",
                       "sfp_unet_create", "sfp_unet_destroy", "sfp_unet_set_params",
                       "sfp_unet_get_params", "sfp_unet_init_params", "sfp_unet_forward",
-                      "sfp_unet_gradients", "sfp_unet_train_step_device", "sfp_unet_last_error"],
+                      "sfp_unet_gradients", "sfp_unet_train_step_device", "sfp_unet_last_error",
+                      "sfp_unet_profile_forward"],
 }
 
 
@@ -55,7 +57,9 @@ class Params(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("precond", C.c_int),
                 ("mg_coarsest", C.c_int), ("cheby_degree", C.c_int), ("cheby_ratio", C.c_double),
                 ("coarse_scale", C.c_double), ("stall_ratio", C.c_double), ("verbose", C.c_int),
-                ("coarse_cheby_degree", C.c_int), ("coarse_cheby_ratio", C.c_double)]
+                ("coarse_cheby_degree", C.c_int), ("coarse_cheby_ratio", C.c_double),
+                ("net_cheby_degree", C.c_int), ("net_cheby_ratio", C.c_double),
+                ("net_coarse_cheby_degree", C.c_int), ("net_coarse_cheby_ratio", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -122,6 +126,9 @@ def lib():
         for name in ("sfp_extrapolate_velocity", "sfp_extrapolate_velocity_device"):
             getattr(L, name).restype = C.c_int
             getattr(L, name).argtypes = [vp, PP, _dp, PP, C.c_int]
+        L.sfp_projection_substep.restype = C.c_int
+        L.sfp_projection_substep.argtypes = [vp, PP, _dp, PP, PP, vp, C.c_double, C.c_int, PP, PP, _dp,
+                                             C.POINTER(Stats)]
         for name in ("sfp_collider_enforce", "sfp_collider_enforce_device"):
             getattr(L, name).restype = C.c_int
             getattr(L, name).argtypes = [vp, PP, PP, PP, PP]
@@ -183,6 +190,8 @@ def lib():
         L.sfp_unet_gradients.argtypes = [vp, _fp, _fp, _fp, _dp, _fp]
         L.sfp_unet_train_step_device.restype = C.c_int
         L.sfp_unet_train_step_device.argtypes = [vp, _fp, _fp, _fp, C.c_float, _dp]
+        L.sfp_unet_profile_forward.restype = C.c_int
+        L.sfp_unet_profile_forward.argtypes = [vp, C.c_int, _dp, _dp]
         L.sfp_unet_last_error.restype = C.c_char_p
         L.sfp_unet_last_error.argtypes = [vp]
         _LIB = L
@@ -395,6 +404,29 @@ class Context:
         self._check(lib().sfp_collider_enforce(self._h, _p3(out), *[_p3(g) for g in arrs]))
         return out
 
+    def projection_substep(self, sc, normal, collider_velocity, steps=6, want_pressure=False):
+        """sfp_projection_substep: the projection, the velocity extrapolation
+        and Collider::Enforce with the velocity kept on the device; returns
+        (velocity[3], pressure or None, stats) on copies of sc's arrays."""
+        for name, arrs in (("velocity", sc["velocity"]), ("fraction", sc["fraction"]),
+                           ("solid_velocity", sc["solid_velocity"]), ("jump", sc.get("jump")),
+                           ("normal", normal), ("collider_velocity", collider_velocity)):
+            self._faces(arrs, name)
+        self._cells(sc["phi"], "phi")
+        vel = [np.array(v, dtype=np.float64, copy=True, order="C") for v in sc["velocity"]]
+        c64 = lambda arrs: [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]  # noqa: E731
+        frac, us, nrm, cv = c64(sc["fraction"]), c64(sc["solid_velocity"]), c64(normal), c64(collider_velocity)
+        jump = sc.get("jump")
+        jump = None if jump is None else c64(jump)
+        phi = np.ascontiguousarray(sc["phi"], dtype=np.float64)
+        p = np.zeros(phi.shape) if want_pressure else None
+        st = Stats()
+        self._check(lib().sfp_projection_substep(
+            self._h, _p3(vel), _d(phi), _p3(frac), _p3(us),
+            C.cast(_p3(jump), C.c_void_p) if jump is not None else None,
+            float(sc.get("vol_error", 0.0)), int(steps), _p3(nrm), _p3(cv), _d(p), C.byref(st)))
+        return vel, p, st.as_dict()
+
     def extrapolate_velocity_device(self, vel, phi, fraction, steps=6):
         self._check(lib().sfp_extrapolate_velocity_device(self._h, _p3(vel), _d(phi), _p3(fraction),
                                                           int(steps)))
@@ -520,7 +552,8 @@ class Context:
         kernel and from two single steps: (x3_pair, x3_steps), FP32."""
         shape = self.level_dims(which, level)
         f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32).reshape(shape)  # noqa: E731
-        b, x1, d1 = f32(b), f32(x1), f32(d1)
+        b, x1 = f32(b), f32(x1)
+        d1 = None if d1 is None else f32(d1)  # None: d1 is x1 (first pre-smoothing passes)
         x3p = np.zeros(shape, np.float32)
         x3s = np.zeros(shape, np.float32)
         self._check(lib().sfp_test_smooth_pair(self._h, int(which), int(level), _f(b), _f(x1), _f(d1),
@@ -626,6 +659,14 @@ class UNet:
         self._check(lib().sfp_unet_gradients(self._h, _f(inp), _f(target), _f(mask), C.byref(loss),
                                              _f(g)))
         return loss.value, g
+
+    def profile_forward(self, reps=10):
+        """(ms per forward pass, [ms per convolution] * 10) on the network's
+        device-resident input, timed with HIP events."""
+        fwd = C.c_double(0)
+        conv = np.zeros(10)
+        self._check(lib().sfp_unet_profile_forward(self._h, int(reps), C.byref(fwd), _d(conv)))
+        return fwd.value, conv
 
     def train_step_device(self, inp, target, mask, lr):
         """One Adam step on device arrays (int pointers); returns the loss before it."""

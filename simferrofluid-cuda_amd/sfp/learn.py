@@ -38,7 +38,7 @@ import time
 
 import numpy as np
 
-from . import Context, UNet, PRECOND_MG, unet_param_count
+from . import Context, UNet, PRECOND_LEARNED, PRECOND_MG, unet_param_count
 from . import scenes
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "weights",
@@ -49,8 +49,20 @@ SMOOTHER_KEYS = ("cheby_degree", "cheby_ratio", "coarse_cheby_degree", "coarse_c
 
 
 def smoother(meta):
-    """The smoother settings recorded with trained weights, as set_params keywords."""
-    return {k: meta["smoother"][k] for k in SMOOTHER_KEYS}
+    """The smoother settings recorded with trained weights, as the set_params
+    keywords of the learned cycle's smoother (net_cheby_degree, ...): the
+    learned cycle runs with them, the plain V-cycle after a fall-back with the
+    context's own."""
+    return {"net_" + k: meta["smoother"][k] for k in SMOOTHER_KEYS}
+
+
+def learned_params(meta):
+    """set_params keywords of the learned preconditioner with these weights:
+    the trained-under smoother and the fall-back threshold recorded with them
+    (stall_ratio, from the plain V-cycle's measured per-iteration residual
+    reduction; 0.5 for weights recorded without one)."""
+    return dict(precond=PRECOND_LEARNED, stall_ratio=float(meta.get("stall_ratio", 0.5)),
+                **smoother(meta))
 
 
 # ------------------------------------------------------------------ scenes

@@ -170,9 +170,10 @@ def uniform_field(geo: Geometry, h):
     return [np.full(geo.face_shape(a), float(h[a])) for a in range(3)]
 
 
-def magnetic_scene(n: int = 128, seed: int = 0, chi: float = 2.0, band_cells: float = 1.5):
-    """Cosine-perturbed interface y = 0.3 + sum of 8 modes, H_ext = (0,1,0)."""
-    geo = Geometry(n, 1.0 / n)
+def magnetic_scene(n=128, seed: int = 0, chi: float = 2.0, band_cells: float = 1.5):
+    """Cosine-perturbed interface y = 0.3 + sum of 8 modes, H_ext = (0,1,0);
+    n is the cells per axis (or (nx, ny, nz), spacing 1 / max)."""
+    geo = Geometry(n, 1.0 / (n if np.isscalar(n) else max(n)))
     phi = _wavy_phi(geo, 0.3, _modes(seed, 8, 0.01))
     return dict(geo=geo, phi=phi, hext=uniform_field(geo, (0.0, 1.0, 0.0)),
                 chi=chi, band=band_cells * geo.spacing)

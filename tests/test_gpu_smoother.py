@@ -59,14 +59,20 @@ def _system(kind, n, seed):
     return ctx, 1
 
 
-@pytest.mark.parametrize("kind,n,level", [
-    ("pressure", (72, 66, 130), 0),   # one plane per block, 3 x 5 tiles, air cells
-    ("pressure", (128, 128, 128), 0),  # several planes per block
-    ("magnetic", 64, 0),               # every cell active
-    ("pressure", (96, 80, 112), 1),    # coarse level narrower than a tile in z
+@pytest.mark.parametrize("kind,n,level,pre", [
+    ("pressure", (72, 66, 130), 0, False),   # one plane per block, 3 x 5 tiles, air cells
+    ("pressure", (128, 128, 128), 0, False),  # several planes per block
+    ("magnetic", 64, 0, False),               # every cell active
+    ("pressure", (96, 80, 112), 1, False),    # coarse level narrower than a tile in z
+    # z rows of 193..256 cells: the row-tiled kernel (whole rows per wavefront)
+    ("pressure", (40, 50, 256), 0, False),    # one plane per block, air rows skipped
+    ("pressure", (40, 50, 256), 0, True),     # the first pre-smoothing passes (d1 = x1)
+    ("pressure", (300, 61, 200), 0, False),   # runs of several planes, lanes past the row's end
+    ("magnetic", (70, 33, 256), 0, False),    # row-sum diagonal rebuilt, every row active
+    ("magnetic", (70, 33, 256), 0, True),
 ])
-def test_pair_matches_two_steps(kind, n, level):
-    print(f"[case] smooth pair {kind} {n} level {level}", flush=True)
+def test_pair_matches_two_steps(kind, n, level, pre):
+    print(f"[case] smooth pair {kind} {n} level {level} pre {pre}", flush=True)
     ctx, which = _system(kind, n, 3)
     with ctx:
         op = ctx.export_level32(which, level)
@@ -78,11 +84,13 @@ def test_pair_matches_two_steps(kind, n, level):
         rng = np.random.default_rng(11)
         vec = lambda s: (rng.uniform(-s, s, diag.shape) * act).astype(np.float32)  # noqa: E731
         b, x1, d1 = vec(1.0), vec(1.0), vec(0.1)
-        x3p, x3s = ctx.smooth_pair(which, level, b, x1, d1, *COEF)
+        if pre:
+            d1 = x1  # the stored first iterate is also the previous direction
+        x3p, x3s = ctx.smooth_pair(which, level, b, x1, None if pre else d1, *COEF)
         # repeated launches: the same bits each time (no race between the
         # block's wavefronts over the shared planes)
         for _ in range(4):
-            again, _ = ctx.smooth_pair(which, level, b, x1, d1, *COEF)
+            again, _ = ctx.smooth_pair(which, level, b, x1, None if pre else d1, *COEF)
             np.testing.assert_array_equal(again, x3p)
     np.testing.assert_array_equal(x3p, x3s)
     assert np.all(x3p[~act] == 0)

@@ -48,9 +48,10 @@ def test_struct_layouts_match_c():
 #include "sfp.h"
 #include "sfp_profile.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(sfp_params), offsetof(sfp_params, max_iters),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(sfp_params), offsetof(sfp_params, max_iters),
          offsetof(sfp_params, cheby_ratio), offsetof(sfp_params, stall_ratio), offsetof(sfp_params, verbose),
-         offsetof(sfp_params, coarse_cheby_ratio),
+         offsetof(sfp_params, coarse_cheby_ratio), offsetof(sfp_params, net_cheby_ratio),
+         offsetof(sfp_params, net_coarse_cheby_ratio),
          sizeof(sfp_stats), offsetof(sfp_stats, rel_residual), offsetof(sfp_stats, unknowns),
          sizeof(sfp_kernel_profile), offsetof(sfp_kernel_profile, bytes));
   return 0;
@@ -65,7 +66,8 @@ int main(void) {
                                                check=True).stdout.split()]
     P, S, K = sfp.Params, sfp.Stats, sfp.KernelProfile
     assert vals == [C.sizeof(P), P.max_iters.offset, P.cheby_ratio.offset, P.stall_ratio.offset,
-                    P.verbose.offset, P.coarse_cheby_ratio.offset, C.sizeof(S), S.rel_residual.offset, S.unknowns.offset,
+                    P.verbose.offset, P.coarse_cheby_ratio.offset, P.net_cheby_ratio.offset,
+                    P.net_coarse_cheby_ratio.offset, C.sizeof(S), S.rel_residual.offset, S.unknowns.offset,
                     C.sizeof(K), K.bytes.offset]
 
 
@@ -138,7 +140,13 @@ def test_trained_weights_load_and_checksum():
     assert np.all(np.isfinite(params))
     assert meta["n"] == 128 and meta["samples"] > 0
     sm = learn.smoother(meta)
-    assert set(sm) == set(learn.SMOOTHER_KEYS) and all(v > 0 for v in sm.values())
+    assert set(sm) == {"net_" + k for k in learn.SMOOTHER_KEYS} and all(v > 0 for v in sm.values())
+    lp = learn.learned_params(meta)
+    assert lp["precond"] == sfp.PRECOND_LEARNED and 0 < lp["stall_ratio"] <= 1
+    # the learned cycle's smoother is a separate set of parameters: the plain
+    # cycle keeps the library's defaults for the fall-back
+    p = sfp.default_params()
+    assert p.net_cheby_degree == 0 and p.net_coarse_cheby_ratio == 0.0
 
 
 def test_apply_full_matches_csr():
