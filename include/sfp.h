@@ -222,6 +222,23 @@ sfp_status sfp_collider_enforce_device(sfp_context ctx, double *const velocity[3
                                        const double *const normal[3],
                                        const double *const collider_velocity[3]);
 
+/* ---- without allocation ------------------------------------------------ */
+/* sfp_extrapolate_velocity_device without a context and without allocating
+ * anything: no counterpart in the reference. For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight loops that forbid it).
+ * velocity, level_set and fraction are device arrays of an nx*ny*nz grid;
+ * the work is enqueued on `stream` (a hipStream_t; null = the default
+ * stream) and the call returns without waiting. Its mask and sweep buffer
+ * (two bytes per face of the largest face grid) live in one fixed device
+ * array of 64 MiB that each GPU holds until the process ends: grids needing
+ * more are rejected with SFP_ERR_ARGUMENT. Calls on one GPU share that array,
+ * so calls that overlap in time on one GPU (other streams, other threads)
+ * race; calls on one stream are ordered. Errors: sfp_last_error(NULL). */
+sfp_status sfp_extrapolate_velocity_noalloc(int nx, int ny, int nz, double *const velocity[3],
+                                            const double *level_set,
+                                            const double *const fraction[3], int steps,
+                                            void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,9 @@ size_t face_count(Dims d, int a) { return d.face(a).cells(); }
 
 }  // namespace
 
+// Error text of the calls without a context (sfp_last_error(NULL)).
+void sfp::set_unbound_error(const char *msg) { g_create_error = msg; }
+
 namespace {
 
 sfp_params to_params(const SolverConfig &c) {

@@ -5,9 +5,11 @@
 // Built with -ffp-contract=off and written in the reference's evaluation
 // order (neighbour ordinal 0..5 = -x,+x,-y,+y,-z,+z; TriLerp corner order and
 // weight products as core/TriLerp.h:27-36; dot products summed x, y, z).
+#include "../../include/sfp.h"
 #include "device_util.hpp"
 #include "kernels.hpp"
 
+#include <mutex>
 
 namespace sfp {
 namespace {
@@ -183,3 +185,80 @@ void launch_collider_enforce(hipStream_t s, Dims d, double spacing, const double
 }
 
 }  // namespace sfp
+
+// ---------------------------------------------------------------- without allocation
+// sfp_extrapolate_velocity_noalloc: the mask and its sweep buffer live in one
+// file-scope device array. Its host-side name is not a device address: each
+// device's copy is found with hipGetSymbolAddress (with that device current)
+// on the first call there, and only that address is passed to kernels.
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;  // two masks of up to 32 Mi faces
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// This device's copy of `scratch` (looked up once per device), or null.
+unsigned char *device_scratch(int dev, hipError_t *err) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (*err != hipSuccess) return nullptr;
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" sfp_status sfp_extrapolate_velocity_noalloc(int nx, int ny, int nz, double *const velocity[3],
+                                                        const double *level_set,
+                                                        const double *const fraction[3], int steps,
+                                                        void *stream) {
+    using namespace sfp;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || steps < 0) {
+        set_unbound_error("grid extents must be positive and steps >= 0");
+        return SFP_ERR_ARGUMENT;
+    }
+    const Dims d{nx, ny, nz};
+    for (int a = 0; a < 3; a++) {
+        const Dims f = d.face(a);
+        if (2.0 * (double)f.nx * (double)f.ny * (double)f.nz > (double)kNoAllocScratchBytes) {
+            set_unbound_error("grid too large for the fixed scratch of the no-allocation entry point "
+                              "(two bytes per face of each face grid, 64 MiB)");
+            return SFP_ERR_ARGUMENT;
+        }
+    }
+    if (!velocity || !fraction || !level_set || !velocity[0] || !velocity[1] || !velocity[2] ||
+        !fraction[0] || !fraction[1] || !fraction[2]) {
+        set_unbound_error("velocity, level_set and fraction must be non-null device arrays");
+        return SFP_ERR_ARGUMENT;
+    }
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_unbound_error("no current HIP device");
+        return SFP_ERR_NO_DEVICE;
+    }
+    unsigned char *base = device_scratch(dev, &e);
+    if (!base) {
+        set_unbound_error(hipGetErrorString(e));
+        return SFP_ERR_HIP;
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    try {
+        for (int a = 0; a < 3; a++) {
+            const Dims fd = d.face(a);
+            unsigned char *mask = base, *sweep = base + fd.cells();
+            launch_velocity_valid(s, d, a, level_set, fraction[a], mask);
+            launch_extrapolate(s, fd, velocity[a], mask, sweep, 0.0, steps);
+        }
+    } catch (const HipError &err) {
+        set_unbound_error(err.what());
+        return SFP_ERR_HIP;
+    }
+    return SFP_OK;
+}

@@ -25,6 +25,8 @@ struct StateError : std::runtime_error {
 };
 
 void hip_fail(hipError_t e, const char *what, const char *file, int line);
+// Sets the text sfp_last_error(NULL) returns (entry points without a context).
+void set_unbound_error(const char *msg);
 #define SFP_HIP(call)                                                    \
     do {                                                                 \
         hipError_t _e = (call);                                          \

@@ -36,7 +36,7 @@ EXPORTS = {
               "sfp_pressure_project_device", "sfp_pressure_reproject_device",
               "sfp_magnetic_solve_device", "sfp_extrapolate_faces", "sfp_extrapolate_velocity",
               "sfp_collider_enforce", "sfp_extrapolate_velocity_device",
-              "sfp_collider_enforce_device",
+              "sfp_collider_enforce_device", "sfp_extrapolate_velocity_noalloc",
               "sfp_projection_substep"],
     "sfp_profile.h": ["sfp_profile_enable", "sfp_profile_query"],
     "sfp_test.h": ["sfp_test_num_levels", "sfp_test_level_dims", "sfp_test_export_level",
@@ -129,6 +129,8 @@ def lib():
         L.sfp_projection_substep.restype = C.c_int
         L.sfp_projection_substep.argtypes = [vp, PP, _dp, PP, PP, vp, C.c_double, C.c_int, PP, PP, _dp,
                                              C.POINTER(Stats)]
+        L.sfp_extrapolate_velocity_noalloc.restype = C.c_int
+        L.sfp_extrapolate_velocity_noalloc.argtypes = [C.c_int] * 3 + [PP, _dp, PP, C.c_int, vp]
         for name in ("sfp_collider_enforce", "sfp_collider_enforce_device"):
             getattr(L, name).restype = C.c_int
             getattr(L, name).argtypes = [vp, PP, PP, PP, PP]
@@ -220,6 +222,17 @@ def _p3(arrs):
     if arrs is None:
         return None
     return P3(*[_d(a) for a in arrs])
+
+
+def extrapolate_velocity_noalloc(n, vel, phi, fraction, steps=6, stream=None):
+    """sfp_extrapolate_velocity_noalloc on device arrays (int pointers) of an
+    n = (nx, ny, nz) grid, enqueued on `stream` (a hipStream_t as an int;
+    None: the default stream); returns without waiting."""
+    st = lib().sfp_extrapolate_velocity_noalloc(int(n[0]), int(n[1]), int(n[2]), _p3(vel), _d(phi),
+                                                _p3(fraction), int(steps),
+                                                None if stream is None else C.c_void_p(int(stream)))
+    if st != OK:
+        raise SfpError(st, lib().sfp_last_error(None).decode())
 
 
 def default_params() -> Params:

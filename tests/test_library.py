@@ -205,3 +205,21 @@ def test_wrapper_rejects_arrays_that_do_not_fit_the_grid():
     # arrays that fit get past the checks and reach the (unbound) library
     with pytest.raises(sfp.SfpError):
         ctx.pressure_project(good)
+
+
+def test_noalloc_extrapolation_rejects_before_touching_a_device():
+    """The entry point without allocation checks its arguments, including
+    that the grid fits its fixed 64 MiB scratch, before any HIP call."""
+    L = sfp.lib()
+    dummy = (C.c_double * 1)()
+    p3 = sfp.P3(*[C.cast(dummy, sfp._dp)] * 3)
+    d = C.cast(dummy, sfp._dp)
+    f = L.sfp_extrapolate_velocity_noalloc
+    # 330^3: each face grid ~36 M faces, two bytes per face do not fit
+    assert f(330, 330, 330, p3, d, p3, 6, None) == sfp.ERR_ARGUMENT
+    assert b"64 MiB" in L.sfp_last_error(None)
+    assert f(0, 4, 4, p3, d, p3, 6, None) == sfp.ERR_ARGUMENT
+    assert f(4, 4, 4, p3, d, p3, -1, None) == sfp.ERR_ARGUMENT
+    assert f(4, 4, 4, None, d, p3, 6, None) == sfp.ERR_ARGUMENT
+    assert f(4, 4, 4, p3, None, p3, 6, None) == sfp.ERR_ARGUMENT
+    assert f(2 ** 30, 2 ** 30, 2 ** 30, p3, d, p3, 6, None) == sfp.ERR_ARGUMENT
