@@ -75,12 +75,12 @@ def parse():
 # rocprofv3 symbol, algorithmic bytes per cell (each distinct array once per
 # launch, as the library counts them) and what the kernel does.
 HOT_KERNELS = {
-    "smooth_pair": ("k_smooth_pair<false, true>", 26.0,
-                    "fine-level pairs of Chebyshev steps in one pass, FP32 7-point stencil, the "
-                    "magnetic diagonal rebuilt from the couplings: the first two pre-smoothing "
-                    "steps (reads the stored first iterate, b, 3 couplings, writes x: 24 B/cell) "
-                    "and the last two post-smoothing steps (reads x, b, d, 3 couplings, writes x: "
-                    "28 B/cell)"),
+    "smooth_pair": ("k_smooth_pair_rows<true", 26.0,
+                    "fine-level pairs of Chebyshev steps in one pass over whole z rows, FP32 "
+                    "7-point stencil, the magnetic diagonal rebuilt from the couplings: the first "
+                    "two pre-smoothing steps (reads the stored first iterate, b, 3 couplings, "
+                    "writes x: 24 B/cell) and the last two post-smoothing steps (reads x, b, d, 3 "
+                    "couplings, writes x: 28 B/cell)"),
     "smooth_restrict": ("k_smooth_pair<true, true>", 28.5,
                         "fine-level last pre-smoothing step and the residual restriction in one "
                         "pass, FP32: reads x, b, d, diag, 3 couplings, writes x and the coarse b"),
@@ -135,9 +135,11 @@ def summarize_profiles(d, n=256):
     alg = bpc * cells
     avg = statistics.mean(fine)
     lines = [f"# {sym} at {n}^3: rocprofv3 kernel trace", "", f"{what}.", "",
-             "Command (GPU box, repo root):", "",
+             "Commands (GPU box, repo root):", "",
+             "    python3 bench.py > gpurun_out/bench_full.json",
              "    rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_trace -o trace -- \\",
-             "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 4 --warmup 0 --systems m2,m8", "",
+             "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 4 --warmup 0 --systems m2,m8",
+             "    python3 bench.py --summarize-profiles gpurun_out", "",
              f"(magnetic systems: every cell active, so every launch moves the full {bpc:g} B per cell)", "",
              f"Fine-level dispatches of `{sym}` (grid {grid} work-items): {len(fine)} "
              f"({gated} more returned at once: iterations enqueued past convergence, left out)",
@@ -183,6 +185,7 @@ def summarize_profiles(d, n=256):
     json.dump(res, open(os.path.join(out_dir, f"traffic_{name}_{n}.json"), "w"), indent=1)
     md = [f"# {sym} at {n}^3: HBM traffic from FETCH_SIZE / WRITE_SIZE", "",
           "Commands (GPU box, repo root), one counter per pass:", "",
+          "    python3 bench.py > gpurun_out/bench_full.json",
           "    rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/prof_fetch -o fetch -- \\",
           "        python3 bench.py --no-cpu-baseline --no-profile --no-learned --steps 2 --warmup 0 --systems m2,m8",
           "    rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/prof_write -o write -- \\",
