@@ -548,7 +548,7 @@ def unet_roofline(params, n, log):
     total = sum(flops) / (fwd_ms * 1e-3) / 1e12
     log(f"network {n}^3: {fwd_ms:.3f} ms per pass ({total:.0f} TFLOP/s over its convolutions' FLOPs); "
         f"dominant conv {k} ({cin[k]}->{cout[k]}): {conv_ms[k] * 1e3:.1f} us, {rate:.0f} TFLOP/s")
-    return {"kernel": f"k_conv3 layer {k} ({cin[k]}->{cout[k]} channels, 3x3x3, BF16 MFMA, FP32 "
+    return {"kernel": f"k_conv3r layer {k} ({cin[k]}->{cout[k]} channels, 3x3x3, BF16 MFMA, FP32 "
                       f"accumulation) at {n >> level[k]}^3", "bound": "mfma",
             "achieved": round(rate, 1), "peak": 2500.0, "unit": "TFLOP/s",
             "frac": round(rate / 2500.0, 4), "flops_per_launch": flops[k],

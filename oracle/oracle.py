@@ -24,7 +24,9 @@ _ip = C.POINTER(C.c_int)
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # SFP_ORACLE_LIB: another build of the same sources (the sanitizer
+        # build, oracle/_san/liboracle.so)
+        path = os.environ.get("SFP_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
         L = C.CDLL(path)

@@ -4,13 +4,11 @@
 // uint16), activation gradients FP32, master weights FP32 with BF16 copies
 // [tap][cout][cin] for the convolutions.
 //
-// k_conv3 is an implicit GEMM: M = output voxels of a (TX x 4 x 16) tile,
-// N = output channels, K = 27 taps x input channels, on
-// v_mfma_f32_16x16x32_bf16 with FP32 accumulation. The input tile with its
-// one-voxel halo is staged in LDS; each of the 4 wavefronts owns TX row blocks
-// of 16 voxels and all N. Lane layouts (checked on the device): A[m][k] from
-// lane m%16 + 16*(k/8), B[k][n] from lane n + 16*(k/8), D[m][n] in lane
-// n + 16*(m/4), element m%4.
+// k_conv3r is an implicit GEMM: M = output voxels (z rows of 16), N = output
+// channels (blocks of 16), K = 27 taps x input channels, on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation. Lane layouts (checked on
+// the device): A[m][k] from lane m%16 + 16*(k/8), B[k][n] from lane
+// n + 16*(k/8), D[m][n] in lane n + 16*(m/4), element m%4.
 #include "device_util.hpp"
 #include "unet.hpp"
 
@@ -42,120 +40,215 @@ struct ConvArgs {
     const double *gate;  // nonzero *gate: return at once (a stopped batch of FCG iterations)
 };
 
+// The 3x3x3 convolution (zero padding), optionally reading its input
+// nearest-up-sampled, with bias, ReLU, a skip tensor added after the ReLU and
+// BF16 or FP32 output. Each input row's fragments are reused in registers by
+// every output row it is a tap of. A wavefront owns one 16-channel block of
+// outputs (nb) on PY z rows (y0 .. y0 + PY - 1, 16 voxels of z each) and
+// marches along x over a run of output planes [xa, xe). Its weight fragments
+// (9 x KC, all 27 taps of its output channels) stay in VGPRs for the march.
+// At each step it loads the fragments of input plane gx (PY + 2 rows, each
+// row shifted by dz = -1, 0, 1 in z, times the channel chunks: KC per row)
+// straight from global memory (neighbouring wavefronts' rows meet in L1/L2),
+// one plane ahead of their use, and each fragment feeds the MFMAs of up to 9
+// output rows: planes gx + 1, gx, gx - 1 (dx = 0, 1, 2) times the rows it
+// neighbours in y. Accumulators of three output planes rotate; plane gx - 1
+// is complete after the step and goes through the epilogue.
 template <int CIN>
-__device__ __forceinline__ void chunk_slice(int chunk, int g, int &tap, int &c0) {
-    if constexpr (CIN == 8) {
-        tap = chunk * 4 + g;
-        c0 = 0;
-    } else if constexpr (CIN == 16) {
-        tap = chunk * 2 + (g >> 1);
-        c0 = 8 * (g & 1);
+constexpr int kchunks() {
+    return CIN == 8 ? 1 : CIN == 16 ? 2 : CIN == 32 ? 3 : 6;
+}
+// K chunk kc of an input row, lane group g (8 of the MFMA's 32 K values): the
+// z shift dz (0..2), the first channel c, and whether the slice exists.
+template <int CIN>
+__device__ __forceinline__ void kslice(int kc, int g, int &dz, int &c, bool &ok) {
+    if constexpr (CIN == 8) {  // dz = g, all 8 channels; group 3 empty
+        dz = g;
+        c = 0;
+        ok = g < 3;
+    } else if constexpr (CIN == 16) {  // two z shifts per chunk, 8 channels per group
+        dz = 2 * kc + (g >> 1);
+        c = 8 * (g & 1);
+        ok = dz < 3;
     } else if constexpr (CIN == 32) {
-        tap = chunk;
-        c0 = 8 * g;
+        dz = kc;
+        c = 8 * g;
+        ok = true;
     } else {
-        tap = chunk >> 1;
-        c0 = 32 * (chunk & 1) + 8 * g;
+        dz = kc % 3;
+        c = 32 * (kc / 3) + 8 * g;
+        ok = true;
     }
 }
 
-template <int CIN>
-constexpr int num_chunks() {
-    return CIN == 8 ? 7 : CIN == 16 ? 14 : CIN == 32 ? 27 : 54;
-}
+// 16 zero bytes: lanes whose operand lies outside the volume (or outside
+// the K chunk) load from here, so no loaded register is overwritten later.
+__device__ __attribute__((aligned(16))) uint16_t g_zero8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
-template <int CIN, int COUT, int TX>
-__global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
-    constexpr int HX = TX + 2, HY = 6, HZ = 18;
+template <int CIN, int COUT, int PY, int WPS>
+__global__ __launch_bounds__(256, WPS) void k_conv3r(ConvArgs a, int run, int xruns) {
+    constexpr int KC = kchunks<CIN>();
     constexpr int NB = COUT / 16;
-    constexpr int RB = TX;  // 16-row blocks per wavefront
-    __shared__ __attribute__((aligned(16))) uint16_t tile[HX * HY * HZ * CIN];
     if (a.gate && *a.gate != 0.0) return;
-    const int tid = threadIdx.x;
-    const int tx_n = (a.X + TX - 1) / TX, ty_n = (a.Y + 3) / 4, tz_n = (a.Z + 15) / 16;
-    unsigned t = blockIdx.x;
-    const int bz = t % tz_n;
-    t /= tz_n;
-    const int by = t % ty_n;
-    t /= ty_n;
-    const int bx = t % tx_n;
-    const int bb = t / tx_n;
-    const int x0 = bx * TX, y0 = by * 4, z0 = bz * 16;
-    // stage the halo tile
-    constexpr int PARTS = CIN / 8;
-    constexpr int TOTAL = HX * HY * HZ * PARTS;
-    for (int idx = tid; idx < TOTAL; idx += 256) {
-        const int vox = idx / PARTS, part = idx - vox * PARTS;
-        const int hz = vox % HZ, hy = (vox / HZ) % HY, hx = vox / (HZ * HY);
-        const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.X && gy < a.Y && gz < a.Z) {
-            const size_t gi =
-                a.up ? ((((size_t)bb * (a.X >> 1) + (gx >> 1)) * (a.Y >> 1) + (gy >> 1)) * (a.Z >> 1) + (gz >> 1)) *
-                               CIN + part * 8
-                     : ((((size_t)bb * a.X + gx) * a.Y + gy) * a.Z + gz) * CIN + part * 8;
-            v = *reinterpret_cast<const uint4 *>(a.in + gi);
-        }
-        *reinterpret_cast<uint4 *>(tile + vox * CIN + part * 8) = v;
+    const uint16_t *__restrict__ in = a.in;
+    const uint16_t *__restrict__ wts = a.w;
+    const int X = a.X, Y = a.Y, Z = a.Z;
+    const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
+    const int ypn = (Y + PY - 1) / PY, zbn = (Z + 15) / 16;
+    unsigned w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nb = (int)(w % NB);
+    w /= NB;
+    const int yp = (int)(w % (unsigned)ypn);
+    w /= (unsigned)ypn;
+    const int zb = (int)(w % (unsigned)zbn);
+    w /= (unsigned)zbn;
+    const int xr = (int)(w % (unsigned)xruns);
+    const int b = (int)(w / (unsigned)xruns);
+    if (b >= a.B) return;  // uniform over the wavefront
+    const int y0 = yp * PY, z0 = zb * 16, xa = xr * run;
+    if (xa >= X) return;
+    const int xe = min(xa + run, X);
+    // weights [dx][dy][kc]: B[k][n] of lane n + 16 (k / 8)
+    v8bf wf[3][3][KC];
+#pragma unroll
+    for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++) {
+                int dz, c;
+                bool ok;
+                kslice<CIN>(kc, g, dz, c, ok);
+                const uint16_t *p =
+                    ok ? wts + ((size_t)((dx * 9 + dy * 3 + dz) * COUT + nb * 16 + r16) * CIN + c) : g_zero8;
+                wf[dx][dy][kc] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4 *>(p));
+            }
+    // this lane's input voxel (z) and channel offset per chunk: A[m][k] of
+    // lane m + 16 (k / 8), m = z - z0 of the output row; the element offset
+    // within an (x, y) row of the (possibly half-resolution) input
+    const int up = a.up;
+    const int Xs = up ? X >> 1 : X, Ys = up ? Y >> 1 : Y, Zs = up ? Z >> 1 : Z;
+    int zoff[KC];
+    bool zok[KC];
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) {
+        int dz, c;
+        bool ok;
+        kslice<CIN>(kc, g, dz, c, ok);
+        const int zi = z0 + r16 + dz - 1;
+        zok[kc] = ok && zi >= 0 && zi < Z;
+        const int cz = min(max(zi, 0), Z - 1);  // a valid address either way
+        zoff[kc] = (up ? cz >> 1 : cz) * CIN + c;
     }
-    __syncthreads();
-    const int lane = tid & 63, wave = tid >> 6;
-    const int g = lane >> 4, r16 = lane & 15;
-    v4f acc[RB][NB];
+    v4f acc[3][PY];
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++)
+    for (int s = 0; s < 3; s++)
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++) acc[rb][nb] = v4f{0.f, 0.f, 0.f, 0.f};
-    for (int chunk = 0; chunk < num_chunks<CIN>(); chunk++) {
-        int tap, c0;
-        chunk_slice<CIN>(chunk, g, tap, c0);
-        const bool valid = tap < 27;
-        const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;
-        v8bf bf[NB];
+        for (int oy = 0; oy < PY; oy++) acc[s][oy] = v4f{0.f, 0.f, 0.f, 0.f};
+    const int steps = xe - xa + 2;  // input planes xa - 1 .. xe
+    // The fragments of input plane gx: rows y0 - 1 .. y0 + PY, zero outside
+    // the volume. Branch-free (the address is selected, never the loaded
+    // value), so the next plane's loads stay in flight during this plane's
+    // MFMAs.
+    auto load_plane = [&](v8bf(&f)[PY + 2][KC], int gx) {
+        const bool xok = gx >= 0 && gx < X;
+        const int cx = min(max(gx, 0), X - 1);
+        const size_t xrow = ((size_t)b * Xs + (up ? cx >> 1 : cx)) * Ys;
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++) {
-            uint4 u = make_uint4(0, 0, 0, 0);
-            if (valid)
-                u = *reinterpret_cast<const uint4 *>(a.w + ((size_t)(tap * COUT + nb * 16 + r16) * CIN + c0));
-            bf[nb] = __builtin_bit_cast(v8bf, u);
-        }
+        for (int iy = 0; iy < PY + 2; iy++) {
+            const int gy = y0 + iy - 1;
+            const bool ok = xok && gy >= 0 && gy < Y;
+            const int cy = min(max(gy, 0), Y - 1);
+            const uint16_t *row = in + (xrow + (up ? cy >> 1 : cy)) * Zs * CIN;
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
-            const int row = wave * (16 * RB) + rb * 16 + r16;
-            const int tz = row & 15, ty = (row >> 4) & 3, tx = row >> 6;
-            uint4 u = make_uint4(0, 0, 0, 0);
-            if (valid)
-                u = *reinterpret_cast<const uint4 *>(tile + (((tx + dx) * HY + (ty + dy)) * HZ + (tz + dz)) * CIN + c0);
-            const v8bf af = __builtin_bit_cast(v8bf, u);
-#pragma unroll
-            for (int nb = 0; nb < NB; nb++)
-                acc[rb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf[nb], acc[rb][nb], 0, 0, 0);
-        }
-    }
-    // epilogue
-#pragma unroll
-    for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int row = wave * (16 * RB) + rb * 16 + 4 * g + j;
-            const int tz = row & 15, ty = (row >> 4) & 3, tx = row >> 6;
-            const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
-            if (gx >= a.X || gy >= a.Y || gz >= a.Z) continue;
-            const size_t vox = (((size_t)bb * a.X + gx) * a.Y + gy) * a.Z + gz;
-#pragma unroll
-            for (int nb = 0; nb < NB; nb++) {
-                const int ch = nb * 16 + r16;
-                float v = acc[rb][nb][j];
-                if (a.bias) v += a.bias[ch];
-                if (a.relu) v = v > 0.f ? v : 0.f;
-                if (a.relu_copy) a.relu_copy[vox * COUT + ch] = f2bf(v);
-                if (a.skip) v += bf2f(a.skip[vox * COUT + ch]);
-                if (a.out32)
-                    a.out32[vox * COUT + ch] = v;
-                else
-                    a.out[vox * COUT + ch] = f2bf(v);
+            for (int kc = 0; kc < KC; kc++) {
+                const uint16_t *p = ok && zok[kc] ? row + zoff[kc] : g_zero8;
+                f[iy][kc] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4 *>(p));
             }
         }
+    };
+    v8bf cur[PY + 2][KC], nxt[PY + 2][KC];
+    load_plane(cur, xa - 1);
+    const int ch = nb * 16 + r16;
+    const float bias = a.bias ? a.bias[ch] : 0.f;
+    const uint16_t *__restrict__ skip = a.skip;
+    // step t reads input plane gx = xa - 1 + t; with U = t mod 3, output
+    // plane gx + 1 - dx accumulates in slot (U - dx) mod 3
+    for (int t0 = 0; t0 < steps; t0 += 3) {
+#pragma unroll
+        for (int U = 0; U < 3; U++) {
+            const int t = t0 + U;
+            if (t >= steps) break;  // uniform
+            const int gx = xa - 1 + t;
+            // the skip values of the plane this step completes, loaded before
+            // the next plane's fragments so that waiting for them leaves those
+            // in flight
+            uint16_t skv[PY][4];
+            if (skip && t >= 2) {
+#pragma unroll
+                for (int oy = 0; oy < PY; oy++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int gy = y0 + oy, gz = z0 + 4 * g + j;
+                        const size_t vox = (((size_t)b * X + (gx - 1)) * Y + gy) * Z + gz;
+                        skv[oy][j] = *(gy < Y && gz < Z ? skip + vox * COUT + ch : g_zero8);
+                    }
+            }
+            load_plane(nxt, gx + 1);  // (one plane past the run at the end: unused)
+#pragma unroll
+            for (int iy = 0; iy < PY + 2; iy++)
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+                    for (int dy = 0; dy < 3; dy++) {
+                        const int oy = iy - dy;
+                        if (oy < 0 || oy >= PY) continue;  // known after unrolling
+#pragma unroll
+                        for (int kc = 0; kc < KC; kc++)
+                            acc[(U - dx + 3) % 3][oy] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                cur[iy][kc], wf[dx][dy][kc], acc[(U - dx + 3) % 3][oy], 0, 0, 0);
+                    }
+            // planes before the run (gx + 1 - dx < xa) took contributions in
+            // the first two steps: clear their slots
+            if (t == 0) {
+#pragma unroll
+                for (int oy = 0; oy < PY; oy++) acc[1][oy] = acc[2][oy] = v4f{0.f, 0.f, 0.f, 0.f};
+            } else if (t == 1) {
+#pragma unroll
+                for (int oy = 0; oy < PY; oy++) acc[2][oy] = v4f{0.f, 0.f, 0.f, 0.f};
+            }
+            // output plane gx - 1 has had its three input planes
+            if (t >= 2) {
+                const int S = (U + 1) % 3;
+                const int ox = gx - 1;
+#pragma unroll
+                for (int oy = 0; oy < PY; oy++) {
+                    const int gy = y0 + oy;
+                    if (gy < Y) {
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            const int gz = z0 + 4 * g + j;
+                            if (gz >= Z) continue;
+                            const size_t vox = (((size_t)b * X + ox) * Y + gy) * Z + gz;
+                            float v = acc[S][oy][j] + bias;
+                            if (a.relu) v = v > 0.f ? v : 0.f;
+                            if (a.relu_copy) a.relu_copy[vox * COUT + ch] = f2bf(v);
+                            if (skip) v += bf2f(skv[oy][j]);
+                            if (a.out32)
+                                a.out32[vox * COUT + ch] = v;
+                            else
+                                a.out[vox * COUT + ch] = f2bf(v);
+                        }
+                    }
+                    acc[S][oy] = v4f{0.f, 0.f, 0.f, 0.f};
+                }
+            }
+#pragma unroll
+            for (int iy = 0; iy < PY + 2; iy++)
+#pragma unroll
+                for (int kc = 0; kc < KC; kc++) cur[iy][kc] = nxt[iy][kc];
+        }
+    }
 }
 
 // Weight gradient partials as an implicit GEMM on MFMA: for tap blockIdx.y
@@ -496,9 +589,22 @@ unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
 
 template <int CIN, int COUT>
 void conv_dispatch(hipStream_t st, const ConvArgs &a) {
-    constexpr int TX = CIN >= 64 ? 2 : 4;
-    const unsigned tiles = (unsigned)((size_t)a.B * ((a.X + TX - 1) / TX) * ((a.Y + 3) / 4) * ((a.Z + 15) / 16));
-    hipLaunchKernelGGL((k_conv3<CIN, COUT, TX>), dim3(tiles), dim3(256), 0, st, a);
+    // rows per wavefront and the wavefronts per SIMD the launch bound asks
+    // for, fastest of PY = 1..4 measured on MI355X for each shape at its
+    // U-Net level (128^3, 64^3, 32^3); more rows reuse each loaded fragment
+    // more often but hold more VGPRs (the weight fragments alone take 36 to
+    // 216)
+    constexpr int PY = CIN == 8 ? 4 : CIN == 16 ? 2 : 1;
+    constexpr int WPS = CIN <= 16 ? 2 : 1;
+    // x runs: about two wavefronts per SIMD of the 256 CUs
+    const long long base = (long long)a.B * ((a.Z + 15) / 16) * ((a.Y + PY - 1) / PY) * (COUT / 16);
+    const long long target = 2048;
+    int xruns = (int)std::min<long long>(a.X, std::max<long long>(1, (target + base - 1) / base));
+    const int run = (a.X + xruns - 1) / xruns;
+    xruns = (a.X + run - 1) / run;
+    const long long waves = base * xruns;
+    hipLaunchKernelGGL((k_conv3r<CIN, COUT, PY, WPS>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, a,
+                       run, xruns);
     SFP_LAUNCH_CHECK();
 }
 
