@@ -15,10 +15,14 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_cpp_dropin_driver():
-    exe = os.path.join(ROOT, "tests", "cpp", "dropin_test")
-    print("[case] C++ drop-in driver", flush=True)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=100)
+@pytest.mark.parametrize("build", ["dropin_test", "dropin_test_ubsan"])
+def test_cpp_dropin_driver(build):
+    """The drop-in classes against the oracle; the second build has the host
+    code under UndefinedBehaviorSanitizer (any report aborts it)."""
+    exe = os.path.join(ROOT, "tests", "cpp", build)
+    print(f"[case] C++ drop-in driver ({build})", flush=True)
+    env = dict(os.environ, UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=100, env=env)
     print(r.stdout[-3000:])
     assert r.returncode == 0, r.stdout + r.stderr
     assert "PASSED" in r.stdout
