@@ -133,7 +133,8 @@ void launch_residual64(hipStream_t s, Dims d, Op64 A, const double *b, const dou
 // wavefronts, where z must be 0; null = every cell (any z).
 void launch_fcg_matvec(hipStream_t s, Dims d, Op64 A, const float *active, const float *z,
                        const double *r, const double *qprev, double *w, int first,
-                       double *partials, KrylovScalars *sc, const double *gate = nullptr);
+                       double *partials, KrylovScalars *sc, const double *gate = nullptr,
+                       hipEvent_t kernel_done = nullptr);
 // The update of x in an FCG iteration: now (x += alpha p), later (left for
 // the next iteration or launch_fcg_flush), or both the previous iteration's
 // and this one's (x += alpha_prev p_prev + alpha p), so x is read and written
@@ -146,7 +147,10 @@ enum XUpdate : int { X_NOW = 0, X_LATER = 1, X_BOTH = 2 };
 void launch_fcg_update(hipStream_t s, Dims d, const float *active, const float *z,
                        const double *w, double *p, double *q, double *x, double *r, float *b32,
                        float *x0, float c0, int first, int xmode, double *partials,
-                       KrylovScalars *sc, const double *gate = nullptr);
+                       KrylovScalars *sc, const double *gate = nullptr,
+                       hipEvent_t kernel_done = nullptr);
+// (kernel_done, if given, is recorded after the stencil kernel and before
+// the one-block finalize that forms the scalars.)
 // x += alpha p with the last iteration's alpha and direction (after an
 // X_LATER iteration), over the update's groups of cells.
 void launch_fcg_flush(hipStream_t s, Dims d, const float *active, const double *p, double *x,

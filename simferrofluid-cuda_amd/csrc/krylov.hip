@@ -566,7 +566,7 @@ void launch_residual64(hipStream_t st, Dims d, Op64 A, const double *b, const do
 
 void launch_fcg_matvec(hipStream_t st, Dims d, Op64 A, const float *active, const float *z,
                        const double *r, const double *qprev, double *w, int first,
-                       double *partials, KrylovScalars *sc, const double *gate) {
+                       double *partials, KrylovScalars *sc, const double *gate, hipEvent_t kernel_done) {
     if (matvec_flat(d)) {
         // pairs of cells: 16-byte (8-byte for z and the flags) aligned vectors
         for (const void *a : {(const void *)A.diag, (const void *)A.cpl[0], (const void *)A.cpl[1],
@@ -578,6 +578,7 @@ void launch_fcg_matvec(hipStream_t st, Dims d, Op64 A, const float *active, cons
         hipLaunchKernelGGL(k_fcg_matvec_flat, dim3(nb), dim3(kBlock), 0, st, d, A, active, z, r,
                            qprev, w, first, partials, gate);
         SFP_LAUNCH_CHECK();
+        if (kernel_done) SFP_HIP(hipEventRecord(kernel_done, st));
         finalize(st, FIN_MATVEC, nb, first, partials, sc, gate);
         return;
     }
@@ -585,13 +586,14 @@ void launch_fcg_matvec(hipStream_t st, Dims d, Op64 A, const float *active, cons
     hipLaunchKernelGGL(k_fcg_matvec, L.grid, L.block, 0, st, L.cg, d, A, active, z, r, qprev, w,
                        first, partials, gate);
     SFP_LAUNCH_CHECK();
+    if (kernel_done) SFP_HIP(hipEventRecord(kernel_done, st));
     finalize(st, FIN_MATVEC, L.blocks(), first, partials, sc, gate);
 }
 
 void launch_fcg_update(hipStream_t st, Dims d, const float *active, const float *z,
                        const double *w, double *p, double *q, double *x, double *r, float *b32,
                        float *x0, float c0, int first, int xmode, double *partials,
-                       KrylovScalars *sc, const double *gate) {
+                       KrylovScalars *sc, const double *gate, hipEvent_t kernel_done) {
     if (xmode == X_BOTH && first) throw ArgError("launch_fcg_update: X_BOTH needs a previous direction");
     // two cells per lane: 16-byte (8-byte for z) aligned vectors
     for (const void *a : {(const void *)w, (const void *)p, (const void *)q, (const void *)x, (const void *)r})
@@ -603,6 +605,7 @@ void launch_fcg_update(hipStream_t st, Dims d, const float *active, const float 
     hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(kBlock), 0, st, d.cells(), active, z, w, p, q,
                        x, r, b32, x0, c0, first, xmode, partials, sc, gate);
     SFP_LAUNCH_CHECK();
+    if (kernel_done) SFP_HIP(hipEventRecord(kernel_done, st));
     finalize(st, FIN_UPDATE, nb, first, partials, sc, gate);
 }
 

@@ -515,23 +515,29 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
             // out; and the diagonal and the FP32 one (skips inactive cells)
             // unless the operator has the row-sum structure, where every cell
             // but cell 0 is active and the diagonal is rebuilt
-            if (profiling)
+            // (the timings cover the stencil kernels, not the finalize after them)
+            hipEvent_t done = nullptr;
+            if (profiling) {
                 ws.timers[K_MATVEC64].begin(st, (4.0 + (A.pinned_rowsum ? 24.0 : 4.0 + 32.0) + 8.0 +
                                                  (first ? 0.0 : 8.0) + 8.0) * cells);
+                done = ws.timers[K_MATVEC64].pending_end();
+            }
             launch_fcg_matvec(st, d, A, A.pinned_rowsum ? nullptr : lv_[level].diag32.p, z, r_.p, q_.p,
-                              w_.p, first ? 1 : 0, ws.partials.p, ws.scalars.p, ws.gate);
-            if (profiling) ws.timers[K_MATVEC64].end(st);
+                              w_.p, first ? 1 : 0, ws.partials.p, ws.scalars.p, ws.gate, done);
+            if (profiling) ws.timers[K_MATVEC64].end_recorded();
             // z, w, p, q, r in; p, q, r out; x in and out every other
             // iteration (and diag in, b and x0 out)
             const int xmode = (run + it) % 2 == 0 ? X_LATER : X_BOTH;
-            if (profiling)
+            if (profiling) {
                 ws.timers[K_UPDATE64].begin(st, (4.0 + 8.0 * 4 + 8.0 * 3 + (xmode == X_LATER ? 0.0 : 16.0) +
                                                  (b32 ? 4.0 + 4.0 + 4.0 : 0.0)) * cells);
+                done = ws.timers[K_UPDATE64].pending_end();
+            }
             ws.x0_level = -1;
             launch_fcg_update(st, d, lv_[level].diag32.p, z, w_.p, p_.p, q_.p, x_.p, r_.p, b32,
                               b32 ? lv_[level].x0.p : nullptr, c0, first ? 1 : 0, xmode, ws.partials.p,
-                              ws.scalars.p, ws.gate);
-            if (profiling) ws.timers[K_UPDATE64].end(st);
+                              ws.scalars.p, ws.gate, done);
+            if (profiling) ws.timers[K_UPDATE64].end_recorded();
             first = false;
             b_ready = b32 != nullptr;
         }

@@ -67,6 +67,10 @@ struct KernelTimer {
     double total_ms = 0.0, total_bytes = 0.0;
     void begin(hipStream_t s, double b);
     void end(hipStream_t s);
+    // the end event of the open pair, for a launcher to record itself (after
+    // its kernel, before a follow-up launch); then end_recorded()
+    hipEvent_t pending_end() const { return ev[used + 1]; }
+    void end_recorded() { used += 2; }
     void harvest();  // after a stream sync: fold recorded pairs into the totals
     void reset();
     ~KernelTimer();

@@ -26,6 +26,9 @@ flips them for augmentation.
 
 Usage (from the package directory):
     python -m sfp.learn [--n 128 --steps 6000 --batch 8 --out weights/unet_precond.f32]
+    python -m sfp.learn --calibrate 256 --calibration-out calib.json
+The calibration measures the fall-back threshold (stall_ratio) the weights'
+metadata records; see calibrate().
 """
 from __future__ import annotations
 
@@ -248,6 +251,65 @@ def load_weights(path=WEIGHTS):
     return params.astype(np.float32), meta
 
 
+# ------------------------------------------------------------------ calibration
+def calibrate(params, meta, n=256, log=print):
+    """Measures, on this GPU and at n^3, what a learned iteration must achieve
+    to pay for itself, and returns (stall_ratio, record).
+
+    For the pressure projection and the magnetic systems (chi 0.5, 2, 8) of
+    the benchmark scenes: the plain MG-PCG solve's mean per-iteration
+    residual reduction f and time per iteration c_mg, the learned
+    preconditioner's time per iteration c_l (fall-back off) and its reduction
+    in each of its first three iterations. A learned iteration costs
+    rho = c_l / c_mg plain iterations, which reduce the residual by f^rho: the
+    break-even reduction. stall_ratio is the geometric mean of the systems'
+    break-evens, so the solver keeps the network only while an iteration
+    reduces ||r|| by more than rho plain iterations would."""
+    from . import scenes as sc_mod
+
+    systems = [("pressure", sc_mod.pressure_scene(n, 0))]
+    systems += [(f"magnetic_chi{c:g}", sc_mod.magnetic_scene(n, 0, c)) for c in (0.5, 2.0, 8.0)]
+    lp = learned_params(meta)
+
+    def solve(ctx, name, sc):
+        if name == "pressure":
+            return ctx.pressure_project(sc, want_pressure=False)[2]
+        ctx.reset_warm_start()
+        return ctx.magnetic_solve(sc, want_field=False)[2]
+
+    rec, evens = {}, []
+    for name, sc in systems:
+        r = {}
+        with Context.for_scene(sc, tol=1e-8, max_iters=200) as ctx:
+            solve(ctx, name, sc)
+            st = solve(ctx, name, sc)
+            r["mg_iterations"] = st["iterations"]
+            r["mg_factor"] = st["rel_residual"] ** (1.0 / st["iterations"])
+            r["mg_ms_per_it"] = st["solve_ms"] / st["iterations"]
+        with Context.for_scene(sc, tol=1e-8, max_iters=200) as ctx:
+            ctx.set_network(params)
+            ctx.set_params(**dict(lp, stall_ratio=100.0))  # never falls back
+            solve(ctx, name, sc)
+            st = solve(ctx, name, sc)
+            r["learned_iterations"] = st["iterations"]
+            r["learned_ms_per_it"] = st["solve_ms"] / st["iterations"]
+            rels = []
+            for k in (1, 2, 3):
+                ctx.set_params(max_iters=k)
+                rels.append(solve(ctx, name, sc)["rel_residual"])
+            r["learned_factors"] = [rels[0], rels[1] / rels[0], rels[2] / rels[1]]
+        r["cost_ratio"] = r["learned_ms_per_it"] / r["mg_ms_per_it"]
+        r["break_even"] = r["mg_factor"] ** r["cost_ratio"]
+        evens.append(r["break_even"])
+        log(f"[calibrate] {name}: MG {r['mg_iterations']} its, factor {r['mg_factor']:.3f}, "
+            f"{r['mg_ms_per_it']:.3f} ms/it; learned {r['learned_ms_per_it']:.3f} ms/it "
+            f"(x{r['cost_ratio']:.2f}), first factors "
+            + ", ".join(f"{v:.3f}" for v in r["learned_factors"]) + f"; break-even {r['break_even']:.2e}")
+        rec[name] = r
+    stall = float(np.exp(np.mean(np.log(evens))))
+    return stall, dict(n=n, systems=rec, stall_ratio=stall)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--n", type=int, default=128)
@@ -258,11 +320,25 @@ def main(argv=None):
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--cheby", default=None, metavar="DEG:RATIO:COARSE_DEG:COARSE_RATIO",
                     help="V-cycle smoother to train under (default: the library's)")
+    ap.add_argument("--calibrate", type=int, default=0, metavar="N",
+                    help="measure the fall-back threshold of the weights at --out on N^3 scenes "
+                         "instead of training; the record goes to --calibration-out")
+    ap.add_argument("--calibration-out", default=None,
+                    help="JSON file for the calibration record (default: print it)")
     a = ap.parse_args(argv)
     import torch  # its HIP runtime must initialise before libsfp's
     torch.cuda.init()
     t0 = time.time()
     log = lambda s: print(s, flush=True)  # noqa: E731
+    if a.calibrate:
+        params, meta = load_weights(a.out)
+        stall, rec = calibrate(params, meta, a.calibrate, log=log)
+        rec["sha256"] = meta["sha256"]
+        if a.calibration_out:
+            with open(a.calibration_out, "w") as f:
+                json.dump(rec, f, indent=1)
+        log(json.dumps(dict(stall_ratio=stall)))
+        return 0
     from . import default_params
     dp = default_params()
     sm = {k: getattr(dp, k) for k in SMOOTHER_KEYS}
