@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <initializer_list>
+#include <type_traits>
 
 namespace sfp {
 
@@ -555,6 +556,8 @@ __global__ __launch_bounds__(64 * kPairRows, 8) void k_smooth_pair(
 // vectors hold there.
 constexpr int kRowPairRows = 12;
 constexpr int kRowPairY = kRowPairRows - 2;  // interior rows: the tiles' stride in y
+// the out-of-tile rows take one value per thread: 3 parts of 256 z values
+static_assert(64 * kRowPairRows >= 3 * 256, "k_smooth_pair_rows: too few threads for the halo rows");
 
 template <bool kRowsum, bool kDx>
 __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
@@ -661,7 +664,7 @@ __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
     // one Chebyshev step of the four cells at plane q from iterate xc (x
     // neighbours xm, xp; y neighbours and couplings given), direction dp
     auto cheb = [&](f4 xc, f4 xm, f4 xp, f4 xym, f4 xyp, f4 cxc, f4 cxp, f4 cym, f4 cyp, f4 cz,
-                    f4 bv, f4 dg, f4 dp, float cd, float cr, f4 &dn_out) {
+                    f4 bv, f4 dg, f4 rdg, f4 dp, float cd, float cr, f4 &dn_out) {
         // every lane takes part in the shuffles: a lane outside the exec mask
         // does not supply its value to ds_bpermute (the reader gets 0)
         const float xzlo = __shfl_up(xc.w, 1, 64);
@@ -680,7 +683,7 @@ __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
             if (q > 0 || zl) off = fmaf(cz[q], xzm, off);
             if (q < 3 || zh) off = fmaf(czp, xzp, off);
             const float r = bv[q] - fmaf(dg[q], xc[q], -off);
-            const float dn = fmaf(cd, dp[q], cr * rcp_or_zero(dg[q]) * r);
+            const float dn = fmaf(cd, dp[q], cr * rdg[q] * r);
             xo4[q] = xc[q] + dn;
             dn_out[q] = dn;
         }
@@ -698,16 +701,20 @@ __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
         return dg;
     };
     // step A at plane q: x2 (returned) and d2; x1 of this row at planes q and
-    // q + 1 is read back from LDS (this wavefront wrote it)
-    auto step_a = [&](int q, f4 xm, const Early &e, f4 cxp, const Rest &r, f4 &d2) {
+    // q + 1 is read back from LDS (this wavefront wrote it). The plane's
+    // diagonal and its reciprocal (dg, rdg) are kept for step B on the same
+    // plane one iteration later.
+    auto step_a = [&](int q, f4 xm, const Early &e, f4 cxp, const Rest &r, f4 &d2, f4 &dg, f4 &rdg) {
         const f4 xc = sx1[q & 1][row + 1][lane], xp = sx1[(q + 1) & 1][row + 1][lane];
         const f4 xym = sx1[q & 1][row][lane], xyp = sx1[q & 1][row + 2][lane];
         const f4 cym = scy[q & 3][row][lane], cyp = scy[q & 3][row + 1][lane];
-        const f4 dg = diag_at(q, e.cx, cxp, cym, cyp, r);
+        dg = diag_at(q, e.cx, cxp, cym, cyp, r);
+#pragma unroll
+        for (int k = 0; k < 4; k++) rdg[k] = rcp_or_zero(dg[k]);
         d2 = z4;
         if (!active(q, amask)) return z4;  // uniform over the wavefront
-        f4 x2 = cheb(xc, xm, xp, xym, xyp, e.cx, cxp, cym, cyp, r.cz, r.bv, dg, kDx ? xc : r.dv, cdA,
-                     crA, d2);
+        f4 x2 = cheb(xc, xm, xp, xym, xyp, e.cx, cxp, cym, cyp, r.cz, r.bv, dg, rdg, kDx ? xc : r.dv,
+                     cdA, crA, d2);
         if (!zon) x2 = z4;
         return x2;
     };
@@ -722,30 +729,42 @@ __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
     put_early(i0 - 1, em);
     put_early(i0, e0);
     __syncthreads();
-    f4 dtmp, d2c;
-    const f4 x2m = step_a(i0 - 1, xpre, em, e0.cx, rm, dtmp);
+    f4 dtmp, d2c, gtmp, rtmp, dgc, rdgc;
+    const f4 x2m = step_a(i0 - 1, xpre, em, e0.cx, rm, dtmp, gtmp, rtmp);
     __syncthreads();
     put_early(i0 + 1, e1);
     __syncthreads();
-    const f4 x2c = step_a(i0, em.x, e0, e1.cx, r0, d2c);
+    const f4 x2c = step_a(i0, em.x, e0, e1.cx, r0, d2c, dgc, rdgc);
     sx2[(i0 + 2) % 3][row][lane] = x2m;  // (i0 - 1) mod 3
     sx2[i0 % 3][row][lane] = x2c;
     __syncthreads();
+    // Rings indexed by plane - i0 mod 3 (the march is unrolled by three, so
+    // every index is a constant and nothing is copied between iterations):
+    // E holds x1 / cpl_x / cpl_y of planes i + 1 and i + 2 and receives plane
+    // i + 3 in the slot of plane i; R the rest of planes i, i + 1 (and i + 2,
+    // loaded into the slot of plane i - 1); D2 / DG / RDG the direction, the
+    // diagonal and its reciprocal of planes i and i + 1 from step A.
+    Early E[3] = {e0, e1, e2};
+    Rest R[3] = {r0, r1, Rest{z4, z4, z4, z4}};
+    f4 D2[3], DG[3], RDG[3];
+    D2[0] = d2c;
+    DG[0] = dgc;
+    RDG[0] = rdgc;
     f4 xa = e0.x;     // x1 at plane i
     f4 cxa = e0.cx;   // cpl_x at plane i
     f4 x2prev = x2m;  // x2 at plane i - 1 (this row)
     (void)x2c;
-    for (int i = i0; i < ie; i++) {
+    auto iter = [&](auto Uc, int i) {
+        constexpr int U = decltype(Uc)::value, U1 = (U + 1) % 3, U2 = (U + 2) % 3;
         // 1. x1 and cpl_y of plane i + 2 into LDS
-        put_early(i + 2, e2);
+        put_early(i + 2, E[U2]);
         // 2. step A on plane i + 1
-        f4 d2n;
-        const f4 x2n = step_a(i + 1, xa, e1, e2.cx, r1, d2n);
+        const f4 x2n = step_a(i + 1, xa, E[U1], E[U2].cx, R[U1], D2[U1], DG[U1], RDG[U1]);
         sx2[(i + 1) % 3][row][lane] = x2n;
         __syncthreads();
-        // 3. loads for the next iteration, in flight during step B
-        const Early e3 = load_early(i + 3);
-        const Rest r2 = load_rest(i + 2);
+        // 3. loads for the next iterations, in flight during step B
+        E[U] = load_early(i + 3);
+        R[U2] = load_rest(i + 2);
         // 4. step B on plane i (the tile's interior rows)
         if (interior) {
             const int bi = i % 3;
@@ -753,22 +772,22 @@ __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
             if (active(i, amask)) {
                 const f4 xym = sx2[bi][row - 1][lane], xyp = sx2[bi][row + 1][lane];
                 const f4 cym = scy[i & 3][row][lane], cyp = scy[i & 3][row + 1][lane];
-                const f4 dg = diag_at(i, cxa, e1.cx, cym, cyp, r0);
                 f4 dn;
-                const f4 x3 = cheb(xc, x2prev, x2n, xym, xyp, cxa, e1.cx, cym, cyp, r0.cz, r0.bv, dg, d2c,
-                                   cdB, crB, dn);
+                const f4 x3 = cheb(xc, x2prev, x2n, xym, xyp, cxa, E[U1].cx, cym, cyp, R[U].cz, R[U].bv, DG[U],
+                                   RDG[U], D2[U], cdB, crB, dn);
                 if (zon) *(f4 *)(xo + c0 + (unsigned)i * sx) = x3;
             }
             x2prev = xc;
         }
-        // rotate
-        xa = e1.x;
-        cxa = e1.cx;
-        e1 = e2;
-        e2 = e3;
-        r0 = r1;
-        r1 = r2;
-        d2c = d2n;
+        xa = E[U1].x;
+        cxa = E[U1].cx;
+    };
+    for (int i = i0; i < ie; i += 3) {  // uniform over the block
+        iter(std::integral_constant<int, 0>{}, i);
+        if (i + 1 >= ie) break;
+        iter(std::integral_constant<int, 1>{}, i + 1);
+        if (i + 2 >= ie) break;
+        iter(std::integral_constant<int, 2>{}, i + 2);
     }
 }
 
