@@ -756,15 +756,17 @@ __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
     (void)x2c;
     auto iter = [&](auto Uc, int i) {
         constexpr int U = decltype(Uc)::value, U1 = (U + 1) % 3, U2 = (U + 2) % 3;
-        // 1. x1 and cpl_y of plane i + 2 into LDS
+        // 1. loads for the next iteration, in flight during this one's two
+        // steps: their slots hold planes i and i - 1, which this iteration
+        // no longer reads (plane i's x1 and cpl_x are in xa, cxa)
+        E[U] = load_early(i + 3);
+        R[U2] = load_rest(i + 2);
+        // 2. x1 and cpl_y of plane i + 2 into LDS
         put_early(i + 2, E[U2]);
-        // 2. step A on plane i + 1
+        // 3. step A on plane i + 1
         const f4 x2n = step_a(i + 1, xa, E[U1], E[U2].cx, R[U1], D2[U1], DG[U1], RDG[U1]);
         sx2[(i + 1) % 3][row][lane] = x2n;
         __syncthreads();
-        // 3. loads for the next iterations, in flight during step B
-        E[U] = load_early(i + 3);
-        R[U2] = load_rest(i + 2);
         // 4. step B on plane i (the tile's interior rows)
         if (interior) {
             const int bi = i % 3;
