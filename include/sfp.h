@@ -21,7 +21,8 @@
 extern "C" {
 #endif
 
-#define SFP_ABI_VERSION 2  /* 2: sfp_params gained the learned cycle's smoother (net_*) */
+#define SFP_ABI_VERSION 3  /* 2: sfp_params gained the learned cycle's smoother (net_*);
+                              3: and magnetic_bc */
 
 typedef enum sfp_status {
     SFP_OK = 0,
@@ -64,7 +65,18 @@ typedef struct sfp_params {
     double net_cheby_ratio;        /* 0 or > 1 */
     int net_coarse_cheby_degree;   /* 0 or 1..8 */
     double net_coarse_cheby_ratio; /* 0 or > 1 */
+    int magnetic_bc;               /* sfp_magnetic_bc */
 } sfp_params;
+
+/* Boundary condition of the magnetic potential on the box faces. */
+typedef enum sfp_magnetic_bc {
+    SFP_MAGNETIC_BC_REFERENCE = 0,    /* MagneticField::BuildMatrix's: box faces skipped (zero
+                                         normal flux of grad phi), cell 0 pinned to 0 */
+    SFP_MAGNETIC_BC_DIRICHLET_BOX = 1 /* phi = 0 on the box faces (mirrored ghost cells: a box
+                                         face adds 2 mu to the diagonal and its (mu - 1) H_ext flux
+                                         to the right-hand side); every cell an unknown. No
+                                         counterpart in the reference: an opt-in variant */
+} sfp_magnetic_bc;
 
 typedef struct sfp_stats {
     int iterations;      /* outer iterations */

@@ -21,6 +21,7 @@ enum Mutation {
     MUT_NO_CLAMP = 5,           // 1/theta at both ghost-fluid sites
     MUT_TRILERP_SWAP = 6,       // trilinear weights (1 - f, f) swapped
     MUT_FACE_ORIGIN = 7,        // face grids of the field placed at the cell centres
+    MUT_BOX_FACE_WEIGHT = 8,    // Dirichlet box: mu instead of 2 mu per box face
 };
 static int g_mutation = MUT_NONE;
 
@@ -298,7 +299,7 @@ void orc_magnetic_coeff(int nx, int ny, int nz, double spacing, const double ori
     }
 }
 
-int64_t orc_magnetic_matrix(int nx, int ny, int nz, const double *const mu[3], int *rowptr,
+int64_t orc_magnetic_matrix(int nx, int ny, int nz, const double *const mu[3], int bc, int *rowptr,
                             int *col, double *val) {
     Dims d{{nx, ny, nz}};
     Dims fd[3] = {face_dims(d, 0), face_dims(d, 1), face_dims(d, 2)};
@@ -308,7 +309,7 @@ int64_t orc_magnetic_matrix(int nx, int ny, int nz, const double *const mu[3], i
     std::vector<std::pair<int, double>> row;
     for (int64_t r = 0; r < total; r++) {
         row.clear();
-        if (r == 0) {
+        if (r == 0 && bc == 0) {
             row.emplace_back(0, 1.);
         } else {
             int cell[3];
@@ -318,12 +319,16 @@ int64_t orc_magnetic_matrix(int nx, int ny, int nz, const double *const mu[3], i
                 int axis = axis_of(i), side = side_of(i);
                 int nb[3] = {cell[0], cell[1], cell[2]};
                 nb[axis] += side;
-                if (!d.valid(nb[0], nb[1], nb[2])) continue;
                 int face[3] = {cell[0], cell[1], cell[2]};
                 face[axis] += (i & 1);
                 double weight = mu[axis][fd[axis].index(face[0], face[1], face[2])];
+                if (!d.valid(nb[0], nb[1], nb[2])) {
+                    // Dirichlet box: phi = 0 on the face, ghost value -phi
+                    if (bc == 1) diag += (g_mutation == MUT_BOX_FACE_WEIGHT ? 1. : 2.) * weight;
+                    continue;
+                }
                 int64_t c = d.index(nb[0], nb[1], nb[2]);
-                if (c != 0) row.emplace_back((int)c, -weight);
+                if (c != 0 || bc == 1) row.emplace_back((int)c, -weight);
                 diag += weight;
             }
             row.emplace_back((int)r, diag);
@@ -341,20 +346,20 @@ int64_t orc_magnetic_matrix(int nx, int ny, int nz, const double *const mu[3], i
 }
 
 void orc_magnetic_rhs(int nx, int ny, int nz, double spacing, const double *const mu[3],
-                      const double *const hext[3], double *rhs) {
+                      const double *const hext[3], int bc, double *rhs) {
     Dims d{{nx, ny, nz}};
     Dims fd[3] = {face_dims(d, 0), face_dims(d, 1), face_dims(d, 2)};
     int64_t total = (int64_t)nx * ny * nz;
     for (int64_t r = 0; r < total; r++) rhs[r] = 0;
 #pragma omp parallel for schedule(static)
-    for (int64_t r = 1; r < total; r++) {
+    for (int64_t r = bc == 1 ? 0 : 1; r < total; r++) {
         int cell[3];
         coord_of(d, r, cell);
         for (int i = 0; i < 6; i++) {
             int axis = axis_of(i), side = side_of(i);
             int nb[3] = {cell[0], cell[1], cell[2]};
             nb[axis] += side;
-            if (!d.valid(nb[0], nb[1], nb[2])) continue;
+            if (!d.valid(nb[0], nb[1], nb[2]) && bc == 0) continue;
             int face[3] = {cell[0], cell[1], cell[2]};
             face[axis] += (i & 1);
             int64_t fidx = fd[axis].index(face[0], face[1], face[2]);
@@ -367,8 +372,8 @@ void orc_magnetic_rhs(int nx, int ny, int nz, double spacing, const double *cons
 // ||rhs - A x||^2 of MagneticField::BuildMatrix's system without storing A
 // (rows as in orc_magnetic_matrix: row 0 the identity, column 0 dropped),
 // for grids whose CSR would not fit; *bb receives ||rhs||^2.
-double orc_magnetic_residual(int nx, int ny, int nz, const double *const mu[3], const double *rhs,
-                             const double *x, double *bb) {
+double orc_magnetic_residual(int nx, int ny, int nz, const double *const mu[3], int bc,
+                             const double *rhs, const double *x, double *bb) {
     Dims d{{nx, ny, nz}};
     Dims fd[3] = {face_dims(d, 0), face_dims(d, 1), face_dims(d, 2)};
     int64_t total = (int64_t)nx * ny * nz;
@@ -376,7 +381,7 @@ double orc_magnetic_residual(int nx, int ny, int nz, const double *const mu[3], 
 #pragma omp parallel for schedule(static) reduction(+ : rr, b2)
     for (int64_t r = 0; r < total; r++) {
         double ax;
-        if (r == 0) {
+        if (r == 0 && bc == 0) {
             ax = x[0];
         } else {
             int cell[3];
@@ -386,12 +391,15 @@ double orc_magnetic_residual(int nx, int ny, int nz, const double *const mu[3], 
                 int axis = axis_of(i), side = side_of(i);
                 int nb[3] = {cell[0], cell[1], cell[2]};
                 nb[axis] += side;
-                if (!d.valid(nb[0], nb[1], nb[2])) continue;
                 int face[3] = {cell[0], cell[1], cell[2]};
                 face[axis] += (i & 1);
                 double weight = mu[axis][fd[axis].index(face[0], face[1], face[2])];
+                if (!d.valid(nb[0], nb[1], nb[2])) {
+                    if (bc == 1) diag += 2. * weight;
+                    continue;
+                }
                 int64_t c = d.index(nb[0], nb[1], nb[2]);
-                if (c != 0) off += weight * x[c];
+                if (c != 0 || bc == 1) off += weight * x[c];
                 diag += weight;
             }
             ax = diag * x[r] - off;

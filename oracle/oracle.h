@@ -62,19 +62,23 @@ int orc_pressure_apply(int nx, int ny, int nz, const double *phi, const double *
 void orc_magnetic_coeff(int nx, int ny, int nz, double spacing, const double origin[3],
                         const double *phi, double chi, double band, double *const mu[3]);
 
-/* MagneticField::BuildMatrix matrix part (core/MagneticField.cpp:62-90). */
-int64_t orc_magnetic_matrix(int nx, int ny, int nz, const double *const mu[3],
+/* MagneticField::BuildMatrix matrix part (core/MagneticField.cpp:62-90).
+ * bc = 0: the reference (box faces skipped, cell 0 pinned); bc = 1: the
+ * library's opt-in Dirichlet box (phi = 0 on the box faces by a mirrored
+ * ghost value: 2 mu_face on the diagonal; no pin), no reference counterpart. */
+int64_t orc_magnetic_matrix(int nx, int ny, int nz, const double *const mu[3], int bc,
                             int *rowptr, int *col, double *val);
 
-/* MagneticField::AssembleRHS (core/MagneticField.cpp:97-118). */
+/* MagneticField::AssembleRHS (core/MagneticField.cpp:97-118); bc = 1 adds
+ * the box faces' (mu - 1) H_ext fluxes and fills row 0. */
 void orc_magnetic_rhs(int nx, int ny, int nz, double spacing, const double *const mu[3],
-                      const double *const hext[3], double *rhs);
+                      const double *const hext[3], int bc, double *rhs);
 
 /* MagneticField::AssignTotalField (core/MagneticField.cpp:125-143). */
 /* ||rhs - A x||_2^2 of the BuildMatrix system (core/MagneticField.cpp:62-90)
  * computed row by row without the CSR (grids whose matrix would not fit);
  * *bb receives ||rhs||_2^2. */
-double orc_magnetic_residual(int nx, int ny, int nz, const double *const mu[3],
+double orc_magnetic_residual(int nx, int ny, int nz, const double *const mu[3], int bc,
                              const double *rhs, const double *x, double *bb);
 
 void orc_magnetic_total_field(int nx, int ny, int nz, double spacing, const double *potential,

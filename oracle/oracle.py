@@ -44,11 +44,11 @@ def lib():
         L.orc_magnetic_coeff.argtypes = [C.c_int] * 3 + [C.c_double, _dp, _dp, C.c_double,
                                                           C.c_double, P3]
         L.orc_magnetic_matrix.restype = C.c_int64
-        L.orc_magnetic_matrix.argtypes = [C.c_int] * 3 + [P3, _ip, _ip, _dp]
+        L.orc_magnetic_matrix.argtypes = [C.c_int] * 3 + [P3, C.c_int, _ip, _ip, _dp]
         L.orc_magnetic_rhs.restype = None
-        L.orc_magnetic_rhs.argtypes = [C.c_int] * 3 + [C.c_double, P3, P3, _dp]
+        L.orc_magnetic_rhs.argtypes = [C.c_int] * 3 + [C.c_double, P3, P3, C.c_int, _dp]
         L.orc_magnetic_residual.restype = C.c_double
-        L.orc_magnetic_residual.argtypes = [C.c_int] * 3 + [P3, _dp, _dp, _dp]
+        L.orc_magnetic_residual.argtypes = [C.c_int] * 3 + [P3, C.c_int, _dp, _dp, _dp]
         L.orc_magnetic_total_field.restype = None
         L.orc_magnetic_total_field.argtypes = [C.c_int] * 3 + [C.c_double, _dp, P3, P3]
         L.orc_magnetic_vertex_pressures.restype = None
@@ -86,7 +86,8 @@ def lib():
 # Planted mistakes (orc_set_mutation in assembly.cpp): each changes one
 # expression of the restatement so that a closed-form test must fail.
 MUTATIONS = {"heaviside_no_sin": 1, "heaviside_half_band": 2, "jump_sign": 3,
-             "jump_no_coef": 4, "no_clamp": 5, "trilerp_swap": 6, "face_origin": 7}
+             "jump_no_coef": 4, "no_clamp": 5, "trilerp_swap": 6, "face_origin": 7,
+             "box_face_weight": 8}
 
 
 class mutated:
@@ -209,8 +210,10 @@ def magnetic_coeff(sc):
     return mu
 
 
-def magnetic_system(sc, mu=None):
-    """MagneticField::BuildMatrix + AssembleRHS. Returns dict with mu, A, rhs."""
+def magnetic_system(sc, mu=None, bc=0):
+    """MagneticField::BuildMatrix + AssembleRHS. Returns dict with mu, A, rhs.
+    bc = 1: the library's opt-in Dirichlet box instead of the reference's
+    boundary (sfp_magnetic_bc)."""
     geo = sc["geo"]
     nx, ny, nz = geo.n
     if mu is None:
@@ -220,15 +223,15 @@ def magnetic_system(sc, mu=None):
     col = np.zeros(7 * n, np.int32)
     val = np.zeros(7 * n, np.float64)
     mp, k1 = _p3(mu)
-    nnz = lib().orc_magnetic_matrix(nx, ny, nz, mp, _i(rowptr), _i(col), _d(val))
+    nnz = lib().orc_magnetic_matrix(nx, ny, nz, mp, int(bc), _i(rowptr), _i(col), _d(val))
     A = sp.csr_matrix((val[:nnz], col[:nnz], rowptr), shape=(n, n))
     rhs = np.zeros(n)
     hp, k2 = _p3(sc["hext"])
-    lib().orc_magnetic_rhs(nx, ny, nz, geo.spacing, mp, hp, _d(rhs))
+    lib().orc_magnetic_rhs(nx, ny, nz, geo.spacing, mp, hp, int(bc), _d(rhs))
     return dict(mu=mu, A=A, rhs=rhs, n=n)
 
 
-def magnetic_rhs(sc, mu):
+def magnetic_rhs(sc, mu, bc=0):
     """MagneticField::AssembleRHS alone (N^3 doubles), for grids whose CSR
     matrix would not fit in memory."""
     geo = sc["geo"]
@@ -236,18 +239,18 @@ def magnetic_rhs(sc, mu):
     rhs = np.zeros(geo.ncells)
     mp, k1 = _p3(mu)
     hp, k2 = _p3(sc["hext"])
-    lib().orc_magnetic_rhs(nx, ny, nz, geo.spacing, mp, hp, _d(rhs))
+    lib().orc_magnetic_rhs(nx, ny, nz, geo.spacing, mp, hp, int(bc), _d(rhs))
     return rhs
 
 
-def magnetic_rel_residual(sc, mu, rhs, x):
+def magnetic_rel_residual(sc, mu, rhs, x, bc=0):
     """||rhs - A x||_2 / ||rhs||_2 of the magnetic system, row by row without
     the CSR (any grid size)."""
     geo = sc["geo"]
     nx, ny, nz = geo.n
     mp, k1 = _p3(mu)
     bb = C.c_double(0)
-    rr = lib().orc_magnetic_residual(nx, ny, nz, mp, _d(np.ascontiguousarray(rhs)),
+    rr = lib().orc_magnetic_residual(nx, ny, nz, mp, int(bc), _d(np.ascontiguousarray(rhs)),
                                      _d(np.ascontiguousarray(np.asarray(x, np.float64).ravel())),
                                      C.byref(bb))
     return float(np.sqrt(rr / bb.value)) if bb.value > 0 else float(np.sqrt(rr))

@@ -260,8 +260,13 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_mu(CellGrid cg, Dims d, int
     }
 }
 
+// dirichlet = 0: MagneticField::BuildMatrix / AssembleRHS (box faces skipped,
+// cell 0 pinned by elimination). dirichlet = 1, an opt-in variant with no
+// reference counterpart: phi = 0 on the box faces (a mirrored ghost value, so
+// a box face adds 2 mu_face to the diagonal and its (mu_face - 1) H_ext flux
+// to the right-hand side, like an interior face), every cell an unknown.
 __global__ __launch_bounds__(kBlock) void k_magnetic_assemble(CellGrid cg, Dims d, Faces mu, Faces hext,
-                                                              double spacing,
+                                                              double spacing, int dirichlet,
                                                               double *__restrict__ diag64,
                                                               MutFaces cpl64,
                                                               double *__restrict__ rhs,
@@ -275,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_assemble(CellGrid cg, Dims 
     for (int i = i0, ie = min(i0 + cg.planes, d.nx); i < ie; i++) {
         const size_t c = d.idx(i, j, k);
         double low[3] = {0.0, 0.0, 0.0};
-        if (c == 0) {  // pinned cell: identity row, zero right-hand side
+        if (c == 0 && !dirichlet) {  // pinned cell: identity row, zero right-hand side
             diag64[c] = 1.0;
             rhs[c] = 0.0;
             diag32[c] = 0.f;
@@ -294,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_assemble(CellGrid cg, Dims 
                 face[axis] += (ord & 1);
                 const size_t fidx = d.face(axis).idx(face[0], face[1], face[2]);
                 nbv[ord] = valid(d, nb);
-                pin[ord] = nb[0] == 0 && nb[1] == 0 && nb[2] == 0;
+                pin[ord] = !dirichlet && nb[0] == 0 && nb[1] == 0 && nb[2] == 0;
                 mv[ord] = mu.a[axis][fidx];
                 hv[ord] = hext.a[axis][fidx];
             }
@@ -310,7 +315,13 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_assemble(CellGrid cg, Dims 
 #pragma unroll
             for (int ord = 0; ord < 6; ord++) {
                 const int axis = ord >> 1, side = (ord & 1) ? 1 : -1;
-                if (!nbv[ord]) continue;
+                if (!nbv[ord]) {
+                    if (dirichlet) {  // box face: ghost value -phi
+                        diag += 2. * mv[ord];
+                        b += (mv[ord] - 1.) * hv[ord] * (double)(-side) * spacing;
+                    }
+                    continue;
+                }
                 const double weight = mv[ord];
                 // column 0 is dropped from every other row (pin by elimination)
                 if (side < 0 && !pin[ord]) low[axis] = weight;
@@ -321,7 +332,9 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_assemble(CellGrid cg, Dims 
             }
             diag64[c] = diag;
             rhs[c] = b;
-            diag32[c] = (float)(next_to_pin ? diag : diag_f);
+            // (with box faces in the diagonal the FP32 one is not a row sum:
+            // the FP64 one rounded)
+            diag32[c] = (float)(next_to_pin || dirichlet ? diag : diag_f);
         }
         for (int a = 0; a < 3; a++) {
             cpl64.a[a][c] = low[a];
@@ -418,11 +431,11 @@ void launch_magnetic_mu(hipStream_t st, Dims d, const MagGeom &g, const double *
 
 void launch_magnetic_assemble(hipStream_t st, Dims d, Faces mu, Faces hext, double spacing,
                               double *diag64, double *const cpl64[3], double *rhs,
-                              float *diag32, float *const cpl32[3]) {
+                              float *diag32, float *const cpl32[3], int dirichlet) {
     Launch L = cell_launch(d, 0, kAssemblyPlanes);
     MutFaces c{{cpl64[0], cpl64[1], cpl64[2]}};
-    hipLaunchKernelGGL(k_magnetic_assemble, L.grid, L.block, 0, st, L.cg, d, mu, hext, spacing, diag64,
-                       c, rhs, diag32, cpl32[0], cpl32[1], cpl32[2]);
+    hipLaunchKernelGGL(k_magnetic_assemble, L.grid, L.block, 0, st, L.cg, d, mu, hext, spacing,
+                       dirichlet, diag64, c, rhs, diag32, cpl32[0], cpl32[1], cpl32[2]);
     SFP_LAUNCH_CHECK();
 }
 

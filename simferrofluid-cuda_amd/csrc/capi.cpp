@@ -40,6 +40,7 @@ sfp_params to_params(const SolverConfig &c) {
     p.net_cheby_ratio = c.net_cheby_ratio;
     p.net_coarse_cheby_degree = c.net_coarse_cheby_degree;
     p.net_coarse_cheby_ratio = c.net_coarse_cheby_ratio;
+    p.magnetic_bc = c.magnetic_bc;
     return p;
 }
 
@@ -193,9 +194,11 @@ void magnetic_dev(sfp_context ctx, const double *phi, const double *const hext[3
     const Faces H{{hext[0], hext[1], hext[2]}};
     double *c64[3] = {L.cpl64[0].p, L.cpl64[1].p, L.cpl64[2].p};
     float *c32[3] = {L.cpl32[0].p, L.cpl32[1].p, L.cpl32[2].p};
+    const int dirichlet = ctx->cfg.magnetic_bc == SFP_MAGNETIC_BC_DIRICHLET_BOX;
     launch_magnetic_assemble(s, d, Mf, H, ctx->spacing, L.diag64.p, c64, sys.rhs().p, L.diag32.p,
-                             c32);
-    L.pinned_rowsum = ctx->ws.rowsum_diag;
+                             c32, dirichlet);
+    // the row-sum diagonal holds for the reference's boundary only
+    L.pinned_rowsum = ctx->ws.rowsum_diag && !dirichlet;
     sys.build_hierarchy(ctx->ws, ctx->cfg);
     SFP_HIP(hipEventRecord(ctx->ev[1], s));
     ctx->ws.phi = phi;
@@ -216,7 +219,7 @@ void magnetic_dev(sfp_context ctx, const double *phi, const double *const hext[3
                                        hipMemcpyDeviceToDevice, s));
     SFP_HIP(hipEventRecord(ctx->ev[3], s));
     SFP_HIP(hipStreamSynchronize(s));
-    fill_stats(st, info, (long long)d.cells() - 1, elapsed(ctx->ev[0], ctx->ev[1]),
+    fill_stats(st, info, (long long)d.cells() - (dirichlet ? 0 : 1), elapsed(ctx->ev[0], ctx->ev[1]),
                elapsed(ctx->ev[1], ctx->ev[2]), elapsed(ctx->ev[2], ctx->ev[3]));
 }
 
@@ -525,7 +528,8 @@ sfp_status sfp_set_params(sfp_context ctx, const sfp_params *p) {
         p->coarse_cheby_degree < 1 || p->coarse_cheby_degree > 8 || !(p->coarse_cheby_ratio > 1) ||
         p->net_cheby_degree < 0 || p->net_cheby_degree > 8 || p->net_coarse_cheby_degree < 0 ||
         p->net_coarse_cheby_degree > 8 || !(p->net_cheby_ratio == 0 || p->net_cheby_ratio > 1) ||
-        !(p->net_coarse_cheby_ratio == 0 || p->net_coarse_cheby_ratio > 1)) {
+        !(p->net_coarse_cheby_ratio == 0 || p->net_coarse_cheby_ratio > 1) ||
+        (p->magnetic_bc != SFP_MAGNETIC_BC_REFERENCE && p->magnetic_bc != SFP_MAGNETIC_BC_DIRICHLET_BOX)) {
         ctx->last_error = "parameter out of range";
         return SFP_ERR_ARGUMENT;
     }
@@ -549,6 +553,7 @@ sfp_status sfp_set_params(sfp_context ctx, const sfp_params *p) {
     c.net_cheby_ratio = p->net_cheby_ratio;
     c.net_coarse_cheby_degree = p->net_coarse_cheby_degree;
     c.net_coarse_cheby_ratio = p->net_coarse_cheby_ratio;
+    c.magnetic_bc = p->magnetic_bc;
     return SFP_OK;
 }
 

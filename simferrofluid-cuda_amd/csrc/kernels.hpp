@@ -201,9 +201,11 @@ void launch_pressure_apply(hipStream_t s, Dims d, const double *phi, Faces frac,
 // MagneticField::BuildMatrix coefficients, matrix and AssembleRHS.
 void launch_magnetic_mu(hipStream_t s, Dims d, const MagGeom &g, const double *phi, double chi,
                         double band, MutFaces mu);
+// dirichlet: the opt-in box condition phi = 0 (no pinned cell) instead of
+// the reference's Neumann box faces with cell 0 pinned.
 void launch_magnetic_assemble(hipStream_t s, Dims d, Faces mu, Faces hext, double spacing,
                               double *diag64, double *const cpl64[3], double *rhs,
-                              float *diag32, float *const cpl32[3]);
+                              float *diag32, float *const cpl32[3], int dirichlet = 0);
 // MagneticField::AssignTotalField.
 void launch_magnetic_total_field(hipStream_t s, Dims d, double inv, const double *pot,
                                  Faces hext, MutFaces htot);

@@ -31,6 +31,9 @@ struct SolverConfig {
     double net_cheby_ratio = 0.0;
     int net_coarse_cheby_degree = 0;
     double net_coarse_cheby_ratio = 0.0;
+    // magnetic box condition: 0 the reference's (Neumann box faces, cell 0
+    // pinned), 1 phi = 0 on the box faces
+    int magnetic_bc = 0;
 };
 
 // Chebyshev recurrence coefficients for D^{-1}A on [lmax/ratio, lmax].

@@ -22,6 +22,7 @@ _ip = C.POINTER(C.c_int)
 
 OK, ERR_ARGUMENT, ERR_INPUT, ERR_HIP, ERR_NO_DEVICE, ERR_STATE, ERR_UNSUPPORTED = range(7)
 PRECOND_MG, PRECOND_LEARNED, PRECOND_JACOBI, PRECOND_SMOOTH = 0, 1, 2, 3
+MAGNETIC_BC_REFERENCE, MAGNETIC_BC_DIRICHLET_BOX = 0, 1
 KERNEL_SMOOTH, KERNEL_MATVEC64, KERNEL_UPDATE64, KERNEL_RESTRICT = 0, 1, 2, 3
 KERNEL_SMOOTH_PAIR, KERNEL_SMOOTH_RESTRICT = 4, 5
 KERNEL_NAMES = {0: "smooth", 1: "matvec64", 2: "update64", 3: "restrict", 4: "smooth_pair",
@@ -59,7 +60,8 @@ class Params(C.Structure):
                 ("coarse_scale", C.c_double), ("stall_ratio", C.c_double), ("verbose", C.c_int),
                 ("coarse_cheby_degree", C.c_int), ("coarse_cheby_ratio", C.c_double),
                 ("net_cheby_degree", C.c_int), ("net_cheby_ratio", C.c_double),
-                ("net_coarse_cheby_degree", C.c_int), ("net_coarse_cheby_ratio", C.c_double)]
+                ("net_coarse_cheby_degree", C.c_int), ("net_coarse_cheby_ratio", C.c_double),
+                ("magnetic_bc", C.c_int)]
 
 
 class Stats(C.Structure):

@@ -188,9 +188,9 @@ int main() {
     M.ptr.resize(ncell + 1);
     M.col.resize(7 * (size_t)ncell);
     M.val.resize(7 * (size_t)ncell);
-    orc_magnetic_matrix(n[0], n[1], n[2], muc, M.ptr.data(), M.col.data(), M.val.data());
+    orc_magnetic_matrix(n[0], n[1], n[2], muc, 0, M.ptr.data(), M.col.data(), M.val.data());
     std::vector<double> mrhs(ncell);
-    orc_magnetic_rhs(n[0], n[1], n[2], sgrid.GetSpacing(), muc, hp, mrhs.data());
+    orc_magnetic_rhs(n[0], n[1], n[2], sgrid.GetSpacing(), muc, hp, 0, mrhs.data());
     double const mrel = orc_rel_residual(ncell, M.ptr.data(), M.col.data(), M.val.data(), mrhs.data(),
                                          field.GetPotential().data());
     check(mrel <= 1e-6, "oracle relative residual of the GPU potential <= 1e-6 (shipped tol)", mrel);

@@ -152,6 +152,48 @@ def test_magnetic_solve_parity(oracle, name):
         assert np.array_equal(H[a], H_ref[a])
 
 
+# ------------------------------------------------------------------ opt-in Dirichlet box
+@pytest.mark.parametrize("name", ["wavy_32_chi2", "spike_ragged"])
+def test_magnetic_dirichlet_box(oracle, name):
+    """sfp_params.magnetic_bc = SFP_MAGNETIC_BC_DIRICHLET_BOX (no reference
+    counterpart): the assembled operator and right-hand side against the
+    oracle's bc = 1 system (within 4 ulp like the reference path's: the
+    permeability goes through the device's sin in the band), the solve to
+    tolerance on that system, and the total field bit for bit given the
+    potential."""
+    sc = _mag(name)
+    print(f"[case] magnetic Dirichlet box {name} shape={sc['geo'].n}", flush=True)
+    ref = oracle.magnetic_system(sc, bc=1)
+    x_ref = spla.spsolve(ref["A"].tocsc(), ref["rhs"])
+    with sfp.Context.for_scene(sc, tol=1e-10, magnetic_bc=sfp.MAGNETIC_BC_DIRICHLET_BOX) as ctx:
+        pot, H, st = ctx.magnetic_solve(sc)
+        got = ctx.export_level(1, 0)
+    n = sc["geo"].ncells
+    diag, cpl, rhs = full_grid_operator(ref["A"], ref["rhs"], np.arange(n), sc["geo"].n)
+    assert ulp_diff(got["diag"], diag) <= 4
+    for a in range(3):
+        assert ulp_diff(got["cpl"][a], cpl[a]) <= 4
+    assert np.abs(got["rhs"] - rhs).max() <= 8 * np.finfo(float).eps * np.abs(rhs).max()
+    x = pot.ravel()
+    rel = oracle.rel_residual(ref["A"], ref["rhs"], x)
+    print(f"  iters={st['iterations']} rel={rel:.3e} unknowns={st['unknowns']}", flush=True)
+    assert st["converged"] == 1 and rel <= 1.01e-10
+    assert st["unknowns"] == n  # cell 0 is an unknown too
+    assert np.abs(x - x_ref).max() / np.abs(x_ref).max() < 1e-6
+    H_ref = oracle.magnetic_total_field(sc, pot)
+    for a in range(3):
+        assert np.array_equal(H[a], H_ref[a])
+    # the reference boundary on the same context afterwards: pinned again
+    with sfp.Context.for_scene(sc, tol=1e-10) as ctx:
+        ctx.set_params(magnetic_bc=sfp.MAGNETIC_BC_DIRICHLET_BOX)
+        ctx.magnetic_solve(sc, want_field=False)
+        ctx.set_params(magnetic_bc=sfp.MAGNETIC_BC_REFERENCE)
+        ctx.reset_warm_start()
+        pot0, _, st0 = ctx.magnetic_solve(sc, want_field=False)
+    ref0 = oracle.magnetic_system(sc)
+    assert pot0.ravel()[0] == 0.0 and oracle.rel_residual(ref0["A"], ref0["rhs"], pot0.ravel()) <= 1.01e-10
+
+
 @pytest.mark.parametrize("n", [(96, 40, 6), (7, 130, 3), (9, 11, 200)])
 def test_anisotropic_grids_solve_to_tolerance(oracle, n):
     """Flat, thin and tall grids: hierarchies whose axes stop halving at

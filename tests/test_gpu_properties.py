@@ -143,12 +143,15 @@ def test_config0_pressure_64_vs_oracle(oracle):
         assert np.array_equal(vel[a], ref_vel[a])
 
 
-def test_config1_magnetic_128_vs_oracle(oracle):
-    print("[case] config 1: magnetic 128^3 chi=2 vs oracle", flush=True)
+@pytest.mark.parametrize("bc", [sfp.MAGNETIC_BC_REFERENCE, sfp.MAGNETIC_BC_DIRICHLET_BOX])
+def test_config1_magnetic_128_vs_oracle(oracle, bc):
+    """Configuration 1 (128^3 wavy interface, chi = 2, band 1.5 dx): the
+    reference's boundary, and the configuration's opt-in Dirichlet box."""
+    print(f"[case] config 1: magnetic 128^3 chi=2 vs oracle, bc={bc}", flush=True)
     sc = scenes.magnetic_scene(128, seed=0, chi=2.0)
-    with sfp.Context.for_scene(sc, tol=1e-8) as ctx:
+    with sfp.Context.for_scene(sc, tol=1e-8, magnetic_bc=bc) as ctx:
         pot, H, st = ctx.magnetic_solve(sc)
-    ref = oracle.magnetic_system(sc)
+    ref = oracle.magnetic_system(sc, bc=bc)
     rel = oracle.rel_residual(ref["A"], ref["rhs"], pot.ravel())
     xr, it, relr, _, _ = oracle.solve(ref["A"], ref["rhs"], "cg", "amg", tol=1e-12)
     linf = np.abs(pot.ravel() - xr).max() / np.abs(xr).max()

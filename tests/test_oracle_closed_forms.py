@@ -14,7 +14,11 @@ planted mistake in the oracle (oracle.mutated) that each one must catch:
     update moves its face by 1000 w p;
   * the trilinear weights and face-grid origins behind the vertex pressures
     (core/TriLerp.h:22-36,54-61, core/MagneticField.cpp:37-48): a linear total
-    field is interpolated exactly at any interior point.
+    field is interpolated exactly at any interior point;
+  * the library's opt-in Dirichlet box for the magnetic potential (no
+    reference counterpart): with mu = 1 its operator is the 7-point Laplacian
+    with phi = 0 on the box faces (mirrored ghosts), whose eigenvectors are
+    products of sin(k pi (i + 1/2) / n) with eigenvalues sum 2 - 2 cos(k pi / n).
 """
 import math
 
@@ -210,3 +214,35 @@ def test_vertex_pressures_of_a_linear_field(oracle):
 def test_trilerp_planted_mistakes_fail_the_closed_form(oracle, mutation):
     with orc.mutated(mutation):
         assert _linear_field_errors(oracle) > 1e-4
+
+
+# ------------------------------------------------------------------ Dirichlet box
+def _box_errors(oracle, shape=(7, 5, 6)):
+    geo = scenes.Geometry(shape, 1.0 / max(shape))
+    X, Y, Z = geo.cell_centers()
+    phi = np.ascontiguousarray(np.broadcast_to(Y - 0.5 + 0 * X + 0 * Z, geo.n)).copy()
+    sc = dict(geo=geo, phi=phi, chi=0.0, band=1.5 * geo.spacing,
+              hext=[np.zeros(geo.face_shape(a)) for a in range(3)])
+    A = orc.magnetic_system(sc, bc=1)["A"]
+    idx = [np.arange(n) + 0.5 for n in shape]
+    worst = 0.0
+    for k in [(1, 1, 1), (2, 3, 1), (7, 5, 6), (4, 1, 5)]:
+        f = [np.sin(k[a] * np.pi * idx[a] / shape[a]) for a in range(3)]
+        u = (f[0][:, None, None] * f[1][None, :, None] * f[2][None, None, :]).ravel()
+        lam = sum(2.0 - 2.0 * np.cos(k[a] * np.pi / shape[a]) for a in range(3))
+        worst = max(worst, np.linalg.norm(A @ u - lam * u) / np.linalg.norm(u))
+    return worst, A
+
+
+def test_dirichlet_box_eigenvectors(oracle):
+    err, A = _box_errors(oracle)
+    assert err <= 1e-12
+    # no pinned row: row 0 has its neighbours; the operator is symmetric
+    assert A[0, 1] == -1.0 and A[0, 0] == 3.0 + 3 * 2.0  # three neighbours, three box faces
+    assert abs(A - A.T).max() == 0.0
+
+
+def test_box_planted_mistake_fails_the_closed_form(oracle):
+    with orc.mutated("box_face_weight"):
+        err, _ = _box_errors(oracle)
+    assert err > 1e-2
