@@ -161,6 +161,19 @@ def harvest_system(ctx: Context, which: int, phi, rng, cg_keep=(2, 6, 16, 40), n
     return out
 
 
+CHIS = (0.5, 1.0, 2.0, 4.0, 8.0)
+
+
+def scene_seeds(count):
+    """Seeds of the training systems: half pressure scenes (100, 101, ...),
+    half magnetic scenes with chi cycling through CHIS; none is used by the
+    tests or the benchmark (seeds 0-9)."""
+    half = count // 2
+    pressure = tuple(range(100, 100 + half))
+    magnetic = tuple((100 + half + i, CHIS[i % len(CHIS)]) for i in range(count - half))
+    return pressure, magnetic
+
+
 def harvest(n=128, pressure_seeds=(100, 101, 102, 103, 104, 105, 106, 107),
             magnetic=((108, 0.5), (109, 1.0), (110, 2.0), (111, 4.0), (112, 8.0), (113, 2.0)),
             seed=0, log=print, smoothing=None):
@@ -184,12 +197,20 @@ def harvest(n=128, pressure_seeds=(100, 101, 102, 103, 104, 105, 106, 107),
 
 
 # ------------------------------------------------------------------ training
-def train(samples, steps=6000, lr=1e-3, seed=0, batch=8, init=None, log=print, log_every=200):
-    """Adam on the masked MSE over batches of randomly flipped samples;
+def train(samples, steps=6000, lr=1e-3, seed=0, batch=8, init=None, log=print, log_every=200,
+          crop=0):
+    """Adam on the masked MSE over batches of randomly flipped samples (crop >
+    0: random crop^3 windows of them, the network being fully convolutional);
     returns (params, losses)."""
     import torch  # device arrays only
 
     shape = samples[0][0].shape[:3]
+    if crop:
+        if crop % 4 or any(crop > e for e in shape):
+            raise ValueError(f"crop {crop} must be a multiple of 4 within the samples' {shape}")
+        out_shape = (crop, crop, crop)
+    else:
+        out_shape = shape
     dev = torch.device("cuda:0")
     inp = [torch.from_numpy(s[0]).to(dev) for s in samples]
     tgt = [torch.from_numpy(s[1]).to(dev) for s in samples]
@@ -197,7 +218,7 @@ def train(samples, steps=6000, lr=1e-3, seed=0, batch=8, init=None, log=print, l
     torch.cuda.synchronize()
     rng = np.random.default_rng(seed)
     losses = []
-    with UNet(batch, *shape, training=True) as net:
+    with UNet(batch, *out_shape, training=True) as net:
         if init is None:
             net.init_params(seed + 1)
         else:
@@ -208,6 +229,10 @@ def train(samples, steps=6000, lr=1e-3, seed=0, batch=8, init=None, log=print, l
             for i in rng.integers(len(samples), size=batch):
                 axes = [a for a in range(3) if rng.uniform() < 0.5]
                 x, t, m = inp[i], tgt[i], msk[i]
+                if crop:
+                    o = [int(rng.integers(0, e - crop + 1)) for e in shape]
+                    win = tuple(slice(o[a], o[a] + crop) for a in range(3))
+                    x, t, m = x[win], t[win], m[win]
                 if axes:
                     x, t, m = torch.flip(x, axes), torch.flip(t, axes), torch.flip(m, axes)
                 xs.append(x)
@@ -317,6 +342,10 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--systems", type=int, default=14,
+                    help="training systems (half pressure, half magnetic scenes)")
+    ap.add_argument("--crop", type=int, default=0,
+                    help="train on random crop^3 windows of the coarse-level samples (0: whole)")
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--cheby", default=None, metavar="DEG:RATIO:COARSE_DEG:COARSE_RATIO",
                     help="V-cycle smoother to train under (default: the library's)")
@@ -346,12 +375,15 @@ def main(argv=None):
         v = a.cheby.split(":")
         sm = dict(cheby_degree=int(v[0]), cheby_ratio=float(v[1]), coarse_cheby_degree=int(v[2]),
                   coarse_cheby_ratio=float(v[3]))
-    samples = harvest(a.n, seed=a.seed, log=log, smoothing=sm)
+    ps, ms = scene_seeds(a.systems)
+    samples = harvest(a.n, ps, ms, seed=a.seed, log=log, smoothing=sm)
     th = time.time() - t0
     log(f"{len(samples)} samples in {th:.1f}s")
-    params, losses = train(samples, a.steps, a.lr, a.seed, a.batch, log=log)
+    params, losses = train(samples, a.steps, a.lr, a.seed, a.batch, log=log, crop=a.crop)
     save_weights(params, a.out, dict(n=a.n, steps=a.steps, lr=a.lr, seed=a.seed, batch=a.batch,
-                                     samples=len(samples), harvest_s=round(th, 1),
+                                     samples=len(samples), systems=a.systems, crop=a.crop,
+                                     crops_seen=a.steps * a.batch if a.crop else 0,
+                                     harvest_s=round(th, 1),
                                      train_s=round(time.time() - t0 - th, 1),
                                      loss_first100=float(np.mean(losses[:100])),
                                      loss_last100=float(np.mean(losses[-100:])), smoother=sm))
