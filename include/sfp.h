@@ -21,8 +21,8 @@
 extern "C" {
 #endif
 
-#define SFP_ABI_VERSION 3  /* 2: sfp_params gained the learned cycle's smoother (net_*);
-                              3: and magnetic_bc */
+#define SFP_ABI_VERSION 4  /* 2: sfp_params gained the learned cycle's smoother (net_*);
+                              3: magnetic_bc; 4: the pressure system's coarse smoother */
 
 typedef enum sfp_status {
     SFP_OK = 0,
@@ -66,6 +66,10 @@ typedef struct sfp_params {
     int net_coarse_cheby_degree;   /* 0 or 1..8 */
     double net_coarse_cheby_ratio; /* 0 or > 1 */
     int magnetic_bc;               /* sfp_magnetic_bc */
+    /* The pressure system's Chebyshev smoother on its coarse levels (0: the
+       coarse_cheby_* above, which the magnetic system uses). */
+    int pressure_coarse_cheby_degree;   /* 0 or 1..8 (default 3) */
+    double pressure_coarse_cheby_ratio; /* 0 or > 1 (default 4) */
 } sfp_params;
 
 /* Boundary condition of the magnetic potential on the box faces. */

@@ -41,6 +41,8 @@ sfp_params to_params(const SolverConfig &c) {
     p.net_coarse_cheby_degree = c.net_coarse_cheby_degree;
     p.net_coarse_cheby_ratio = c.net_coarse_cheby_ratio;
     p.magnetic_bc = c.magnetic_bc;
+    p.pressure_coarse_cheby_degree = c.pressure_coarse_cheby_degree;
+    p.pressure_coarse_cheby_ratio = c.pressure_coarse_cheby_ratio;
     return p;
 }
 
@@ -144,7 +146,7 @@ void pressure_dev(sfp_context ctx, double *const vel[3], const double *phi,
             fill_stats(st, SolveInfo{0, 0.0, true, 0, ctx->cfg.precond}, 0, 0.f, 0.f, 0.f);
             return;
         }
-        sys.build_hierarchy(ctx->ws, ctx->cfg);
+        sys.build_hierarchy(ctx->ws, ctx->cfg.for_pressure());
         ctx->pressure_unknowns = unknowns;
         ctx->pressure_ready = true;
     } else {
@@ -158,7 +160,7 @@ void pressure_dev(sfp_context ctx, double *const vel[3], const double *phi,
     SFP_HIP(hipEventRecord(ctx->ev[1], s));
     ctx->ws.phi = phi;
     ctx->ws.spacing = ctx->spacing;
-    SolveInfo info = sys.solve(ctx->ws, ctx->cfg, reproject);
+    SolveInfo info = sys.solve(ctx->ws, ctx->cfg.for_pressure(), reproject);
     SFP_HIP(hipEventRecord(ctx->ev[2], s));
     MutFaces Vm{{vel[0], vel[1], vel[2]}};
     launch_pressure_apply(s, d, phi, F, jump ? &J : nullptr, ctx->liquid.p, sys.x().p, Vm,
@@ -529,7 +531,9 @@ sfp_status sfp_set_params(sfp_context ctx, const sfp_params *p) {
         p->net_cheby_degree < 0 || p->net_cheby_degree > 8 || p->net_coarse_cheby_degree < 0 ||
         p->net_coarse_cheby_degree > 8 || !(p->net_cheby_ratio == 0 || p->net_cheby_ratio > 1) ||
         !(p->net_coarse_cheby_ratio == 0 || p->net_coarse_cheby_ratio > 1) ||
-        (p->magnetic_bc != SFP_MAGNETIC_BC_REFERENCE && p->magnetic_bc != SFP_MAGNETIC_BC_DIRICHLET_BOX)) {
+        (p->magnetic_bc != SFP_MAGNETIC_BC_REFERENCE && p->magnetic_bc != SFP_MAGNETIC_BC_DIRICHLET_BOX) ||
+        p->pressure_coarse_cheby_degree < 0 || p->pressure_coarse_cheby_degree > 8 ||
+        !(p->pressure_coarse_cheby_ratio == 0 || p->pressure_coarse_cheby_ratio > 1)) {
         ctx->last_error = "parameter out of range";
         return SFP_ERR_ARGUMENT;
     }
@@ -554,6 +558,8 @@ sfp_status sfp_set_params(sfp_context ctx, const sfp_params *p) {
     c.net_coarse_cheby_degree = p->net_coarse_cheby_degree;
     c.net_coarse_cheby_ratio = p->net_coarse_cheby_ratio;
     c.magnetic_bc = p->magnetic_bc;
+    c.pressure_coarse_cheby_degree = p->pressure_coarse_cheby_degree;
+    c.pressure_coarse_cheby_ratio = p->pressure_coarse_cheby_ratio;
     return SFP_OK;
 }
 

@@ -30,6 +30,10 @@ sfp_status run(sfp_context ctx, F &&f) {
     }
 }
 
+// The configuration the system `which` solves with (the pressure system's
+// own coarse smoother).
+SolverConfig cfg_of(sfp_context ctx, int which) { return which == 0 ? ctx->cfg.for_pressure() : ctx->cfg; }
+
 System &pick(sfp_context ctx, int which) {
     if (which == 0) {
         if (!ctx->pressure_ready || ctx->pressure_unknowns == 0)
@@ -130,7 +134,7 @@ sfp_status sfp_test_precondition(sfp_context ctx, int which, const double *r, fl
         DevBuf<double> dr;
         dr.ensure(n);
         SFP_HIP(hipMemcpyAsync(dr.p, r, n * sizeof(double), hipMemcpyHostToDevice, ctx->ws.stream));
-        const float *dz = s.precondition(ctx->ws, ctx->cfg, dr.p);
+        const float *dz = s.precondition(ctx->ws, cfg_of(ctx, which), dr.p);
         d2h(ctx, z, dz, n);
         SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
     });
@@ -172,7 +176,8 @@ sfp_status sfp_test_solve(sfp_context ctx, int which, int level, const double *r
         SFP_HIP(hipMemcpyAsync(s.rhs().p, rhs, nl * sizeof(double), hipMemcpyHostToDevice, st));
         SolveInfo info;
         try {
-            info = level == 0 ? s.solve(ctx->ws, ctx->cfg, false) : s.solve_level(ctx->ws, ctx->cfg, level);
+            info = level == 0 ? s.solve(ctx->ws, cfg_of(ctx, which), false)
+                               : s.solve_level(ctx->ws, cfg_of(ctx, which), level);
         } catch (...) {
             (void)hipMemcpyAsync(s.rhs().p, keep_b.p, n * sizeof(double), hipMemcpyDeviceToDevice, st);
             (void)hipMemcpyAsync(s.x().p, keep_x.p, n * sizeof(double), hipMemcpyDeviceToDevice, st);
@@ -204,7 +209,7 @@ sfp_status sfp_test_restrict_smoothed(sfp_context ctx, int which, const double *
         DevBuf<double> dr;
         dr.ensure(n);
         SFP_HIP(hipMemcpyAsync(dr.p, r, n * sizeof(double), hipMemcpyHostToDevice, ctx->ws.stream));
-        const float *b = s.restrict_smoothed(ctx->ws, ctx->cfg, dr.p);
+        const float *b = s.restrict_smoothed(ctx->ws, cfg_of(ctx, which), dr.p);
         d2h(ctx, rc, b, s.level(1).d.cells());
         SFP_HIP(hipStreamSynchronize(ctx->ws.stream));
     });

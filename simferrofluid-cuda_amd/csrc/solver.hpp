@@ -34,6 +34,17 @@ struct SolverConfig {
     // magnetic box condition: 0 the reference's (Neumann box faces, cell 0
     // pinned), 1 phi = 0 on the box faces
     int magnetic_bc = 0;
+    // the pressure system's coarse-level smoother (0: coarse_cheby_*): its
+    // iteration counts do not gain from the magnetic system's stronger one
+    int pressure_coarse_cheby_degree = 3;
+    double pressure_coarse_cheby_ratio = 4.0;
+    // this configuration as the pressure system runs it
+    SolverConfig for_pressure() const {
+        SolverConfig c = *this;
+        if (pressure_coarse_cheby_degree > 0) c.coarse_cheby_degree = pressure_coarse_cheby_degree;
+        if (pressure_coarse_cheby_ratio > 0) c.coarse_cheby_ratio = pressure_coarse_cheby_ratio;
+        return c;
+    }
 };
 
 // Chebyshev recurrence coefficients for D^{-1}A on [lmax/ratio, lmax].

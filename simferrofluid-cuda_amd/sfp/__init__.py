@@ -61,7 +61,8 @@ class Params(C.Structure):
                 ("coarse_cheby_degree", C.c_int), ("coarse_cheby_ratio", C.c_double),
                 ("net_cheby_degree", C.c_int), ("net_cheby_ratio", C.c_double),
                 ("net_coarse_cheby_degree", C.c_int), ("net_coarse_cheby_ratio", C.c_double),
-                ("magnetic_bc", C.c_int)]
+                ("magnetic_bc", C.c_int), ("pressure_coarse_cheby_degree", C.c_int),
+                ("pressure_coarse_cheby_ratio", C.c_double)]
 
 
 class Stats(C.Structure):

@@ -48,10 +48,11 @@ def test_struct_layouts_match_c():
 #include "sfp.h"
 #include "sfp_profile.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(sfp_params), offsetof(sfp_params, max_iters),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(sfp_params), offsetof(sfp_params, max_iters),
          offsetof(sfp_params, cheby_ratio), offsetof(sfp_params, stall_ratio), offsetof(sfp_params, verbose),
          offsetof(sfp_params, coarse_cheby_ratio), offsetof(sfp_params, net_cheby_ratio),
          offsetof(sfp_params, net_coarse_cheby_ratio), offsetof(sfp_params, magnetic_bc),
+         offsetof(sfp_params, pressure_coarse_cheby_ratio),
          sizeof(sfp_stats), offsetof(sfp_stats, rel_residual), offsetof(sfp_stats, unknowns),
          sizeof(sfp_kernel_profile), offsetof(sfp_kernel_profile, bytes));
   return 0;
@@ -67,7 +68,8 @@ int main(void) {
     P, S, K = sfp.Params, sfp.Stats, sfp.KernelProfile
     assert vals == [C.sizeof(P), P.max_iters.offset, P.cheby_ratio.offset, P.stall_ratio.offset,
                     P.verbose.offset, P.coarse_cheby_ratio.offset, P.net_cheby_ratio.offset,
-                    P.net_coarse_cheby_ratio.offset, P.magnetic_bc.offset, C.sizeof(S), S.rel_residual.offset,
+                    P.net_coarse_cheby_ratio.offset, P.magnetic_bc.offset, P.pressure_coarse_cheby_ratio.offset,
+                    C.sizeof(S), S.rel_residual.offset,
                     S.unknowns.offset,
                     C.sizeof(K), K.bytes.offset]
 
@@ -149,6 +151,8 @@ def test_trained_weights_load_and_checksum():
     p = sfp.default_params()
     assert p.net_cheby_degree == 0 and p.net_coarse_cheby_ratio == 0.0
     assert p.magnetic_bc == sfp.MAGNETIC_BC_REFERENCE
+    assert (p.pressure_coarse_cheby_degree, p.pressure_coarse_cheby_ratio) == (3, 4.0)
+    assert (p.coarse_cheby_degree, p.coarse_cheby_ratio) == (5, 6.0)
 
 
 def test_apply_full_matches_csr():
