@@ -72,8 +72,11 @@ __device__ __forceinline__ void block_partials(double (&v)[N], double *partials)
 // in thread 0. Each thread keeps U running sums so U loads per sum are in
 // flight at a time (tens of thousands of partials in a few microseconds);
 // the order is fixed by nb and T.
+// (stride: the distance between the N arrays, nb when 0)
 template <int N, int T = kBlock, unsigned U = 8>
-__device__ __forceinline__ void sum_partials(const double *partials, unsigned nb, double (&v)[N]) {
+__device__ __forceinline__ void sum_partials(const double *partials, unsigned nb, double (&v)[N],
+                                             size_t stride = 0) {
+    if (stride == 0) stride = nb;
     static_assert(U == 8 || U == 16, "U is 8 or 16");
     double acc[N][U];
 #pragma unroll
@@ -85,11 +88,11 @@ __device__ __forceinline__ void sum_partials(const double *partials, unsigned nb
 #pragma unroll
         for (unsigned u = 0; u < U; u++)
 #pragma unroll
-            for (int q = 0; q < N; q++) acc[q][u] += partials[(size_t)q * nb + b + u * T];
+            for (int q = 0; q < N; q++) acc[q][u] += partials[(size_t)q * stride + b + u * T];
     }
     for (; b < nb; b += T)
 #pragma unroll
-        for (int q = 0; q < N; q++) acc[q][0] += partials[(size_t)q * nb + b];
+        for (int q = 0; q < N; q++) acc[q][0] += partials[(size_t)q * stride + b];
 #pragma unroll
     for (int q = 0; q < N; q++) {
         if constexpr (U == 16) {

@@ -514,8 +514,50 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int op, unsigned 
     }
 }
 
-void finalize(hipStream_t st, int op, unsigned nb, int first, const double *partials,
-              KrylovScalars *sc, const double *gate = nullptr) {
+// Above kStageMin partials the sums take two launches: kStageThreads-wide
+// blocks add fixed chunks of kStageChunk partials each (every block one
+// round of loads instead of the single block's many), the finalize adds the
+// chunks' sums. The order is fixed by nb, so results stay deterministic.
+constexpr unsigned kStageMin = 4096, kStageChunk = 1024, kStageMaxBlocks = 64;
+constexpr int kStageThreads = 1024;
+
+template <int N>
+__global__ __launch_bounds__(kStageThreads) void k_partials_stage(unsigned nb, unsigned chunk,
+                                                                  const double *__restrict__ partials,
+                                                                  double *__restrict__ out,
+                                                                  const double *gate) {
+    if (gated(gate)) return;
+    const unsigned b0 = blockIdx.x * chunk;  // < nb: the launcher sizes the grid
+    const unsigned n = min(chunk, nb - b0);
+    double v[N];
+    sum_partials<N, kStageThreads, 8>(partials + b0, n, v, nb);
+    if (flat_tid() == 0)
+#pragma unroll
+        for (int q = 0; q < N; q++) out[(size_t)q * gridDim.x + blockIdx.x] = v[q];
+}
+
+// `partials` holds the N sums of op for nb blocks ([q * nb + b]); the stage
+// writes its sums after them (reduction_partials leaves room).
+void finalize(hipStream_t st, int op, unsigned nb, int first, double *partials, KrylovScalars *sc,
+              const double *gate = nullptr) {
+    const int N = op == FIN_MATVEC ? 3 : op == FIN_RESIDUAL ? 2 : 1;
+    if (nb > kStageMin) {
+        const unsigned chunk = std::max(kStageChunk, (nb + kStageMaxBlocks - 1) / kStageMaxBlocks);
+        const unsigned G = (nb + chunk - 1) / chunk;
+        double *out = partials + (size_t)N * nb;
+        if (N == 3)
+            hipLaunchKernelGGL(k_partials_stage<3>, dim3(G), dim3(kStageThreads), 0, st, nb, chunk, partials, out,
+                               gate);
+        else if (N == 2)
+            hipLaunchKernelGGL(k_partials_stage<2>, dim3(G), dim3(kStageThreads), 0, st, nb, chunk, partials, out,
+                               gate);
+        else
+            hipLaunchKernelGGL(k_partials_stage<1>, dim3(G), dim3(kStageThreads), 0, st, nb, chunk, partials, out,
+                               gate);
+        SFP_LAUNCH_CHECK();
+        partials = out;
+        nb = G;
+    }
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, st, op, nb, first, partials,
                        sc, gate);
     SFP_LAUNCH_CHECK();
@@ -536,8 +578,10 @@ static unsigned matvec_blocks(Dims d) {
 static unsigned update_blocks(Dims d) { return (unsigned)((d.cells() + kUpdateCells - 1) / kUpdateCells); }
 
 size_t reduction_partials(Dims d) {
+    // The largest set of partials, then room for the two-stage finalize's chunk sums.
     return std::max({3 * (size_t)cell_launch(d, kReducePlanes).blocks(),
-                     3 * (size_t)matvec_blocks(d), (size_t)update_blocks(d)});
+                     3 * (size_t)matvec_blocks(d), (size_t)update_blocks(d)}) +
+           3 * (size_t)kStageMaxBlocks;
 }
 
 void launch_count_active(hipStream_t st, Dims d, const float *diag32, double *partials,
