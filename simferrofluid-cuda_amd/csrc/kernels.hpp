@@ -121,7 +121,8 @@ size_t reduction_partials(Dims d);
 // ->stop_rr = stop_rr, ->stall_rr = stall_rr (0: no stall test),
 // ->stalled = 0.
 void launch_batch_begin(hipStream_t s, KrylovScalars *sc, double stop_rr, double stall_rr = 0.0);
-// r = b - A x (x may be null = 0); scalars->rr = r.r, scalars->bb = b.b.
+// r = b - A x (x may be null = 0); scalars->rr = r.r, scalars->bb = b.b;
+// r may be null (only the sums).
 // With b32, also b32 = r rounded to FP32 and, with x0, x0 = c0 D32^-1 b32
 // (diag32: the FP32 diagonal): the right-hand side and first Chebyshev
 // iterate of a V-cycle on r, as the FCG update stores them.

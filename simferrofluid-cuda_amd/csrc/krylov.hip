@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_residual64(CellGrid cg, Dims d, Op64
         if (on)
             for (int i = i0; i < ie; i++, c += s.sx) {
                 const double bc = b[c];
-                r[c] = bc;
+                if (r) r[c] = bc;
                 store32(c, bc);
                 v[0] += bc * bc;
                 v[1] += bc * bc;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_residual64(CellGrid cg, Dims d, Op64
             if (zhi) off += czp * xzp;
             const double bc = b[c];
             const double rc = bc - (A.diag[c] * xc - off);
-            r[c] = rc;
+            if (r) r[c] = rc;
             store32(c, rc);
             v[0] += rc * rc;
             v[1] += bc * bc;
@@ -106,6 +106,100 @@ __global__ __launch_bounds__(kBlock) void k_residual64(CellGrid cg, Dims d, Op64
         xm = xc;
         xc = xp;
         cxc = cxp;
+    }
+    block_partials<2>(v, partials);
+}
+
+// The same residual over the flat cell index, two consecutive cells of one z
+// row per lane (grids with nz even), the mapping of k_fcg_matvec_flat: 16-byte
+// loads of the FP64 operator and vectors at the pair and at its x and y
+// neighbour pairs, one extra x value and z coupling at each end. Each cell's
+// off-diagonal sum keeps k_residual64's order, so r is the same to the bit; a
+// row-sum diagonal is rebuilt from the couplings as in the matvec. r, and b32
+// with x0, may be null (then not stored: a convergence check needs only the
+// sums).
+constexpr unsigned kResidualCells = 2 * kBlock;
+
+__global__ __launch_bounds__(kBlock) void k_residual64_flat(Dims d, Op64 A,
+                                                            const double *__restrict__ b,
+                                                            const double *__restrict__ x,
+                                                            double *__restrict__ r,
+                                                            double *__restrict__ partials,
+                                                            float *__restrict__ b32,
+                                                            float *__restrict__ x0,
+                                                            const float *__restrict__ diag32,
+                                                            float c0) {
+    const Strides s(d);
+    const size_t c = (size_t)blockIdx.x * kResidualCells + 2 * (size_t)flat_tid();
+    double v[2] = {0.0, 0.0};  // r.r, b.b
+    if (c < d.cells()) {       // cells() is even: then c + 1 is a cell too
+        const double2v d0 = {0.0, 0.0};
+        const double2v bc = ld_stream((const double2v *)(b + c));
+        double2v rc = bc;
+        if (x) {
+            const unsigned cu = (unsigned)c;
+            const unsigned t = cu / (unsigned)d.nz;
+            const int k = (int)(cu - t * (unsigned)d.nz);
+            const unsigned iu = t / (unsigned)d.ny;
+            const int i = (int)iu, j = (int)(t - iu * (unsigned)d.ny);
+            const bool xl = i > 0, xh = i + 1 < d.nx, yl = j > 0, yh = j + 1 < d.ny;
+            const bool zl = k > 0, zh = k + 2 < d.nz;
+            const double2v xc = *(const double2v *)(x + c);
+            const double2v xxm = xl ? *(const double2v *)(x + c - s.sx) : d0;
+            const double2v xxp = xh ? *(const double2v *)(x + c + s.sx) : d0;
+            const double2v xym = yl ? *(const double2v *)(x + c - s.sy) : d0;
+            const double2v xyp = yh ? *(const double2v *)(x + c + s.sy) : d0;
+            const double xzm = zl ? x[c - 1] : 0.0;
+            const double xzp = zh ? x[c + 2] : 0.0;
+            const double2v cxm = xl ? *(const double2v *)(A.cpl[0] + c) : d0;
+            const double2v cxp = xh ? *(const double2v *)(A.cpl[0] + c + s.sx) : d0;
+            const double2v cym = yl ? *(const double2v *)(A.cpl[1] + c) : d0;
+            const double2v cyp = yh ? *(const double2v *)(A.cpl[1] + c + s.sy) : d0;
+            const double2v cz = *(const double2v *)(A.cpl[2] + c);
+            const double czp = zh ? A.cpl[2][c + 2] : 0.0;
+            double2v dg;
+            if (A.pinned_rowsum && cu != 0 && cu != (unsigned)d.nz && cu != (unsigned)s.sx) {
+                dg.x = ((((cxm.x + cxp.x) + cym.x) + cyp.x) + cz.x) + cz.y;
+                dg.y = ((((cxm.y + cxp.y) + cym.y) + cyp.y) + cz.y) + czp;
+            } else {
+                dg = *(const double2v *)(A.diag + c);
+            }
+            double o0 = 0.0, o1 = 0.0;
+            if (xl) {
+                o0 += cxm.x * xxm.x;
+                o1 += cxm.y * xxm.y;
+            }
+            if (xh) {
+                o0 += cxp.x * xxp.x;
+                o1 += cxp.y * xxp.y;
+            }
+            if (yl) {
+                o0 += cym.x * xym.x;
+                o1 += cym.y * xym.y;
+            }
+            if (yh) {
+                o0 += cyp.x * xyp.x;
+                o1 += cyp.y * xyp.y;
+            }
+            if (zl) o0 += cz.x * xzm;
+            o0 += cz.y * xc.y;
+            o1 += cz.y * xc.x;
+            if (zh) o1 += czp * xzp;
+            rc.x = bc.x - (dg.x * xc.x - o0);
+            rc.y = bc.y - (dg.y * xc.y - o1);
+        }
+        if (r) st_stream((double2v *)(r + c), rc);
+        if (b32) {
+            const float2v bf = {(float)rc.x, (float)rc.y};
+            *(float2v *)(b32 + c) = bf;
+            if (x0) {
+                const float2v dd = *(const float2v *)(diag32 + c);
+                const float2v xf = {c0 * rcp_or_zero(dd.x) * bf.x, c0 * rcp_or_zero(dd.y) * bf.y};
+                *(float2v *)(x0 + c) = xf;
+            }
+        }
+        v[0] = rc.x * rc.x + rc.y * rc.y;
+        v[1] = bc.x * bc.x + bc.y * bc.y;
     }
     block_partials<2>(v, partials);
 }
@@ -601,6 +695,19 @@ void launch_residual64(hipStream_t st, Dims d, Op64 A, const double *b, const do
                        double *r, double *partials, KrylovScalars *sc, float *b32, float *x0,
                        const float *diag32, float c0) {
     if (x0 && (!b32 || !diag32)) throw ArgError("launch_residual64: x0 needs b32 and the diagonal");
+    if (matvec_flat(d)) {
+        for (const void *a : {(const void *)A.diag, (const void *)A.cpl[0], (const void *)A.cpl[1],
+                              (const void *)A.cpl[2], (const void *)b, (const void *)x, (const void *)r})
+            if ((uintptr_t)a % 16) throw ArgError("launch_residual64: FP64 array not 16-byte aligned");
+        if ((uintptr_t)b32 % 8 || (uintptr_t)x0 % 8 || (uintptr_t)diag32 % 8)
+            throw ArgError("launch_residual64: b32, x0 or the diagonal not 8-byte aligned");
+        const unsigned nb = (unsigned)((d.cells() + kResidualCells - 1) / kResidualCells);
+        hipLaunchKernelGGL(k_residual64_flat, dim3(nb), dim3(kBlock), 0, st, d, A, b, x, r, partials, b32,
+                           x0, diag32, c0);
+        SFP_LAUNCH_CHECK();
+        finalize(st, FIN_RESIDUAL, nb, 0, partials, sc);
+        return;
+    }
     Launch L = cell_launch(d, kReducePlanes);
     hipLaunchKernelGGL(k_residual64, L.grid, L.block, 0, st, L.cg, d, A, b, x, r, partials, b32, x0,
                        diag32, c0);

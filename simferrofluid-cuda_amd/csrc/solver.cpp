@@ -475,16 +475,20 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     while (true) {
         if (rel <= cfg.tol) {
             if (checked) break;
-            // recursive residual reached tol: recompute it from x in FP64 and
-            // restart from the true residual if the two have drifted apart
+            // recursive residual reached tol: recompute its norm from x in
+            // FP64 (the sums only), and restart from the true residual if the
+            // two have drifted apart
             bring_x_up_to_date();
-            launch_residual64(st, d, A, rhs_.p, x_.p, r_.p, ws.partials.p,
-                              ws.scalars.p, b32, x0, lv_[level].diag32.p, c0);
+            launch_residual64(st, d, A, rhs_.p, x_.p, nullptr, ws.partials.p, ws.scalars.p);
             ws.read_scalars();
             rel = std::sqrt(ws.host_sc->rr / bb);
             checked = true;
-            first = true;
-            b_ready = b32 != nullptr;
+            if (rel > cfg.tol) {
+                launch_residual64(st, d, A, rhs_.p, x_.p, r_.p, ws.partials.p,
+                                  ws.scalars.p, b32, x0, lv_[level].diag32.p, c0);
+                first = true;
+                b_ready = b32 != nullptr;
+            }
             continue;
         }
         if (info.iterations >= cfg.max_iters) break;
@@ -577,7 +581,7 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     }
     bring_x_up_to_date();
     if (!checked) {
-        launch_residual64(st, d, A, rhs_.p, x_.p, r_.p, ws.partials.p, ws.scalars.p);
+        launch_residual64(st, d, A, rhs_.p, x_.p, nullptr, ws.partials.p, ws.scalars.p);
         ws.read_scalars();
         rel = std::sqrt(ws.host_sc->rr / bb);
     }
