@@ -1196,46 +1196,155 @@ __global__ __launch_bounds__(kBlock) void k_coarse_solve(int n, const float *ain
 // ---------------------------------------------------------------- V-cycle tail
 // One block runs the coarse end of the V-cycle (levels of at most
 // kTailMaxCells cells) with the levels' arrays in LDS: each sweep covers a
-// level's cells with the block's threads and ends at a barrier. The per-cell arithmetic is that of k_smooth,
-// k_restrict and k_coarse_solve (neighbour values equal to what those take
-// from adjacent lanes), so the result is the same bit for bit.
+// level's cells with the block's threads and ends at a barrier. The per-cell
+// arithmetic is that of k_smooth, k_restrict and k_coarse_solve (neighbour
+// values equal to what those take from adjacent lanes), so the result is the
+// same bit for bit.
+//
+// The block is one CU, so its sweeps are bound by instructions and LDS
+// bandwidth: a step takes a group of four consecutive cells of a z row per
+// thread with 16-byte LDS reads (the index arithmetic once per group), the
+// zero-start first iterate c0 D^-1 b is formed once per cell in a sweep of
+// its own, the restriction gives each of a coarse cell's eight children to a
+// lane of an aligned group of eight (the children's residuals added across
+// those lanes in k_restrict's order), and the operators come in with all of
+// their loads in flight at once.
 constexpr int kTailThreads = 1024;
 constexpr int kTailLdsFloats = (int)(kTailLdsBytes / sizeof(float));
 
+// x = c0 D^-1 b on every cell of the level (k_smooth's SM_FIRST iterate).
+__device__ void tail_first(const TailLevel &L, float *__restrict__ x, float c0) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int n = (int)L.d.cells();
+    if ((n & 3) == 0) {
+        for (int g = threadIdx.x; 4 * g < n; g += kTailThreads) {
+            const f4 dg = *(const f4 *)(L.A.diag + 4 * g), bv = *(const f4 *)(L.b + 4 * g);
+            f4 xv;
+#pragma unroll
+            for (int q = 0; q < 4; q++) xv[q] = c0 * rcp_or_zero(dg[q]) * bv[q];
+            *(f4 *)(x + 4 * g) = xv;
+        }
+    } else {
+        for (int q = threadIdx.x; q < n; q += kTailThreads) x[q] = c0 * rcp_or_zero(L.A.diag[q]) * L.b[q];
+    }
+    __syncthreads();
+}
+
+// One Chebyshev step (modes without SM_FIRST: the first iterate is stored by
+// tail_first and read with SM_XFIRST). Levels whose z rows hold whole groups
+// of four cells take k_smooth_flat's mapping: a thread a group, 16-byte LDS
+// reads of the group and of its x and y neighbour groups (a wavefront's
+// reads then touch consecutive banks), one value at each z end. Other levels
+// take one cell per thread.
 template <int MODE>
 __device__ void tail_smooth(const TailLevel &L, const float *__restrict__ xin, float *__restrict__ xout,
-                            float c0, float cd, float cr, const float *__restrict__ e, Dims dc) {
-    constexpr bool kFirst = (MODE & SM_FIRST) != 0;
+                            float cd, float cr, const float *__restrict__ e, Dims dc) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    static_assert((MODE & SM_FIRST) == 0, "tail step: the first iterate comes from tail_first");
     constexpr bool kProlong = (MODE & SM_PROLONG) != 0;
+    constexpr bool kXFirst = (MODE & SM_XFIRST) != 0;
     const Dims d = L.d;
     const Op32 A = L.A;
     const float *__restrict__ b = L.b;
-    const Strides s(d);
-    const int n = (int)d.cells(), nyz = d.ny * d.nz;
-    auto xval = [&](size_t q, int qi, int qj, int qk) -> float {
-        if constexpr (kFirst) {
-            return c0 * rcp_or_zero(A.diag[q]) * b[q];
-        } else if constexpr (kProlong) {
-            return xin[q] + e[dc.idx(qi >> 1, qj >> 1, qk >> 1)];
-        } else {
-            return xin[q];
+    const int sx = d.ny * d.nz, sy = d.nz;
+    const int n = (int)d.cells();
+    if ((d.nz & 3) == 0) {
+        const f4 z4 = {0.f, 0.f, 0.f, 0.f};
+        for (int g = threadIdx.x; 4 * g < n; g += kTailThreads) {
+            const int c = 4 * g;
+            const int t = c / d.nz, k = c - t * d.nz;
+            const int i = t / d.ny, j = t - i * d.ny;
+            const bool up = i + 1 < d.nx, ylo = j > 0, yhi = j + 1 < d.ny;
+            const bool zl = k > 0, zh = k + 4 < d.nz;  // neighbours below the first, above the last
+            const f4 dg = *(const f4 *)(A.diag + c);
+            f4 xc = *(const f4 *)(xin + c);
+            f4 xm = i > 0 ? *(const f4 *)(xin + c - sx) : z4;
+            f4 xp = up ? *(const f4 *)(xin + c + sx) : z4;
+            const f4 cxc = *(const f4 *)(A.cpl[0] + c);
+            const f4 cxp = up ? *(const f4 *)(A.cpl[0] + c + sx) : z4;
+            const f4 cym = ylo ? *(const f4 *)(A.cpl[1] + c) : z4;
+            f4 xym = ylo ? *(const f4 *)(xin + c - sy) : z4;
+            const f4 cyp = yhi ? *(const f4 *)(A.cpl[1] + c + sy) : z4;
+            f4 xyp = yhi ? *(const f4 *)(xin + c + sy) : z4;
+            const f4 cz = *(const f4 *)(A.cpl[2] + c);
+            float xzlo = zl ? xin[c - 1] : 0.f;
+            float xzhi = zh ? xin[c + 4] : 0.f;
+            const float czhi = zh ? A.cpl[2][c + 4] : 0.f;
+            const f4 bc = *(const f4 *)(b + c);
+            f4 dprev = z4;
+            if constexpr ((MODE & SM_READ_D) != 0) dprev = *(const f4 *)(L.dv + c);
+            const f4 xin_c = xc;
+            if constexpr (kProlong) {
+                // the group's two coarse cells in this row and in the
+                // neighbouring rows and planes (k >> 1 even: 8-byte aligned)
+                const int kc = k >> 1;
+                auto e2 = [&](int ii, int jj) {
+                    return *(const f2 *)(e + ((ii >> 1) * dc.ny + (jj >> 1)) * dc.nz + kc);
+                };
+                const f2 ec = e2(i, j);
+                const f2 e_xm = i > 0 ? e2(i - 1, j) : f2{0.f, 0.f};
+                const f2 e_xp = up ? e2(i + 1, j) : f2{0.f, 0.f};
+                const f2 e_ym = ylo ? e2(i, j - 1) : f2{0.f, 0.f};
+                const f2 e_yp = yhi ? e2(i, j + 1) : f2{0.f, 0.f};
+                const int er = ((i >> 1) * dc.ny + (j >> 1)) * dc.nz;
+                const float ezlo = zl ? e[er + kc - 1] : 0.f;
+                const float ezhi = zh ? e[er + kc + 2] : 0.f;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    xc[q] = xc[q] + ec[q >> 1];
+                    xm[q] = xm[q] + e_xm[q >> 1];
+                    xp[q] = xp[q] + e_xp[q >> 1];
+                    xym[q] = xym[q] + e_ym[q >> 1];
+                    xyp[q] = xyp[q] + e_yp[q >> 1];
+                }
+                xzlo = xzlo + ezlo;
+                xzhi = xzhi + ezhi;
+            }
+            f4 xo, dno;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float xzm = q > 0 ? xc[q - 1] : xzlo;
+                const float xzp = q < 3 ? xc[q + 1] : xzhi;
+                const float czp = q < 3 ? cz[q + 1] : czhi;
+                float off = fmaf(cxc[q], xm[q], cxp[q] * xp[q]);
+                if (ylo) off = fmaf(cym[q], xym[q], off);
+                if (yhi) off = fmaf(cyp[q], xyp[q], off);
+                if (q > 0 || zl) off = fmaf(cz[q], xzm, off);
+                if (q < 3 || zh) off = fmaf(czp, xzp, off);
+                // centre iterate: the coarse correction only lands on active cells
+                const float x0 = (kProlong && !(dg[q] > 0.f)) ? xin_c[q] : xc[q];
+                const float r = bc[q] - fmaf(dg[q], x0, -off);
+                const float dp = kXFirst ? x0 : dprev[q];
+                const float dn = fmaf(cd, dp, cr * rcp_or_zero(dg[q]) * r);
+                xo[q] = x0 + dn;
+                dno[q] = dn;
+            }
+            *(f4 *)(xout + c) = xo;
+            if constexpr ((MODE & SM_WRITE_D) != 0) *(f4 *)(L.dv + c) = dno;
         }
+        __syncthreads();
+        return;
+    }
+    const int nyz = d.ny * d.nz;
+    auto xval = [&](int q, int qi, int qj, int qk) -> float {
+        if constexpr (kProlong) return xin[q] + e[dc.idx(qi >> 1, qj >> 1, qk >> 1)];
+        else return xin[q];
     };
-    for (int t = threadIdx.x; t < n; t += kTailThreads) {
-        const int i = t / nyz, rem = t - i * nyz, j = rem / d.nz, k = rem - j * d.nz;
-        const size_t c = (size_t)t;
+    for (int c = threadIdx.x; c < n; c += kTailThreads) {
+        const int i = c / nyz, rem = c - i * nyz, j = rem / d.nz, k = rem - j * d.nz;
         const float dg = A.diag[c];
         const bool up = i + 1 < d.nx, zlo = k > 0, zhi = k + 1 < d.nz;
-        const float xm = i > 0 ? xval(c - s.sx, i - 1, j, k) : 0.f;
+        const float xm = i > 0 ? xval(c - sx, i - 1, j, k) : 0.f;
         const float xc = xval(c, i, j, k);
         const float cxc = A.cpl[0][c];
-        const float xp = up ? xval(c + s.sx, i + 1, j, k) : 0.f;
-        const float cxp = up ? A.cpl[0][c + s.sx] : 0.f;
+        const float xp = up ? xval(c + sx, i + 1, j, k) : 0.f;
+        const float cxp = up ? A.cpl[0][c + sx] : 0.f;
         const float czm = A.cpl[2][c];
         const float cym = j > 0 ? A.cpl[1][c] : 0.f;
-        const float xym = j > 0 ? xval(c - s.sy, i, j - 1, k) : 0.f;
-        const float cyp = j + 1 < d.ny ? A.cpl[1][c + s.sy] : 0.f;
-        const float xyp = j + 1 < d.ny ? xval(c + s.sy, i, j + 1, k) : 0.f;
+        const float xym = j > 0 ? xval(c - sy, i, j - 1, k) : 0.f;
+        const float cyp = j + 1 < d.ny ? A.cpl[1][c + sy] : 0.f;
+        const float xyp = j + 1 < d.ny ? xval(c + sy, i, j + 1, k) : 0.f;
         const float bc = b[c];
         float dprev = 0.f;
         if constexpr ((MODE & SM_READ_D) != 0) dprev = L.dv[c];
@@ -1252,7 +1361,7 @@ __device__ void tail_smooth(const TailLevel &L, const float *__restrict__ xin, f
         if (zlo) off = fmaf(czm, xzm, off);
         if (zhi) off = fmaf(czp, xzp, off);
         const float r = bc - fmaf(dg, x0, -off);
-        if constexpr (kFirst) dprev = x0;
+        if constexpr (kXFirst) dprev = x0;
         const float dn = fmaf(cd, dprev, cr * rcp_or_zero(dg) * r);
         xout[c] = x0 + dn;
         if constexpr ((MODE & SM_WRITE_D) != 0) L.dv[c] = dn;
@@ -1260,17 +1369,19 @@ __device__ void tail_smooth(const TailLevel &L, const float *__restrict__ xin, f
     __syncthreads();
 }
 
-// bc = R (b - A x): k_restrict's residual per child and its pairwise sums
-// (z pair, then y pair, then x pair onto 0).
+// bc = R (b - A x): k_restrict's residual per child and its pairwise sums (z
+// pair, then y pair, then x pair onto 0). Lane 8m + (4 di + 2 dj + dk) of a
+// wavefront takes child (di, dj, dk) of one coarse cell; the sums run across
+// those lanes (every lane of the group takes part in each exchange).
 __device__ void tail_restrict(const TailLevel &F, const float *__restrict__ x, float *__restrict__ bcv,
                               Dims dc) {
     const Dims d = F.d;
     const Op32 A = F.A;
     const Strides s(d);
-    const int nc = (int)dc.cells(), nyz = dc.ny * dc.nz;
+    const int nyz = dc.ny * dc.nz;
+    const int nw = 8 * (int)dc.cells();  // a multiple of 8: whole groups, one wavefront each
     // residual of a child (0 outside the grid or inactive): all loads from
-    // clamped addresses, then a select, so the 8 children's loads are in
-    // flight together
+    // clamped addresses, then a select
     auto res = [&](int iu, int ju, int ku) -> float {
         const bool in = iu < d.nx && ju < d.ny && ku < d.nz;
         const int i = min(iu, d.nx - 1), j = min(ju, d.ny - 1), k = min(ku, d.nz - 1);
@@ -1295,30 +1406,33 @@ __device__ void tail_restrict(const TailLevel &F, const float *__restrict__ x, f
         const float r = bcv - fmaf(dg, xc, -off);
         return (in && dg > 0.f) ? r : 0.f;
     };
-    for (int t = threadIdx.x; t < nc; t += kTailThreads) {
+    for (int w = threadIdx.x; w < nw; w += kTailThreads) {
+        const int t = w >> 3, ch = w & 7;
         const int I = t / nyz, rem = t - I * nyz, J = rem / dc.nz, K = rem - J * dc.nz;
-        float acc = 0.f;
-        for (int di = 0; di < 2 && 2 * I + di < d.nx; di++) {
-            const int i = 2 * I + di;
-            float zp[2];
-            for (int dj = 0; dj < 2; dj++) {
-                const int j = 2 * J + dj;
-                const float r0 = res(i, j, 2 * K);
-                zp[dj] = d.nz > 1 ? r0 + res(i, j, 2 * K + 1) : r0;
-            }
-            acc += zp[0] + zp[1];
+        const int di = ch >> 2, dj = (ch >> 1) & 1, dk = ch & 1;
+        const float r = res(2 * I + di, 2 * J + dj, 2 * K + dk);
+        const float rz = __shfl_xor(r, 1, 64);                  // the z pair's other child
+        const float zp = d.nz > 1 ? (dk ? rz + r : r + rz) : r;  // (r0 + r1 on both lanes)
+        const float zy = __shfl_xor(zp, 2, 64);
+        const float ay = dj ? zy + zp : zp + zy;  // zp[0] + zp[1]
+        const float ax = __shfl_xor(ay, 4, 64);   // the other x layer's sum
+        if (ch == 0) {
+            float acc = 0.f;
+            acc += ay;
+            if (2 * I + 1 < d.nx) acc += ax;
+            bcv[t] = acc;
         }
-        bcv[t] = acc;
     }
     __syncthreads();
 }
 
 __global__ __launch_bounds__(kTailThreads) void k_vcycle_tail(TailArgs t, const double *gate) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
     if (gated(gate)) return;
     // every array of the tail's levels lives in LDS for the whole cycle (8
     // per level, levels one after another): the operators and the first
     // level's b are copied in, the first level's result is copied out
-    __shared__ float lds[kTailLdsFloats];
+    __shared__ __attribute__((aligned(16))) float lds[kTailLdsFloats];
     auto level = [&](int l) -> TailLevel {
         size_t off = 0;
         for (int q = 0; q < l; q++) off += 8 * t.lv[q].d.cells();
@@ -1327,10 +1441,60 @@ __global__ __launch_bounds__(kTailThreads) void k_vcycle_tail(TailArgs t, const 
         return TailLevel{t.lv[l].d, Op32{p, {p + n, p + 2 * n, p + 3 * n}}, p + 4 * n, p + 5 * n,
                          p + 6 * n, p + 7 * n};
     };
+    const int tid = threadIdx.x;
+    const int last = t.nlev - 1;
+    const int nC = (int)t.lv[last].d.cells();
+    // The coarsest solve reads the dense inverse from the first level's
+    // direction array, which no sweep uses between that level's last
+    // pre-smoothing step and its first post-smoothing step (which writes it
+    // before reading): loaded here with the operators, stored there.
+    const bool ainv_lds = (int)t.lv[0].d.cells() >= nC * nC && (nC * nC) % 4 == 0 &&
+                          ((uintptr_t)t.ainv & 15) == 0;
+    f4 ainv4 = {0.f, 0.f, 0.f, 0.f};
+    if (ainv_lds && 4 * tid < nC * nC) ainv4 = *(const f4 *)(t.ainv + 4 * tid);
+    // Copy-in: where every level has a multiple of 4 cells (16-byte aligned
+    // LDS arrays and source arrays), each thread loads at most one 16-byte
+    // vector of each array (kTailMaxCells / 4 <= kTailThreads) of the first
+    // three levels, all in flight together, then stores them.
+    bool vec = true;
     for (int l = 0; l < t.nlev; l++) {
+        const TailLevel &G = t.lv[l];
+        vec = vec && G.d.cells() % 4 == 0 &&
+              (((uintptr_t)G.A.diag | (uintptr_t)G.A.cpl[0] | (uintptr_t)G.A.cpl[1] | (uintptr_t)G.A.cpl[2]) & 15) == 0;
+    }
+    vec = vec && ((uintptr_t)t.lv[0].b & 15) == 0;
+    int lstart = 0;  // levels still to copy by the scalar loop
+    if (vec) {
+        constexpr int kV = 3;
+        f4 v[kV][5];
+#pragma unroll
+        for (int l = 0; l < kV; l++) {
+            const TailLevel &G = t.lv[l < t.nlev ? l : 0];
+            const bool on = l < t.nlev && 4 * tid < (int)G.d.cells();
+            const f4 z = {0.f, 0.f, 0.f, 0.f};
+            v[l][0] = on ? *(const f4 *)(G.A.diag + 4 * tid) : z;
+            v[l][1] = on ? *(const f4 *)(G.A.cpl[0] + 4 * tid) : z;
+            v[l][2] = on ? *(const f4 *)(G.A.cpl[1] + 4 * tid) : z;
+            v[l][3] = on ? *(const f4 *)(G.A.cpl[2] + 4 * tid) : z;
+            v[l][4] = (on && l == 0) ? *(const f4 *)(G.b + 4 * tid) : z;
+        }
+#pragma unroll
+        for (int l = 0; l < kV; l++) {
+            if (l < t.nlev && 4 * tid < (int)t.lv[l].d.cells()) {
+                const TailLevel L = level(l);
+                *(f4 *)(const_cast<float *>(L.A.diag) + 4 * tid) = v[l][0];
+                *(f4 *)(const_cast<float *>(L.A.cpl[0]) + 4 * tid) = v[l][1];
+                *(f4 *)(const_cast<float *>(L.A.cpl[1]) + 4 * tid) = v[l][2];
+                *(f4 *)(const_cast<float *>(L.A.cpl[2]) + 4 * tid) = v[l][3];
+                if (l == 0) *(f4 *)(L.b + 4 * tid) = v[l][4];
+            }
+        }
+        lstart = kV;
+    }
+    for (int l = lstart; l < t.nlev; l++) {
         const TailLevel L = level(l);
         const TailLevel &G = t.lv[l];
-        for (int q = threadIdx.x; q < (int)G.d.cells(); q += kTailThreads) {
+        for (int q = tid; q < (int)G.d.cells(); q += kTailThreads) {
             const_cast<float *>(L.A.diag)[q] = G.A.diag[q];
             const_cast<float *>(L.A.cpl[0])[q] = G.A.cpl[0][q];
             const_cast<float *>(L.A.cpl[1])[q] = G.A.cpl[1][q];
@@ -1339,7 +1503,6 @@ __global__ __launch_bounds__(kTailThreads) void k_vcycle_tail(TailArgs t, const 
         }
     }
     __syncthreads();
-    const int last = t.nlev - 1;
     unsigned in_x1 = 0;  // bit l: level l's pre-smoothed iterate is in x1
     // descend: pre-smoothing and restriction (as the separate launches);
     // coefficient set 0 on the tail's first level, set 1 below it
@@ -1350,23 +1513,20 @@ __global__ __launch_bounds__(kTailThreads) void k_vcycle_tail(TailArgs t, const 
         const float *cd = t.cd[k], *cr = t.cr[k];
         float *c = L.x0, *o = L.x1;
         const Dims dc = t.lv[l + 1].d;
-        if (deg == 1) {
-            for (int q = threadIdx.x; q < (int)L.d.cells(); q += kTailThreads)
-                c[q] = cr[0] * rcp_or_zero(L.A.diag[q]) * L.b[q];
-            __syncthreads();
-        } else {
+        tail_first(L, c, cr[0]);
+        if (deg > 1) {
             if (deg > 2)
-                tail_smooth<SM_FIRST | SM_WRITE_D>(L, c, o, cr[0], cd[1], cr[1], nullptr, dc);
+                tail_smooth<SM_XFIRST | SM_WRITE_D>(L, c, o, cd[1], cr[1], nullptr, dc);
             else
-                tail_smooth<SM_FIRST>(L, c, o, cr[0], cd[1], cr[1], nullptr, dc);
+                tail_smooth<SM_XFIRST>(L, c, o, cd[1], cr[1], nullptr, dc);
             float *tmp = c;
             c = o;
             o = tmp;
             for (int s = 2; s < deg; s++) {
                 if (s < deg - 1)
-                    tail_smooth<SM_READ_D | SM_WRITE_D>(L, c, o, cr[0], cd[s], cr[s], nullptr, dc);
+                    tail_smooth<SM_READ_D | SM_WRITE_D>(L, c, o, cd[s], cr[s], nullptr, dc);
                 else
-                    tail_smooth<SM_READ_D>(L, c, o, cr[0], cd[s], cr[s], nullptr, dc);
+                    tail_smooth<SM_READ_D>(L, c, o, cd[s], cr[s], nullptr, dc);
                 tmp = c;
                 c = o;
                 o = tmp;
@@ -1378,11 +1538,27 @@ __global__ __launch_bounds__(kTailThreads) void k_vcycle_tail(TailArgs t, const 
     // coarsest: dense solve (k_coarse_solve's order)
     const TailLevel C = level(last);
     {
-        const int n = (int)C.d.cells();
-        for (int r = threadIdx.x; r < n; r += kTailThreads) {
-            float acc = 0.f;
-            for (int c = 0; c < n; c++) acc = fmaf(t.ainv[r * n + c], C.b[c], acc);
-            C.x0[r] = acc;
+        if (ainv_lds) {
+            // stored transposed (column c at a + c nC), so the row threads
+            // read consecutive words
+            float *a = level(0).dv;
+            if (4 * tid < nC * nC) {
+                const int r = (4 * tid) / nC, c0 = 4 * tid - r * nC;
+#pragma unroll
+                for (int q = 0; q < 4; q++) a[(c0 + q) * nC + r] = ainv4[q];
+            }
+            __syncthreads();
+            for (int r = tid; r < nC; r += kTailThreads) {
+                float acc = 0.f;
+                for (int c = 0; c < nC; c++) acc = fmaf(a[c * nC + r], C.b[c], acc);
+                C.x0[r] = acc;
+            }
+        } else {
+            for (int r = tid; r < nC; r += kTailThreads) {
+                float acc = 0.f;
+                for (int c = 0; c < nC; c++) acc = fmaf(t.ainv[r * nC + c], C.b[c], acc);
+                C.x0[r] = acc;
+            }
         }
         __syncthreads();
     }
@@ -1400,14 +1576,14 @@ __global__ __launch_bounds__(kTailThreads) void k_vcycle_tail(TailArgs t, const 
             const bool wd = s < deg - 1;
             if (s == 0) {
                 if (wd)
-                    tail_smooth<SM_PROLONG | SM_WRITE_D>(L, c, o, cr[0], 0.f, cr[0], e, dc);
+                    tail_smooth<SM_PROLONG | SM_WRITE_D>(L, c, o, 0.f, cr[0], e, dc);
                 else
-                    tail_smooth<SM_PROLONG>(L, c, o, cr[0], 0.f, cr[0], e, dc);
+                    tail_smooth<SM_PROLONG>(L, c, o, 0.f, cr[0], e, dc);
             } else {
                 if (wd)
-                    tail_smooth<SM_READ_D | SM_WRITE_D>(L, c, o, cr[0], cd[s], cr[s], nullptr, dc);
+                    tail_smooth<SM_READ_D | SM_WRITE_D>(L, c, o, cd[s], cr[s], nullptr, dc);
                 else
-                    tail_smooth<SM_READ_D>(L, c, o, cr[0], cd[s], cr[s], nullptr, dc);
+                    tail_smooth<SM_READ_D>(L, c, o, cd[s], cr[s], nullptr, dc);
             }
             float *tmp = c;
             c = o;
@@ -1416,7 +1592,7 @@ __global__ __launch_bounds__(kTailThreads) void k_vcycle_tail(TailArgs t, const 
         e = c;
     }
     // the first level's result (in its x1, where the level-by-level cycle leaves it)
-    for (int q = threadIdx.x; q < (int)t.lv[0].d.cells(); q += kTailThreads) t.lv[0].x1[q] = e[q];
+    for (int q = tid; q < (int)t.lv[0].d.cells(); q += kTailThreads) t.lv[0].x1[q] = e[q];
 }
 
 }  // namespace

@@ -195,13 +195,16 @@ def test_pair_path_solves_to_tolerance(oracle, n):
 
 
 @pytest.mark.parametrize("kind,n", [("pressure", (14, 14, 14)), ("pressure", (40, 37, 45)),
-                                    ("magnetic", 33), ("pressure", (96, 80, 112))])
+                                    ("magnetic", 33), ("pressure", (96, 80, 112)),
+                                    ("magnetic", 64), ("pressure", (64, 48, 32))])
 @pytest.mark.parametrize("degree,coarse", [(1, 1), (2, 5), (3, 4), (4, 3), (3, 1)])
 def test_vcycle_tail_matches_level_by_level(kind, n, degree, coarse):
     """The V-cycle with its small levels in one launch gives the same
     preconditioned vector, bit for bit, as level-by-level launches, and the
     whole solve the same solution and iteration count (fine and coarse
-    levels with their own smoother degrees)."""
+    levels with their own smoother degrees). The 64^3 and 64 x 48 x 32 grids
+    put z extents divisible by four into the tail (its 16-byte sweeps), the
+    others odd or ragged ones (one cell per thread)."""
     print(f"[case] vcycle tail {kind} {n} degree {degree}/{coarse}", flush=True)
     sc = scenes.random_pressure_scene(n, seed=5) if kind == "pressure" else \
         scenes.magnetic_scene(n, seed=5)
