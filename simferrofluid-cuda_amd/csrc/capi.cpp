@@ -491,6 +491,9 @@ sfp_status sfp_create(sfp_context *out, int nx, int ny, int nz, double spacing,
     try {
         SFP_HIP(hipSetDevice(device));
         SFP_HIP(hipStreamCreateWithFlags(&ctx->ws.stream, hipStreamNonBlocking));
+        SFP_HIP(hipStreamCreateWithFlags(&ctx->ws.side, hipStreamNonBlocking));
+        SFP_HIP(hipEventCreateWithFlags(&ctx->ws.side_in, hipEventDisableTiming));
+        SFP_HIP(hipEventCreateWithFlags(&ctx->ws.side_out, hipEventDisableTiming));
         for (auto &e : ctx->ev) SFP_HIP(hipEventCreate(&e));
         ctx->ws.partials.ensure(reduction_partials(ctx->d));
         ctx->ws.scalars.ensure(1);
@@ -512,14 +515,19 @@ void sfp_destroy(sfp_context ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->ws.stream) (void)hipStreamSynchronize(ctx->ws.stream);
+    if (ctx->ws.side) (void)hipStreamSynchronize(ctx->ws.side);
     for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {ctx->ws.side_in, ctx->ws.side_out})
         if (e) (void)hipEventDestroy(e);
     if (ctx->ws.host_sc) (void)hipHostFree(ctx->ws.host_sc);
     if (ctx->ws.host_err) (void)hipHostFree(ctx->ws.host_err);
-    hipStream_t s = ctx->ws.stream;
+    hipStream_t s = ctx->ws.stream, side = ctx->ws.side;
     ctx->ws.stream = nullptr;
+    ctx->ws.side = nullptr;
     delete ctx;  // frees device buffers
     if (s) (void)hipStreamDestroy(s);
+    if (side) (void)hipStreamDestroy(side);
 }
 
 sfp_status sfp_set_params(sfp_context ctx, const sfp_params *p) {

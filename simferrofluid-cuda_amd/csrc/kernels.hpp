@@ -107,8 +107,19 @@ void launch_vcycle_tail(hipStream_t s, const TailArgs &t, const double *gate = n
 // The buffer that holds the tail's result for its first level.
 inline float *tail_result(const TailArgs &t) { return t.lv[0].x1; }
 void launch_coarse_invert(hipStream_t s, Dims d, Op64 A, float *ainv);
-// v[0..3][c] = 0 (and w[c] = 0 when w is given) at the cells with diag32[c] == 0.
-void launch_clear_inactive(hipStream_t s, Dims d, const float *diag32, float *const v[4], double *w);
+// On every level l < nlev: v[l][0..3][c] = 0 (and on level 0 w[c] = 0 when
+// w is given) at the cells with diag32[l][c] == 0. start[l] is the first
+// cell of level l in the launch (start[l + 1] = start[l] + n[l]).
+constexpr int kClearMaxLevels = 16;
+struct ClearArgs {
+    int nlev = 0;
+    unsigned start[kClearMaxLevels + 1] = {};
+    unsigned n[kClearMaxLevels] = {};
+    const float *diag32[kClearMaxLevels] = {};
+    float *v[kClearMaxLevels][4] = {};
+    double *w = nullptr;
+};
+void launch_clear_inactive(hipStream_t s, const ClearArgs &a);
 void launch_coarse_solve(hipStream_t s, int n, const float *ainv, const float *b, float *x,
                          const double *gate = nullptr);
 

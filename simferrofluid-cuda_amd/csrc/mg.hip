@@ -990,21 +990,24 @@ __global__ __launch_bounds__(kBlock) void k_restrict_flat(Dims df, Op32 A, const
     }
 }
 
-// Zeros the level vectors (and, if given, one FP64 vector) at the inactive
-// cells (FP32 diagonal 0), one cell per lane: a new operator may make inactive
-// cells that were unknowns before, and the values left there would survive in
-// the groups of inactive cells the stencil kernels skip. Active cells are
-// written before they are read in every cycle, so they need no clearing.
-__global__ __launch_bounds__(kBlock) void k_clear_inactive(size_t n, const float *__restrict__ diag32,
-                                                           float *v0, float *v1, float *v2, float *v3,
-                                                           double *w) {
-    const size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n || diag32[c] > 0.f) return;
-    v0[c] = 0.f;
-    v1[c] = 0.f;
-    v2[c] = 0.f;
-    v3[c] = 0.f;
-    if (w) w[c] = 0.0;
+// Zeros the level vectors (and, if given, one FP64 vector of the first
+// level) at the inactive cells (FP32 diagonal 0): a new operator may make
+// inactive cells that were unknowns before, and the values left there would
+// survive in the groups of inactive cells the stencil kernels skip. Active
+// cells are written before they are read in every cycle, so they need no
+// clearing. Every level in one launch (the small levels' launches would each
+// cost more than their bytes), one cell per lane (consecutive lanes,
+// consecutive cells: the sparse stores stay coalesced).
+__global__ __launch_bounds__(kBlock) void k_clear_inactive(ClearArgs a) {
+    const unsigned gid = blockIdx.x * kBlock + threadIdx.x;
+    if (gid >= a.start[a.nlev]) return;
+    int l = 0;
+    while (gid >= a.start[l + 1]) l++;
+    const unsigned c = gid - a.start[l];
+    if (a.diag32[l][c] > 0.f) return;
+#pragma unroll
+    for (int k = 0; k < 4; k++) a.v[l][k][c] = 0.f;
+    if (l == 0 && a.w) a.w[c] = 0.0;
 }
 
 // Galerkin coarsening over the fine grid (the mapping of k_restrict): each
@@ -1103,6 +1106,131 @@ __global__ __launch_bounds__(kBlock) void k_coarsen_fine(CellGrid cg, Dims df, O
             }
 #pragma unroll
             for (int q = 0; q < kCoarsenSums; q++) acc[q] = 0.0;
+        }
+    }
+}
+
+// The same coarsening over the flat cell index (grids with nz a multiple of
+// 4), in k_restrict_flat's mapping: a wavefront covers 64 cells of a z row in
+// two rows and two planes, four cells per lane (lanes 0-15: row j, plane i;
+// 16-31: row j+1; 32-63: plane i+1). Each cell's contributions are
+// k_coarsen_fine's, and the children are added in its order -- the z pair in
+// the lane, the y pair across lanes 16 apart, the x pair across lanes 32
+// apart onto a zero -- so the coarse operator is the same to the bit; the
+// first 16 lanes store two coarse cells each.
+__global__ __launch_bounds__(kBlock) void k_coarsen_flat(Dims df, Op64 Af, const float *__restrict__ diag32f,
+                                                         Dims dc, double scale, double *diag64c, MutFaces cpl64c,
+                                                         float *diag32c, float *cpl32c0, float *cpl32c1,
+                                                         float *cpl32c2) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63;
+    const unsigned wave = (unsigned)(blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const unsigned zch = (unsigned)(df.nz + 63) / 64;
+    const unsigned yp = (unsigned)(df.ny + 1) / 2;
+    const unsigned kz = wave % zch, t = wave / zch;
+    const unsigned jp = t % yp, ip = t / yp;
+    if (ip >= (unsigned)(df.nx + 1) / 2) return;  // uniform over the wavefront (no barriers follow)
+    const int i = 2 * (int)ip + (lane >> 5), j = 2 * (int)jp + ((lane >> 4) & 1);
+    const int k = 64 * (int)kz + 4 * (lane & 15);
+    const bool on = i < df.nx && j < df.ny && k < df.nz;
+    const Strides s(df);
+    // v[h][q]: the sums of coarse z cell k / 2 + h (row sum, 3 lower, 3 upper, active count)
+    double v[2][kCoarsenSums];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int q = 0; q < kCoarsenSums; q++) v[h][q] = 0.0;
+    if (on) {
+        const size_t c = df.idx(i, j, k);
+        const bool upx = i + 1 < df.nx, upy = j + 1 < df.ny;
+        auto ld4 = [&](const double *p, double (&o)[4]) {
+            const d2 a = *(const d2 *)p, b = *(const d2 *)(p + 2);
+            o[0] = a.x;
+            o[1] = a.y;
+            o[2] = b.x;
+            o[3] = b.y;
+        };
+        double cl0[4], cl1[4], cl2[4], cu0[4] = {0.0, 0.0, 0.0, 0.0}, cu1[4] = {0.0, 0.0, 0.0, 0.0}, dg[4];
+        ld4(Af.cpl[0] + c, cl0);
+        ld4(Af.cpl[1] + c, cl1);
+        ld4(Af.cpl[2] + c, cl2);
+        if (upx) ld4(Af.cpl[0] + c + s.sx, cu0);
+        if (upy) ld4(Af.cpl[1] + c + s.sy, cu1);
+        const double cz4 = k + 4 < df.nz ? Af.cpl[2][c + 4] : 0.0;  // the face above the group
+        ld4(Af.diag + c, dg);
+        const f4 d32 = *(const f4 *)(diag32f + c);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int kq = k + q;
+            const bool upz = kq + 1 < df.nz;
+            const double cl[3] = {cl0[q], cl1[q], cl2[q]};
+            const double cu[3] = {upx ? cu0[q] : 0.0, upy ? cu1[q] : 0.0,
+                                  upz ? (q < 3 ? cl2[q + 1] : cz4) : 0.0};
+            const bool upv[3] = {upx, upy, upz};
+            const int idx[3] = {i, j, kq};
+            double w[kCoarsenSums];
+#pragma unroll
+            for (int e = 0; e < kCoarsenSums; e++) w[e] = 0.0;
+            if (d32[q] > 0.f) {
+                double off = cl[0] + cl[1] + cl[2];
+                if (upv[0]) off += cu[0];
+                if (upv[1]) off += cu[1];
+                if (upv[2]) off += cu[2];
+                w[0] = dg[q] - off;
+                w[7] = 1.0;
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                if ((idx[a] & 1) == 0) w[1 + a] = cl[a];  // lower layer: the face below
+                else w[4 + a] = cu[a];                     // upper layer: the face above
+            }
+            // the z pair: the even cell's sums first, then the odd cell's added
+#pragma unroll
+            for (int e = 0; e < kCoarsenSums; e++) v[q >> 1][e] = (q & 1) ? v[q >> 1][e] + w[e] : w[e];
+        }
+    }
+    // y pair (the row-j lanes keep row j's sums first), then the x pair onto 0
+    double acc[2][kCoarsenSums];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int e = 0; e < kCoarsenSums; e++) {
+            const double y = v[h][e] + __shfl_xor(v[h][e], 16, 64);
+            const double o = __shfl_xor(y, 32, 64);
+            double a = 0.0;
+            a += y;
+            if (2 * (int)ip + 1 < df.nx) a += o;
+            acc[h][e] = a;
+        }
+    if (lane < 16 && on) {
+        float *cpl32c[3] = {cpl32c0, cpl32c1, cpl32c2};
+        const int I = (int)ip, J = (int)jp;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int K = (k >> 1) + h;
+            const int IC[3] = {I, J, K};
+            double lower[3], upper[3];
+            for (int a = 0; a < 3; a++) {
+                lower[a] = IC[a] > 0 ? scale * acc[h][1 + a] : 0.0;
+                upper[a] = IC[a] + 1 < dc.n(a) ? scale * acc[h][4 + a] : 0.0;
+            }
+            const bool active = acc[h][7] > 0.0;
+            const size_t cc = dc.idx(I, J, K);
+            if (active) {
+                const double dgc =
+                    scale * acc[h][0] + lower[0] + upper[0] + lower[1] + upper[1] + lower[2] + upper[2];
+                diag64c[cc] = dgc;
+                diag32c[cc] = dgc > 0.0 ? (float)dgc : 0.f;
+            } else {
+                diag64c[cc] = 1.0;
+                diag32c[cc] = 0.f;
+            }
+            for (int a = 0; a < 3; a++) {
+                const double cv = active ? lower[a] : 0.0;
+                cpl64c.a[a][cc] = cv;
+                cpl32c[a][cc] = (float)cv;
+            }
         }
     }
 }
@@ -1789,10 +1917,11 @@ void launch_prolong_add(hipStream_t st, Dims d, const float *diag32, const float
     SFP_LAUNCH_CHECK();
 }
 
-void launch_clear_inactive(hipStream_t st, Dims d, const float *diag32, float *const v[4], double *w) {
-    const size_t n = d.cells();
-    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_clear_inactive, dim3(nb), dim3(kBlock), 0, st, n, diag32, v[0], v[1], v[2], v[3], w);
+void launch_clear_inactive(hipStream_t st, const ClearArgs &a) {
+    if (a.nlev < 1 || a.nlev > kClearMaxLevels) throw ArgError("launch_clear_inactive: bad level count");
+    const unsigned groups = a.start[a.nlev];
+    const unsigned nb = (groups + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_clear_inactive, dim3(nb), dim3(kBlock), 0, st, a);
     SFP_LAUNCH_CHECK();
 }
 
@@ -1826,10 +1955,20 @@ void launch_restrict(hipStream_t st, Dims df, Op32 A, const float *b, const floa
 void launch_coarsen(hipStream_t st, Dims df, Op64 Af, const float *diag32f, Dims dc,
                     double scale, double *diag64c, double *const cpl64c[3], float *diag32c,
                     float *const cpl32c[3]) {
+    MutFaces m{{cpl64c[0], cpl64c[1], cpl64c[2]}};
+    bool flat = flat_step(df) && (uintptr_t)diag32f % 16 == 0;
+    for (const double *a : {Af.diag, Af.cpl[0], Af.cpl[1], Af.cpl[2]}) flat = flat && (uintptr_t)a % 16 == 0;
+    if (flat) {
+        const size_t waves = (size_t)((df.nz + 63) / 64) * ((df.ny + 1) / 2) * ((df.nx + 1) / 2);
+        const unsigned nb = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_coarsen_flat, dim3(nb), dim3(kBlock), 0, st, df, Af, diag32f, dc, scale, diag64c,
+                           m, diag32c, cpl32c[0], cpl32c[1], cpl32c[2]);
+        SFP_LAUNCH_CHECK();
+        return;
+    }
     // over the fine grid, with even plane pairs per block (as the restriction)
     Launch L = cell_launch(df, 0, 8);
     if (L.cg.planes & 1) L = cell_launch(df, L.cg.planes + 1);
-    MutFaces m{{cpl64c[0], cpl64c[1], cpl64c[2]}};
     hipLaunchKernelGGL(k_coarsen_fine, L.grid, L.block, 0, st, L.cg, df, Af, diag32f, dc, scale, diag64c,
                        m, diag32c, cpl32c[0], cpl32c[1], cpl32c[2]);
     SFP_LAUNCH_CHECK();

@@ -122,7 +122,15 @@ void System::allocate(Dims d) {
     built_coarsest_ = -1;
 }
 
+void System::join_ainv(Workspace &ws) {
+    if (!ainv_pending_) return;
+    SFP_HIP(hipStreamWaitEvent(ws.stream, ws.side_out, 0));
+    ainv_pending_ = false;
+}
+
 void System::build_hierarchy(Workspace &ws, const SolverConfig &cfg) {
+    // an inverse still being formed reads the coarsest operator rebuilt below
+    join_ainv(ws);
     const int coarsest = std::max(2, std::min(cfg.coarsest, 4));
     if (built_coarsest_ != coarsest) {
         lv_.resize(1);
@@ -157,7 +165,18 @@ void System::build_hierarchy(Workspace &ws, const SolverConfig &cfg) {
         launch_coarsen(ws.stream, F.d, F.op64(), F.diag32.p, C.d, cfg.coarse_scale, C.diag64.p, c64,
                        C.diag32.p, c32);
     }
-    launch_coarse_invert(ws.stream, lv_.back().d, lv_.back().op64(), ainv_.p);
+    // The dense inverse of the coarsest level (a one-block kernel bound by its
+    // pivot steps) runs on the side stream, overlapping the clears below, the
+    // initial residual and the host's read of it; its first reader joins it.
+    if (ws.side) {
+        SFP_HIP(hipEventRecord(ws.side_in, ws.stream));
+        SFP_HIP(hipStreamWaitEvent(ws.side, ws.side_in, 0));
+        launch_coarse_invert(ws.side, lv_.back().d, lv_.back().op64(), ainv_.p);
+        SFP_HIP(hipEventRecord(ws.side_out, ws.side));
+        ainv_pending_ = true;
+    } else {
+        launch_coarse_invert(ws.stream, lv_.back().d, lv_.back().op64(), ainv_.p);
+    }
     // the row-tiled step pairs skip the loads of z rows without active cells
     for (Level &L : lv_) {
         if (row_pair(L.d) && !L.pinned_rowsum) {
@@ -174,11 +193,22 @@ void System::build_hierarchy(Workspace &ws, const SolverConfig &cfg) {
     // written before they are read in every cycle; p and q are written at
     // every cell of the update's groups by the first iteration and read in
     // those groups only.
+    if (lv_.size() > (size_t)kClearMaxLevels) throw ArgError("too many multigrid levels");
+    ClearArgs ca;
+    ca.nlev = (int)lv_.size();
+    size_t total = 0;
     for (size_t l = 0; l < lv_.size(); l++) {
         Level &L = lv_[l];
+        total += L.d.cells();
+        if (total >= ((size_t)1 << 32)) throw ArgError("grid too large for the level clear");
+        ca.n[l] = (unsigned)L.d.cells();
+        ca.start[l + 1] = (unsigned)total;
+        ca.diag32[l] = L.diag32.p;
         float *const v[4] = {L.x0.p, L.x1.p, L.dv.p, L.b.p};
-        launch_clear_inactive(ws.stream, L.d, L.diag32.p, v, l == 0 ? w_.p : nullptr);
+        for (int k = 0; k < 4; k++) ca.v[l][k] = v[k];
     }
+    ca.w = w_.p;
+    launch_clear_inactive(ws.stream, ca);
 }
 
 float *System::vcycle_tail(Workspace &ws, const Smoothing &sm, int l) {
@@ -201,6 +231,7 @@ float *System::vcycle_tail(Workspace &ws, const Smoothing &sm, int l) {
     }
     t.ainv = ainv_.p;
     if (tail_lds_bytes(t) > kTailLdsBytes) return nullptr;
+    join_ainv(ws);
     launch_vcycle_tail(ws.stream, t, ws.gate);
     return tail_result(t);
 }
@@ -233,6 +264,7 @@ float *System::vlevel(Workspace &ws, const Smoothing &sm, int l, const double *r
     const double cells = (double)L.d.cells();
     if (l + 1 == (int)lv_.size()) {
         if (r64) launch_smooth_first(st, L.d, L.op32(), r64, L.b.p, L.x1.p, 0.f, ws.gate);
+        join_ainv(ws);
         launch_coarse_solve(st, (int)L.d.cells(), ainv_.p, L.b.p, L.x0.p, ws.gate);
         return L.x0.p;
     }

@@ -123,6 +123,11 @@ struct Workspace {
     // level whose x0 holds the first Chebyshev iterate of its b (stored by
     // the FCG update for the next cycle), or -1
     int x0_level = -1;
+    // a second stream (default priority) on which the coarsest level's dense
+    // inverse is formed while `stream` runs the rest of the set-up; side_in
+    // orders it after the coarsening, side_out orders its first use after it
+    hipStream_t side = nullptr;
+    hipEvent_t side_in = nullptr, side_out = nullptr;
     void read_scalars();               // synchronous copy of *scalars to host_sc
 };
 
@@ -193,6 +198,10 @@ class System {
     std::vector<Level> lv_;
     DevBuf<double> rhs_, x_, r_, p_, q_, w_;
     DevBuf<float> ainv_;
+    // the dense inverse is being formed on ws.side: join before reading it
+    // (or before the coarsest operator it reads is rewritten)
+    bool ainv_pending_ = false;
+    void join_ainv(Workspace &ws);
     int built_coarsest_ = -1;
 };
 

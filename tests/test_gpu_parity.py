@@ -271,13 +271,16 @@ def test_magnetic_zero_susceptibility_gives_applied_field(oracle):
         assert np.array_equal(H[a], sc["hext"][a])
 
 
-@pytest.mark.parametrize("which", [0, 1])
-def test_coarsening_is_scaled_galerkin(oracle, which):
-    print(f"[case] coarsening which={which}", flush=True)
+@pytest.mark.parametrize("which,case", [(0, "ragged"), (1, "ragged"), (0, "z4"), (1, "z4")])
+def test_coarsening_is_scaled_galerkin(oracle, which, case):
+    """The coarse operators are ½ P^T A P of the aggregation prolongation P,
+    on grids whose z extent is not a multiple of 4 (the x-plane tiled
+    coarsening) and on grids where it is (the flat one)."""
+    print(f"[case] coarsening which={which} {case}", flush=True)
     if which == 0:
-        sc = scenes.random_pressure_scene((12, 10, 14), seed=9)
+        sc = scenes.random_pressure_scene((12, 10, 14) if case == "ragged" else (16, 12, 20), seed=9)
     else:
-        sc = _mag("spike_ragged")
+        sc = _mag("spike_ragged") if case == "ragged" else scenes.magnetic_scene(32, seed=3)
     with sfp.Context.for_scene(sc) as ctx:
         if which == 0:
             ctx.pressure_project(sc)
