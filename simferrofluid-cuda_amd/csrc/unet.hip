@@ -12,6 +12,8 @@
 #include "device_util.hpp"
 #include "unet.hpp"
 
+#include <type_traits>
+
 namespace sfp {
 namespace {
 
@@ -38,7 +40,26 @@ struct ConvArgs {
     int relu;
     int up;  // 1: `in` is [B][X/2][Y/2][Z/2][CIN], read nearest-up-sampled
     const double *gate;  // nonzero *gate: return at once (a stopped batch of FCG iterations)
+    unsigned in_bytes;   // size of `in` (at most kInLimit)
+    unsigned out_elems;  // B X Y Z COUT: the elements of out, out32, relu_copy and skip
 };
+
+// Raw buffer resources over the tensors: a load at an offset past the
+// buffer's size reads 0 and a store there is dropped, so the padding and the
+// partial tiles need no branches or address selects. An input offset is the
+// sum of a row part and a (z, channel) part, either of which is kInOutside
+// when outside the volume: with the input at most kInLimit bytes, the sum is
+// then past it too. Output offsets outside the volume are kOutOutside (the
+// outputs are at most 2 GiB).
+constexpr unsigned kInLimit = 0x40000000u, kInOutside = 0x40000000u, kOutOutside = 0x80000000u;
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ int ch_of(int nb, int r16) { return nb * 16 + r16; }
+__device__ __forceinline__ uint16_t bf16_bits(float f) {  // round to nearest even (v_cvt_pk_bf16_f32)
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
+}
 
 // The 3x3x3 convolution (zero padding), optionally reading its input
 // nearest-up-sampled, with bias, ReLU, a skip tensor added after the ReLU and
@@ -85,12 +106,72 @@ __device__ __forceinline__ void kslice(int kc, int g, int &dz, int &c, bool &ok)
 // the K chunk) load from here, so no loaded register is overwritten later.
 __device__ __attribute__((aligned(16))) uint16_t g_zero8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
+// An input row is loaded once per plane: fragment C holds voxel z0 + m of
+// lane m (its channel slice), and H the two halo voxels, z0 - 1 in lane m = 0
+// and z0 + 16 in lane m = 15 (of each 16-lane row). The z-shifted fragments
+// of the K chunks come from C by DPP shifts within the 16-lane rows (lane m
+// takes lane m - 1, or m + 1), with H at the row's end, so they equal the
+// fragments a load of the shifted voxels would give, bit for bit.
+template <int CIN>
+constexpr int kcentres() {  // C fragments per row: the channel halves of CIN = 64
+    return CIN == 64 ? 2 : 1;
+}
+// the channel offset of lane group g's slice in centre fragment h
+template <int CIN>
+__device__ __forceinline__ int centre_channel(int h, int g) {
+    if constexpr (CIN == 8) return 0;
+    else if constexpr (CIN == 16) return 8 * (g & 1);
+    else if constexpr (CIN == 32) return 8 * g;
+    else return 32 * h + 8 * g;
+}
+// whether lane group g of centre fragment h needs the halo below (z0 - 1) or above (z0 + 16)
+template <int CIN>
+__device__ __forceinline__ bool halo_lo(int g) {
+    return CIN == 8 ? g == 0 : CIN == 16 ? g < 2 : true;
+}
+template <int CIN>
+__device__ __forceinline__ bool halo_hi(int g) {
+    return CIN == 8 ? g == 2 : CIN == 16 ? g < 2 : true;
+}
+// x of lane m - 1 (from_below) or m + 1 within each 16-lane row; the lane at
+// the row's end, which has no such neighbour, keeps `end` (bound_ctrl off:
+// the DPP move leaves it unwritten), so the halo needs no select.
+__device__ __forceinline__ v8bf dpp_shift(v8bf x, v8bf end, bool from_below) {
+    const u4v u = __builtin_bit_cast(u4v, x), e = __builtin_bit_cast(u4v, end);
+    u4v r;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        r[q] = from_below ? (unsigned)__builtin_amdgcn_update_dpp((int)e[q], (int)u[q], 0x111, 0xf, 0xf, false)
+                          : (unsigned)__builtin_amdgcn_update_dpp((int)e[q], (int)u[q], 0x101, 0xf, 0xf, false);
+    return __builtin_bit_cast(v8bf, r);
+}
+// The KC chunks of one row (kslice's layout) from its centre and halo fragments.
+template <int CIN>
+__device__ __forceinline__ void row_chunks(const v8bf (&c)[kcentres<CIN>()], const v8bf (&h)[kcentres<CIN>()],
+                                           int g, v8bf (&out)[kchunks<CIN>()]) {
+    const v8bf zero = __builtin_bit_cast(v8bf, u4v{0u, 0u, 0u, 0u});
+#pragma unroll
+    for (int q = 0; q < kcentres<CIN>(); q++) {
+        const v8bf lo = dpp_shift(c[q], h[q], true);   // voxel z0 + m - 1 (row_shr:1; m = 0: the halo)
+        const v8bf hi = dpp_shift(c[q], h[q], false);  // voxel z0 + m + 1 (row_shl:1; m = 15: the halo)
+        if constexpr (CIN == 8) {
+            out[0] = g == 0 ? lo : g == 1 ? c[0] : g == 2 ? hi : zero;
+        } else if constexpr (CIN == 16) {
+            out[0] = g < 2 ? lo : c[0];
+            out[1] = g < 2 ? hi : zero;
+        } else {  // 32, 64: chunk dz + 3 q
+            out[3 * q] = lo;
+            out[3 * q + 1] = c[q];
+            out[3 * q + 2] = hi;
+        }
+    }
+}
+
 template <int CIN, int COUT, int PY, int WPS>
 __global__ __launch_bounds__(256, WPS) void k_conv3r(ConvArgs a, int run, int xruns) {
     constexpr int KC = kchunks<CIN>();
     constexpr int NB = COUT / 16;
     if (a.gate && *a.gate != 0.0) return;
-    const uint16_t *__restrict__ in = a.in;
     const uint16_t *__restrict__ wts = a.w;
     const int X = a.X, Y = a.Y, Z = a.Z;
     const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
@@ -123,131 +204,186 @@ __global__ __launch_bounds__(256, WPS) void k_conv3r(ConvArgs a, int run, int xr
                     ok ? wts + ((size_t)((dx * 9 + dy * 3 + dz) * COUT + nb * 16 + r16) * CIN + c) : g_zero8;
                 wf[dx][dy][kc] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4 *>(p));
             }
-    // this lane's input voxel (z) and channel offset per chunk: A[m][k] of
-    // lane m + 16 (k / 8), m = z - z0 of the output row; the element offset
-    // within an (x, y) row of the (possibly half-resolution) input
+    // this lane's input voxel (z) and channel byte offset per chunk within an
+    // (x, y) row of the (possibly half-resolution) input: A[m][k] of lane
+    // m + 16 (k / 8), m = z - z0 of the output row
     const int up = a.up;
     const int Xs = up ? X >> 1 : X, Ys = up ? Y >> 1 : Y, Zs = up ? Z >> 1 : Z;
-    int zoff[KC];
-    bool zok[KC];
+    const auto rin = buf_rsrc(a.in, a.in_bytes);
+    // 16 input channels and more: one centre and one halo fragment per row
+    // and channel half (the chunks derived by DPP); 8 channels: the single
+    // chunk, which already holds the three z shifts, loaded directly
+    constexpr bool kDpp = CIN >= 16;
+    constexpr int NC = kDpp ? kcentres<CIN>() : KC;
+    // byte offsets within an (x, y) row: of the centre and halo fragments, or
+    // of the chunks
+    unsigned zc[NC], zh[NC];
 #pragma unroll
-    for (int kc = 0; kc < KC; kc++) {
-        int dz, c;
-        bool ok;
-        kslice<CIN>(kc, g, dz, c, ok);
-        const int zi = z0 + r16 + dz - 1;
-        zok[kc] = ok && zi >= 0 && zi < Z;
-        const int cz = min(max(zi, 0), Z - 1);  // a valid address either way
-        zoff[kc] = (up ? cz >> 1 : cz) * CIN + c;
+    for (int q = 0; q < NC; q++) {
+        if constexpr (kDpp) {
+            const int c = centre_channel<CIN>(q, g);
+            auto zoff = [&](int zi) { return (unsigned)(((up ? zi >> 1 : zi) * CIN + c) * 2); };
+            const int zi = z0 + r16;
+            zc[q] = zi < Z ? zoff(zi) : kInOutside;
+            zh[q] = kInOutside;
+            if (r16 == 0 && halo_lo<CIN>(g) && z0 > 0) zh[q] = zoff(z0 - 1);
+            if (r16 == 15 && halo_hi<CIN>(g) && z0 + 16 < Z) zh[q] = zoff(z0 + 16);
+        } else {
+            int dz, c;
+            bool ok;
+            kslice<CIN>(q, g, dz, c, ok);
+            const int zi = z0 + r16 + dz - 1;
+            zc[q] = ok && zi >= 0 && zi < Z ? (unsigned)(((up ? zi >> 1 : zi) * CIN + c) * 2) : kInOutside;
+            zh[q] = kInOutside;
+        }
     }
+    const unsigned row_bytes = (unsigned)(Zs * CIN * 2);
+    // Offsets that do not change along the march, per lane: the input rows
+    // y0 - 1 .. y0 + PY (their y part plus the z/channel part; kInOutside
+    // parts where outside the volume), and the output values' (y, z, channel)
+    // part (kOutOutside outside). Each step adds its plane's part, a value
+    // common to the wavefront.
+    unsigned rzc[PY + 2][NC], rzh[PY + 2][NC];
+#pragma unroll
+    for (int iy = 0; iy < PY + 2; iy++) {
+        const int gy = y0 + iy - 1;
+        const unsigned ry = gy >= 0 && gy < Y ? (unsigned)(up ? gy >> 1 : gy) * row_bytes : kInOutside;
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            rzc[iy][q] = ry + zc[q];
+            rzh[iy][q] = ry + zh[q];
+        }
+    }
+    unsigned ob[PY][4];  // byte offsets of the 16-bit outputs within a plane
+#pragma unroll
+    for (int oy = 0; oy < PY; oy++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int gy = y0 + oy, gz = z0 + 4 * g + j;
+            ob[oy][j] = gy < Y && gz < Z ? (unsigned)((gy * Z + gz) * COUT + ch_of(nb, r16)) * 2 : kOutOutside;
+        }
     v4f acc[3][PY];
 #pragma unroll
     for (int s = 0; s < 3; s++)
 #pragma unroll
         for (int oy = 0; oy < PY; oy++) acc[s][oy] = v4f{0.f, 0.f, 0.f, 0.f};
     const int steps = xe - xa + 2;  // input planes xa - 1 .. xe
-    // The fragments of input plane gx: rows y0 - 1 .. y0 + PY, zero outside
-    // the volume. Branch-free (the address is selected, never the loaded
-    // value), so the next plane's loads stay in flight during this plane's
-    // MFMAs.
-    auto load_plane = [&](v8bf(&f)[PY + 2][KC], int gx) {
-        const bool xok = gx >= 0 && gx < X;
-        const int cx = min(max(gx, 0), X - 1);
-        const size_t xrow = ((size_t)b * Xs + (up ? cx >> 1 : cx)) * Ys;
+    // The centre and halo fragments of input plane gx: rows y0 - 1 .. y0 +
+    // PY, zero outside the volume (offsets past the buffer), so the next
+    // plane's loads stay in flight during this plane's MFMAs.
+    auto load_plane = [&](v8bf(&fc)[PY + 2][NC], v8bf(&fh)[PY + 2][NC], int gx) {
+        // the plane's part (every term below 1 GiB or kInOutside: the sum of
+        // three stays below 4 GiB, and past the buffer if any term is outside)
+        const unsigned po = gx >= 0 && gx < X ? (unsigned)((b * Xs + (up ? gx >> 1 : gx)) * Ys) * row_bytes
+                                              : kInOutside;
+#pragma unroll
+        for (int iy = 0; iy < PY + 2; iy++)
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                fc[iy][q] = __builtin_bit_cast(v8bf, __builtin_amdgcn_raw_buffer_load_b128(rin, po + rzc[iy][q], 0, 0));
+                if constexpr (kDpp)
+                    fh[iy][q] =
+                        __builtin_bit_cast(v8bf, __builtin_amdgcn_raw_buffer_load_b128(rin, po + rzh[iy][q], 0, 0));
+            }
+    };
+    const int ch = nb * 16 + r16;
+    const float bias = a.bias ? a.bias[ch] : 0.f;
+    const auto rskip = buf_rsrc(a.skip, a.skip ? 2 * a.out_elems : 0);
+    const auto rout = buf_rsrc(a.out, a.out ? 2 * a.out_elems : 0);
+    const auto rout32 = buf_rsrc(a.out32, a.out32 ? 4 * a.out_elems : 0);
+    const auto rrelu = buf_rsrc(a.relu_copy, a.relu_copy ? 2 * a.out_elems : 0);
+    const bool has_skip = a.skip != nullptr, has_relu_copy = a.relu_copy != nullptr;
+    const bool relu = a.relu != 0, f32 = a.out32 != nullptr;
+    // Two fragment buffers alternate with the step's parity and three
+    // accumulator slots rotate with the step mod 3: the march is unrolled by
+    // six, so every index is a constant and no fragment is copied. Step t
+    // reads input plane gx = xa - 1 + t; output plane gx + 1 - dx
+    // accumulates in slot (t - dx) mod 3.
+    v8bf fc[2][PY + 2][NC], fh[2][PY + 2][NC];
+    load_plane(fc[0], fh[0], xa - 1);
+    auto step = [&](auto Pc, auto Uc, int t) {
+        constexpr int P = decltype(Pc)::value, U = decltype(Uc)::value;
+        const int gx = xa - 1 + t;
+        // byte offset of output plane gx - 1 (the plane this step completes)
+        // in the 16-bit outputs
+        const unsigned pb = (unsigned)((b * X + (gx - 1)) * Y * Z * COUT) * 2;
+        // the skip values of that plane, loaded before the next plane's
+        // fragments so that waiting for them leaves those in flight
+        uint16_t skv[PY][4];
+        if (has_skip && t >= 2) {
+#pragma unroll
+            for (int oy = 0; oy < PY; oy++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    skv[oy][j] = __builtin_amdgcn_raw_buffer_load_b16(rskip, pb + ob[oy][j], 0, 0);
+        }
+        load_plane(fc[1 - P], fh[1 - P], gx + 1);  // (one plane past the run at the end: unused)
 #pragma unroll
         for (int iy = 0; iy < PY + 2; iy++) {
-            const int gy = y0 + iy - 1;
-            const bool ok = xok && gy >= 0 && gy < Y;
-            const int cy = min(max(gy, 0), Y - 1);
-            const uint16_t *row = in + (xrow + (up ? cy >> 1 : cy)) * Zs * CIN;
+            v8bf ck[KC];
+            if constexpr (kDpp)
+                row_chunks<CIN>(fc[P][iy], fh[P][iy], g, ck);
+            else
 #pragma unroll
-            for (int kc = 0; kc < KC; kc++) {
-                const uint16_t *p = ok && zok[kc] ? row + zoff[kc] : g_zero8;
-                f[iy][kc] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4 *>(p));
+                for (int kc = 0; kc < KC; kc++) ck[kc] = fc[P][iy][kc];
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++) {
+                    const int oy = iy - dy;
+                    if (oy < 0 || oy >= PY) continue;  // known after unrolling
+#pragma unroll
+                    for (int kc = 0; kc < KC; kc++)
+                        acc[(U - dx + 3) % 3][oy] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            ck[kc], wf[dx][dy][kc], acc[(U - dx + 3) % 3][oy], 0, 0, 0);
+                }
+        }
+        // planes before the run (gx + 1 - dx < xa) took contributions in
+        // the first two steps: clear their slots
+        if (t == 0) {
+#pragma unroll
+            for (int oy = 0; oy < PY; oy++) acc[(U + 1) % 3][oy] = acc[(U + 2) % 3][oy] = v4f{0.f, 0.f, 0.f, 0.f};
+        } else if (t == 1) {
+#pragma unroll
+            for (int oy = 0; oy < PY; oy++) acc[(U + 1) % 3][oy] = v4f{0.f, 0.f, 0.f, 0.f};
+        }
+        // output plane gx - 1 has had its three input planes
+        if (t >= 2) {
+            constexpr int S = (U + 1) % 3;
+#pragma unroll
+            for (int oy = 0; oy < PY; oy++) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const unsigned o = pb + ob[oy][j];  // past the buffers where outside the volume
+                    float v = acc[S][oy][j] + bias;
+                    if (relu) v = v > 0.f ? v : 0.f;
+                    if (has_relu_copy) __builtin_amdgcn_raw_buffer_store_b16(bf16_bits(v), rrelu, o, 0, 0);
+                    if (has_skip) v += bf2f(skv[oy][j]);
+                    if (f32)  // (FP32 output: twice the 16-bit offset)
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rout32,
+                                                              ob[oy][j] == kOutOutside ? kOutOutside : 2 * o, 0, 0);
+                    else
+                        __builtin_amdgcn_raw_buffer_store_b16(bf16_bits(v), rout, o, 0, 0);
+                }
+                acc[S][oy] = v4f{0.f, 0.f, 0.f, 0.f};
             }
         }
     };
-    v8bf cur[PY + 2][KC], nxt[PY + 2][KC];
-    load_plane(cur, xa - 1);
-    const int ch = nb * 16 + r16;
-    const float bias = a.bias ? a.bias[ch] : 0.f;
-    const uint16_t *__restrict__ skip = a.skip;
-    // step t reads input plane gx = xa - 1 + t; with U = t mod 3, output
-    // plane gx + 1 - dx accumulates in slot (U - dx) mod 3
-    for (int t0 = 0; t0 < steps; t0 += 3) {
-#pragma unroll
-        for (int U = 0; U < 3; U++) {
-            const int t = t0 + U;
-            if (t >= steps) break;  // uniform
-            const int gx = xa - 1 + t;
-            // the skip values of the plane this step completes, loaded before
-            // the next plane's fragments so that waiting for them leaves those
-            // in flight
-            uint16_t skv[PY][4];
-            if (skip && t >= 2) {
-#pragma unroll
-                for (int oy = 0; oy < PY; oy++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int gy = y0 + oy, gz = z0 + 4 * g + j;
-                        const size_t vox = (((size_t)b * X + (gx - 1)) * Y + gy) * Z + gz;
-                        skv[oy][j] = *(gy < Y && gz < Z ? skip + vox * COUT + ch : g_zero8);
-                    }
-            }
-            load_plane(nxt, gx + 1);  // (one plane past the run at the end: unused)
-#pragma unroll
-            for (int iy = 0; iy < PY + 2; iy++)
-#pragma unroll
-                for (int dx = 0; dx < 3; dx++)
-#pragma unroll
-                    for (int dy = 0; dy < 3; dy++) {
-                        const int oy = iy - dy;
-                        if (oy < 0 || oy >= PY) continue;  // known after unrolling
-#pragma unroll
-                        for (int kc = 0; kc < KC; kc++)
-                            acc[(U - dx + 3) % 3][oy] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                cur[iy][kc], wf[dx][dy][kc], acc[(U - dx + 3) % 3][oy], 0, 0, 0);
-                    }
-            // planes before the run (gx + 1 - dx < xa) took contributions in
-            // the first two steps: clear their slots
-            if (t == 0) {
-#pragma unroll
-                for (int oy = 0; oy < PY; oy++) acc[1][oy] = acc[2][oy] = v4f{0.f, 0.f, 0.f, 0.f};
-            } else if (t == 1) {
-#pragma unroll
-                for (int oy = 0; oy < PY; oy++) acc[2][oy] = v4f{0.f, 0.f, 0.f, 0.f};
-            }
-            // output plane gx - 1 has had its three input planes
-            if (t >= 2) {
-                const int S = (U + 1) % 3;
-                const int ox = gx - 1;
-#pragma unroll
-                for (int oy = 0; oy < PY; oy++) {
-                    const int gy = y0 + oy;
-                    if (gy < Y) {
-#pragma unroll
-                        for (int j = 0; j < 4; j++) {
-                            const int gz = z0 + 4 * g + j;
-                            if (gz >= Z) continue;
-                            const size_t vox = (((size_t)b * X + ox) * Y + gy) * Z + gz;
-                            float v = acc[S][oy][j] + bias;
-                            if (a.relu) v = v > 0.f ? v : 0.f;
-                            if (a.relu_copy) a.relu_copy[vox * COUT + ch] = f2bf(v);
-                            if (skip) v += bf2f(skv[oy][j]);
-                            if (a.out32)
-                                a.out32[vox * COUT + ch] = v;
-                            else
-                                a.out[vox * COUT + ch] = f2bf(v);
-                        }
-                    }
-                    acc[S][oy] = v4f{0.f, 0.f, 0.f, 0.f};
-                }
-            }
-#pragma unroll
-            for (int iy = 0; iy < PY + 2; iy++)
-#pragma unroll
-                for (int kc = 0; kc < KC; kc++) cur[iy][kc] = nxt[iy][kc];
-        }
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    for (int t0 = 0; t0 < steps; t0 += 6) {  // uniform over the wavefront
+        step(I0{}, I0{}, t0);
+        if (t0 + 1 >= steps) break;
+        step(I1{}, I1{}, t0 + 1);
+        if (t0 + 2 >= steps) break;
+        step(I0{}, I2{}, t0 + 2);
+        if (t0 + 3 >= steps) break;
+        step(I1{}, I0{}, t0 + 3);
+        if (t0 + 4 >= steps) break;
+        step(I0{}, I1{}, t0 + 4);
+        if (t0 + 5 >= steps) break;
+        step(I1{}, I2{}, t0 + 5);
     }
 }
 
@@ -594,8 +730,8 @@ void conv_dispatch(hipStream_t st, const ConvArgs &a) {
     // U-Net level (128^3, 64^3, 32^3); more rows reuse each loaded fragment
     // more often but hold more VGPRs (the weight fragments alone take 36 to
     // 216)
-    constexpr int PY = CIN == 8 ? 4 : CIN == 16 ? 2 : 1;
-    constexpr int WPS = CIN <= 16 ? 2 : 1;
+    constexpr int PY = CIN == 8 ? 4 : COUT == 64 ? 1 : 2;
+    constexpr int WPS = CIN <= 32 ? 2 : 1;
     // x runs: about two wavefronts per SIMD of the 256 CUs
     const long long base = (long long)a.B * ((a.Z + 15) / 16) * ((a.Y + PY - 1) / PY) * (COUT / 16);
     const long long target = 2048;
@@ -623,7 +759,12 @@ void unet_conv(hipStream_t st, int cin, int cout, const uint16_t *in, const uint
                uint16_t *relu_copy, int B, int X, int Y, int Z, bool relu, bool up_input,
                const double *gate) {
     if (up_input && (X % 2 || Y % 2 || Z % 2)) throw ArgError("up-sampled input needs even extents");
-    ConvArgs a{in, w, bias, skip, out, out32, relu_copy, B, X, Y, Z, relu ? 1 : 0, up_input ? 1 : 0, gate};
+    const size_t in_bytes = (size_t)B * (up_input ? (size_t)(X / 2) * (Y / 2) * (Z / 2) : (size_t)X * Y * Z) * cin * 2;
+    const size_t out_elems = (size_t)B * X * Y * Z * cout;
+    if (in_bytes > kInLimit || out_elems * 4 > kOutOutside)
+        throw ArgError("convolution tensors too large for 32-bit buffer offsets");
+    ConvArgs a{in, w, bias, skip, out, out32, relu_copy, B, X, Y, Z, relu ? 1 : 0, up_input ? 1 : 0, gate,
+               (unsigned)in_bytes, (unsigned)out_elems};
 #define SFP_CONV(CI, CO) \
     if (cin == CI && cout == CO) return conv_dispatch<CI, CO>(st, a);
     SFP_CONV(8, 16)
