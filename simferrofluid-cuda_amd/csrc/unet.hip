@@ -167,8 +167,17 @@ __device__ __forceinline__ void row_chunks(const v8bf (&c)[kcentres<CIN>()], con
     }
 }
 
-template <int CIN, int COUT, int PY, int WPS>
+// UP: the input is read nearest-up-sampled (a.up set) and PY is even, so y0
+// is even and rows y0 + 2k - 1 and y0 + 2k (k >= 1) read the same source row:
+// the second takes the first's fragments instead of a load.
+template <int PY, bool UP>
+constexpr int src_row(int iy) {
+    return UP && PY % 2 == 0 && iy >= 2 && iy <= PY && iy % 2 == 0 ? iy - 1 : iy;
+}
+
+template <int CIN, int COUT, int PY, int WPS, bool UP = false>
 __global__ __launch_bounds__(256, WPS) void k_conv3r(ConvArgs a, int run, int xruns) {
+    static_assert(!UP || PY % 2 == 0, "the shared source rows need an even row count");
     constexpr int KC = kchunks<CIN>();
     constexpr int NB = COUT / 16;
     if (a.gate && *a.gate != 0.0) return;
@@ -207,7 +216,7 @@ __global__ __launch_bounds__(256, WPS) void k_conv3r(ConvArgs a, int run, int xr
     // this lane's input voxel (z) and channel byte offset per chunk within an
     // (x, y) row of the (possibly half-resolution) input: A[m][k] of lane
     // m + 16 (k / 8), m = z - z0 of the output row
-    const int up = a.up;
+    const int up = UP ? 1 : a.up;
     const int Xs = up ? X >> 1 : X, Ys = up ? Y >> 1 : Y, Zs = up ? Z >> 1 : Z;
     const auto rin = buf_rsrc(a.in, a.in_bytes);
     // 16 input channels and more: one centre and one halo fragment per row
@@ -280,6 +289,7 @@ __global__ __launch_bounds__(256, WPS) void k_conv3r(ConvArgs a, int run, int xr
         for (int iy = 0; iy < PY + 2; iy++)
 #pragma unroll
             for (int q = 0; q < NC; q++) {
+                if (src_row<PY, UP>(iy) != iy) continue;  // (known after unrolling)
                 fc[iy][q] = __builtin_bit_cast(v8bf, __builtin_amdgcn_raw_buffer_load_b128(rin, po + rzc[iy][q], 0, 0));
                 if constexpr (kDpp)
                     fh[iy][q] =
@@ -301,8 +311,11 @@ __global__ __launch_bounds__(256, WPS) void k_conv3r(ConvArgs a, int run, int xr
     // accumulates in slot (t - dx) mod 3.
     v8bf fc[2][PY + 2][NC], fh[2][PY + 2][NC];
     load_plane(fc[0], fh[0], xa - 1);
-    auto step = [&](auto Pc, auto Uc, int t) {
+    // Lc: whether this step loads the next plane (with UP, every other step:
+    // planes 2k - 1 and 2k read the same source plane, see below)
+    auto step = [&](auto Pc, auto Uc, auto Lc, int t) {
         constexpr int P = decltype(Pc)::value, U = decltype(Uc)::value;
+        constexpr bool L = decltype(Lc)::value;
         const int gx = xa - 1 + t;
         // byte offset of output plane gx - 1 (the plane this step completes)
         // in the 16-bit outputs
@@ -317,15 +330,15 @@ __global__ __launch_bounds__(256, WPS) void k_conv3r(ConvArgs a, int run, int xr
                 for (int j = 0; j < 4; j++)
                     skv[oy][j] = __builtin_amdgcn_raw_buffer_load_b16(rskip, pb + ob[oy][j], 0, 0);
         }
-        load_plane(fc[1 - P], fh[1 - P], gx + 1);  // (one plane past the run at the end: unused)
+        if constexpr (L) load_plane(fc[1 - P], fh[1 - P], gx + 1);  // (one past the run at the end: unused)
 #pragma unroll
         for (int iy = 0; iy < PY + 2; iy++) {
             v8bf ck[KC];
             if constexpr (kDpp)
-                row_chunks<CIN>(fc[P][iy], fh[P][iy], g, ck);
+                row_chunks<CIN>(fc[P][src_row<PY, UP>(iy)], fh[P][src_row<PY, UP>(iy)], g, ck);
             else
 #pragma unroll
-                for (int kc = 0; kc < KC; kc++) ck[kc] = fc[P][iy][kc];
+                for (int kc = 0; kc < KC; kc++) ck[kc] = fc[P][src_row<PY, UP>(iy)][kc];
 #pragma unroll
             for (int dx = 0; dx < 3; dx++)
 #pragma unroll
@@ -372,18 +385,55 @@ __global__ __launch_bounds__(256, WPS) void k_conv3r(ConvArgs a, int run, int xr
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
     using I2 = std::integral_constant<int, 2>;
-    for (int t0 = 0; t0 < steps; t0 += 6) {  // uniform over the wavefront
-        step(I0{}, I0{}, t0);
-        if (t0 + 1 >= steps) break;
-        step(I1{}, I1{}, t0 + 1);
-        if (t0 + 2 >= steps) break;
-        step(I0{}, I2{}, t0 + 2);
-        if (t0 + 3 >= steps) break;
-        step(I1{}, I0{}, t0 + 3);
-        if (t0 + 4 >= steps) break;
-        step(I0{}, I1{}, t0 + 4);
-        if (t0 + 5 >= steps) break;
-        step(I1{}, I2{}, t0 + 5);
+    using LD = std::true_type;
+    using NL = std::false_type;
+    if constexpr (UP) {
+        // The run starts on an even plane (the launcher makes runs even), so
+        // step t's plane xa - 1 + t reads source plane (xa - 1 + t) >> 1: the
+        // same as step t - 1's for even t >= 2. Steps with even t load the
+        // next source plane, into the other buffer; odd steps load nothing,
+        // and the next step reads their buffer again. Buffer of step t:
+        // ((t + 1) >> 1) mod 2, period 4; with the accumulator slots' period
+        // 3 the march is unrolled by twelve.
+        for (int t0 = 0; t0 < steps; t0 += 12) {  // uniform over the wavefront
+            step(I0{}, I0{}, LD{}, t0);
+            if (t0 + 1 >= steps) break;
+            step(I1{}, I1{}, NL{}, t0 + 1);
+            if (t0 + 2 >= steps) break;
+            step(I1{}, I2{}, LD{}, t0 + 2);
+            if (t0 + 3 >= steps) break;
+            step(I0{}, I0{}, NL{}, t0 + 3);
+            if (t0 + 4 >= steps) break;
+            step(I0{}, I1{}, LD{}, t0 + 4);
+            if (t0 + 5 >= steps) break;
+            step(I1{}, I2{}, NL{}, t0 + 5);
+            if (t0 + 6 >= steps) break;
+            step(I1{}, I0{}, LD{}, t0 + 6);
+            if (t0 + 7 >= steps) break;
+            step(I0{}, I1{}, NL{}, t0 + 7);
+            if (t0 + 8 >= steps) break;
+            step(I0{}, I2{}, LD{}, t0 + 8);
+            if (t0 + 9 >= steps) break;
+            step(I1{}, I0{}, NL{}, t0 + 9);
+            if (t0 + 10 >= steps) break;
+            step(I1{}, I1{}, LD{}, t0 + 10);
+            if (t0 + 11 >= steps) break;
+            step(I0{}, I2{}, NL{}, t0 + 11);
+        }
+    } else {
+        for (int t0 = 0; t0 < steps; t0 += 6) {  // uniform over the wavefront
+            step(I0{}, I0{}, LD{}, t0);
+            if (t0 + 1 >= steps) break;
+            step(I1{}, I1{}, LD{}, t0 + 1);
+            if (t0 + 2 >= steps) break;
+            step(I0{}, I2{}, LD{}, t0 + 2);
+            if (t0 + 3 >= steps) break;
+            step(I1{}, I0{}, LD{}, t0 + 3);
+            if (t0 + 4 >= steps) break;
+            step(I0{}, I1{}, LD{}, t0 + 4);
+            if (t0 + 5 >= steps) break;
+            step(I1{}, I2{}, LD{}, t0 + 5);
+        }
     }
 }
 
@@ -739,6 +789,20 @@ void conv_dispatch(hipStream_t st, const ConvArgs &a) {
     const int run = (a.X + xruns - 1) / xruns;
     xruns = (a.X + run - 1) / run;
     const long long waves = base * xruns;
+    // the decoder's up-sampled inputs (32 -> 16 and 64 -> 32 channels) take
+    // the kernel that reads each shared source row once
+    // (its runs start on even planes: each loaded source plane serves two)
+    if constexpr (PY % 2 == 0 && ((CIN == 32 && COUT == 16) || (CIN == 64 && COUT == 32))) {
+        if (a.up) {
+            const int run2 = (run + 1) & ~1;
+            const int xruns2 = (a.X + run2 - 1) / run2;
+            const long long waves2 = base * xruns2;
+            hipLaunchKernelGGL((k_conv3r<CIN, COUT, PY, WPS, true>), dim3((unsigned)((waves2 + 3) / 4)), dim3(256), 0,
+                               st, a, run2, xruns2);
+            SFP_LAUNCH_CHECK();
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_conv3r<CIN, COUT, PY, WPS>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, a,
                        run, xruns);
     SFP_LAUNCH_CHECK();
