@@ -201,6 +201,7 @@ void magnetic_dev(sfp_context ctx, const double *phi, const double *const hext[3
                              c32, dirichlet);
     // the row-sum diagonal holds for the reference's boundary only
     L.pinned_rowsum = ctx->ws.rowsum_diag && !dirichlet;
+    sys.pinned_ = !dirichlet;
     sys.build_hierarchy(ctx->ws, ctx->cfg);
     SFP_HIP(hipEventRecord(ctx->ev[1], s));
     ctx->ws.phi = phi;

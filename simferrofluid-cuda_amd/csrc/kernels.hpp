@@ -67,9 +67,11 @@ void launch_restrict(hipStream_t s, Dims df, Op32 A, const float *b, const float
                      const double *gate = nullptr, float *x0c = nullptr,
                      const float *diag32c = nullptr, float c0 = 0.f);
 // Galerkin coarsening by 2x2x2 aggregation, scaled: writes the coarse FP64
-// and FP32 operators.
+// and FP32 operators, the couplings scaled by `scale` and each row's sum
+// (diagonal minus couplings) by `rs_scale` (scale: scale * P^T A P).
 void launch_coarsen(hipStream_t s, Dims df, Op64 Af, const float *diag32f, Dims dc,
-                    double scale, double *diag64c, double *const cpl64c[3], float *diag32c,
+                    double scale, double rs_scale, double *diag64c, double *const cpl64c[3],
+                    float *diag32c,
                     float *const cpl32c[3]);
 // Dense inverse of the coarsest operator (n = cells <= kMaxCoarse) into ainv (n*n floats).
 constexpr int kMaxCoarse = 64;

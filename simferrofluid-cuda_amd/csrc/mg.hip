@@ -1017,11 +1017,13 @@ __global__ __launch_bounds__(kBlock) void k_clear_inactive(ClearArgs a) {
 // lies on the upper layer -- and the 8 children are added pairwise (z through
 // the lane pair, y through LDS, x over the block's plane pairs). Same operator
 // as a per-aggregate loop over the children, up to the order of the sums.
+// The coarse couplings are the summed ones times `scale`, the coarse diagonal
+// the summed row sum times `rs_scale` plus the coarse couplings.
 constexpr int kCoarsenSums = 8;  // row sum, 3 lower, 3 upper, active count
 
 __global__ __launch_bounds__(kBlock) void k_coarsen_fine(CellGrid cg, Dims df, Op64 Af,
                                                          const float *__restrict__ diag32f, Dims dc,
-                                                         double scale, double *diag64c,
+                                                         double scale, double rs_scale, double *diag64c,
                                                          MutFaces cpl64c, float *diag32c,
                                                          float *cpl32c0, float *cpl32c1,
                                                          float *cpl32c2) {
@@ -1090,7 +1092,7 @@ __global__ __launch_bounds__(kBlock) void k_coarsen_fine(CellGrid cg, Dims df, O
                 const bool active = acc[7] > 0.0;
                 const size_t cc = dc.idx(I, J, K);
                 if (active) {
-                    const double dg = scale * acc[0] + lower[0] + upper[0] + lower[1] + upper[1] +
+                    const double dg = rs_scale * acc[0] + lower[0] + upper[0] + lower[1] + upper[1] +
                                       lower[2] + upper[2];
                     diag64c[cc] = dg;
                     diag32c[cc] = dg > 0.0 ? (float)dg : 0.f;
@@ -1119,7 +1121,8 @@ __global__ __launch_bounds__(kBlock) void k_coarsen_fine(CellGrid cg, Dims df, O
 // apart onto a zero -- so the coarse operator is the same to the bit; the
 // first 16 lanes store two coarse cells each.
 __global__ __launch_bounds__(kBlock) void k_coarsen_flat(Dims df, Op64 Af, const float *__restrict__ diag32f,
-                                                         Dims dc, double scale, double *diag64c, MutFaces cpl64c,
+                                                         Dims dc, double scale, double rs_scale,
+                                                         double *diag64c, MutFaces cpl64c,
                                                          float *diag32c, float *cpl32c0, float *cpl32c1,
                                                          float *cpl32c2) {
     typedef double d2 __attribute__((ext_vector_type(2)));
@@ -1219,7 +1222,7 @@ __global__ __launch_bounds__(kBlock) void k_coarsen_flat(Dims df, Op64 Af, const
             const size_t cc = dc.idx(I, J, K);
             if (active) {
                 const double dgc =
-                    scale * acc[h][0] + lower[0] + upper[0] + lower[1] + upper[1] + lower[2] + upper[2];
+                    rs_scale * acc[h][0] + lower[0] + upper[0] + lower[1] + upper[1] + lower[2] + upper[2];
                 diag64c[cc] = dgc;
                 diag32c[cc] = dgc > 0.0 ? (float)dgc : 0.f;
             } else {
@@ -1953,24 +1956,24 @@ void launch_restrict(hipStream_t st, Dims df, Op32 A, const float *b, const floa
 }
 
 void launch_coarsen(hipStream_t st, Dims df, Op64 Af, const float *diag32f, Dims dc,
-                    double scale, double *diag64c, double *const cpl64c[3], float *diag32c,
-                    float *const cpl32c[3]) {
+                    double scale, double rs_scale, double *diag64c, double *const cpl64c[3],
+                    float *diag32c, float *const cpl32c[3]) {
     MutFaces m{{cpl64c[0], cpl64c[1], cpl64c[2]}};
     bool flat = flat_step(df) && (uintptr_t)diag32f % 16 == 0;
     for (const double *a : {Af.diag, Af.cpl[0], Af.cpl[1], Af.cpl[2]}) flat = flat && (uintptr_t)a % 16 == 0;
     if (flat) {
         const size_t waves = (size_t)((df.nz + 63) / 64) * ((df.ny + 1) / 2) * ((df.nx + 1) / 2);
         const unsigned nb = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_coarsen_flat, dim3(nb), dim3(kBlock), 0, st, df, Af, diag32f, dc, scale, diag64c,
-                           m, diag32c, cpl32c[0], cpl32c[1], cpl32c[2]);
+        hipLaunchKernelGGL(k_coarsen_flat, dim3(nb), dim3(kBlock), 0, st, df, Af, diag32f, dc, scale, rs_scale,
+                           diag64c, m, diag32c, cpl32c[0], cpl32c[1], cpl32c[2]);
         SFP_LAUNCH_CHECK();
         return;
     }
     // over the fine grid, with even plane pairs per block (as the restriction)
     Launch L = cell_launch(df, 0, 8);
     if (L.cg.planes & 1) L = cell_launch(df, L.cg.planes + 1);
-    hipLaunchKernelGGL(k_coarsen_fine, L.grid, L.block, 0, st, L.cg, df, Af, diag32f, dc, scale, diag64c,
-                       m, diag32c, cpl32c[0], cpl32c[1], cpl32c[2]);
+    hipLaunchKernelGGL(k_coarsen_fine, L.grid, L.block, 0, st, L.cg, df, Af, diag32f, dc, scale, rs_scale,
+                       diag64c, m, diag32c, cpl32c[0], cpl32c[1], cpl32c[2]);
     SFP_LAUNCH_CHECK();
 }
 

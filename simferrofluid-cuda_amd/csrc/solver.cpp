@@ -162,8 +162,8 @@ void System::build_hierarchy(Workspace &ws, const SolverConfig &cfg) {
         Level &C = lv_[l];
         double *c64[3] = {C.cpl64[0].p, C.cpl64[1].p, C.cpl64[2].p};
         float *c32[3] = {C.cpl32[0].p, C.cpl32[1].p, C.cpl32[2].p};
-        launch_coarsen(ws.stream, F.d, F.op64(), F.diag32.p, C.d, cfg.coarse_scale, C.diag64.p, c64,
-                       C.diag32.p, c32);
+        launch_coarsen(ws.stream, F.d, F.op64(), F.diag32.p, C.d, cfg.coarse_scale,
+                       pinned_ ? 1.0 : cfg.coarse_scale, C.diag64.p, c64, C.diag32.p, c32);
     }
     // The dense inverse of the coarsest level (a one-block kernel bound by its
     // pivot steps) runs on the side stream, overlapping the clears below, the

@@ -181,6 +181,15 @@ class System {
     }
     // Number of active rows (sets ws.scalars->pad[0]); synchronous.
     long long count_active(Workspace &ws);
+    // The operator is a Neumann problem made definite by one eliminated cell
+    // (the magnetic potential's pin at cell 0). Its coarse levels then scale
+    // only the couplings by coarse_scale and keep each aggregate's row sum
+    // (the pin's eliminated couplings, in the aggregate holding cell 0, and
+    // rounding elsewhere) unscaled: the pin's conductance is that of one
+    // fine cell on every level. Scaled with the couplings it would halve per
+    // level, the coarse problems would approach the singular Neumann one,
+    // and the V-cycle would overshoot the near-constant mode.
+    bool pinned_ = false;
 
   private:
     enum Cycle { kCycleLearned, kCycleSmooth, kCycleRestrictOnly };
