@@ -494,7 +494,11 @@ SolveInfo System::fcg(Workspace &ws, const SolverConfig &cfg, int level, bool wa
     const double cells = (double)n;
     if (learned) prepare_learned(ws);
     bool b_ready = b32 != nullptr;  // lv_[level].b holds r rounded (and lv_[level].x0 its first iterate)
-    int next_batch = kBatch;        // iterations to enqueue before the host reads the residual
+    // iterations to enqueue before the host reads the residual; the first
+    // learned iteration runs alone: it decides whether the network pays,
+    // and iterations enqueued behind a stall would still launch every
+    // kernel of the learned cycle (each returning at once)
+    int next_batch = learned ? 1 : kBatch;
     // x is updated every other iteration (X_LATER, then X_BOTH): `run` counts
     // the updates since it was last current; an odd count leaves alpha p of
     // the last one pending, added before anything reads x
