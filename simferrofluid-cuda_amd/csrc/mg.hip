@@ -701,14 +701,16 @@ __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
         return dg;
     };
     // step A at plane q: x2 (returned) and d2; x1 of this row at planes q and
-    // q + 1 is read back from LDS (this wavefront wrote it). The plane's
-    // diagonal and its reciprocal (dg, rdg) are kept for step B on the same
-    // plane one iteration later.
-    auto step_a = [&](int q, f4 xm, const Early &e, f4 cxp, const Rest &r, f4 &d2, f4 &dg, f4 &rdg) {
+    // q + 1 is read back from LDS (this wavefront wrote it). Step B forms the
+    // plane's diagonal and its reciprocal again from the same operands (bit
+    // for bit the same values): carried in registers for an iteration they
+    // made the kernel spill.
+    auto step_a = [&](int q, f4 xm, const Early &e, f4 cxp, const Rest &r, f4 &d2) {
         const f4 xc = sx1[q & 1][row + 1][lane], xp = sx1[(q + 1) & 1][row + 1][lane];
         const f4 xym = sx1[q & 1][row][lane], xyp = sx1[q & 1][row + 2][lane];
         const f4 cym = scy[q & 3][row][lane], cyp = scy[q & 3][row + 1][lane];
-        dg = diag_at(q, e.cx, cxp, cym, cyp, r);
+        const f4 dg = diag_at(q, e.cx, cxp, cym, cyp, r);
+        f4 rdg;
 #pragma unroll
         for (int k = 0; k < 4; k++) rdg[k] = rcp_or_zero(dg[k]);
         d2 = z4;
@@ -729,12 +731,12 @@ __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
     put_early(i0 - 1, em);
     put_early(i0, e0);
     __syncthreads();
-    f4 dtmp, d2c, gtmp, rtmp, dgc, rdgc;
-    const f4 x2m = step_a(i0 - 1, xpre, em, e0.cx, rm, dtmp, gtmp, rtmp);
+    f4 dtmp, d2c;
+    const f4 x2m = step_a(i0 - 1, xpre, em, e0.cx, rm, dtmp);
     __syncthreads();
     put_early(i0 + 1, e1);
     __syncthreads();
-    const f4 x2c = step_a(i0, em.x, e0, e1.cx, r0, d2c, dgc, rdgc);
+    const f4 x2c = step_a(i0, em.x, e0, e1.cx, r0, d2c);
     sx2[(i0 + 2) % 3][row][lane] = x2m;  // (i0 - 1) mod 3
     sx2[i0 % 3][row][lane] = x2c;
     __syncthreads();
@@ -742,14 +744,12 @@ __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
     // every index is a constant and nothing is copied between iterations):
     // E holds x1 / cpl_x / cpl_y of planes i + 1 and i + 2 and receives plane
     // i + 3 in the slot of plane i; R the rest of planes i, i + 1 (and i + 2,
-    // loaded into the slot of plane i - 1); D2 / DG / RDG the direction, the
-    // diagonal and its reciprocal of planes i and i + 1 from step A.
+    // loaded into the slot of plane i - 1); D2 the direction of planes i and
+    // i + 1 from step A.
     Early E[3] = {e0, e1, e2};
     Rest R[3] = {r0, r1, Rest{z4, z4, z4, z4}};
-    f4 D2[3], DG[3], RDG[3];
+    f4 D2[3];
     D2[0] = d2c;
-    DG[0] = dgc;
-    RDG[0] = rdgc;
     f4 xa = e0.x;     // x1 at plane i
     f4 cxa = e0.cx;   // cpl_x at plane i
     f4 x2prev = x2m;  // x2 at plane i - 1 (this row)
@@ -764,7 +764,7 @@ __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
         // 2. x1 and cpl_y of plane i + 2 into LDS
         put_early(i + 2, E[U2]);
         // 3. step A on plane i + 1
-        const f4 x2n = step_a(i + 1, xa, E[U1], E[U2].cx, R[U1], D2[U1], DG[U1], RDG[U1]);
+        const f4 x2n = step_a(i + 1, xa, E[U1], E[U2].cx, R[U1], D2[U1]);
         sx2[(i + 1) % 3][row][lane] = x2n;
         __syncthreads();
         // 4. step B on plane i (the tile's interior rows)
@@ -774,9 +774,13 @@ __global__ __launch_bounds__(64 * kRowPairRows, 1) void k_smooth_pair_rows(
             if (active(i, amask)) {
                 const f4 xym = sx2[bi][row - 1][lane], xyp = sx2[bi][row + 1][lane];
                 const f4 cym = scy[i & 3][row][lane], cyp = scy[i & 3][row + 1][lane];
+                const f4 dgB = diag_at(i, cxa, E[U1].cx, cym, cyp, R[U]);
+                f4 rdgB;
+#pragma unroll
+                for (int k = 0; k < 4; k++) rdgB[k] = rcp_or_zero(dgB[k]);
                 f4 dn;
-                const f4 x3 = cheb(xc, x2prev, x2n, xym, xyp, cxa, E[U1].cx, cym, cyp, R[U].cz, R[U].bv, DG[U],
-                                   RDG[U], D2[U], cdB, crB, dn);
+                const f4 x3 = cheb(xc, x2prev, x2n, xym, xyp, cxa, E[U1].cx, cym, cyp, R[U].cz, R[U].bv, dgB,
+                                   rdgB, D2[U], cdB, crB, dn);
                 if (zon) *(f4 *)(xo + c0 + (unsigned)i * sx) = x3;
             }
             x2prev = xc;
