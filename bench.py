@@ -204,6 +204,47 @@ def summarize_profiles(d, n=256):
           f"| algorithmic bytes ({bpc:g} B/cell) | {alg / 1e6:.1f} MB |",
           f"| traffic / algorithmic | {(rd + wr) / alg:.3f} |"]
     open(os.path.join(out_dir, f"traffic_{name}_{n}cube.md"), "w").write("\n".join(md) + "\n")
+
+    # every hot fine-level kernel from the same trace and counter runs
+    fetch_rows = counter("prof_fetch", "FETCH_SIZE")
+    write_rows = counter("prof_write", "WRITE_SIZE")
+    table, per = [], {}
+    for key in ("update64", "matvec64", "smooth_pair", "smooth", "restrict"):
+        ksym, kbpc, _ = HOT_KERNELS[key]
+        ks = [r for r in trace if ksym in r["Kernel_Name"]]
+        if not ks:
+            continue
+        kgrid = max(int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) for r in ks)
+        kt = real([(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3 for r in ks
+                   if int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) == kgrid])
+        kf = real([float(r["Counter_Value"]) for r in fetch_rows
+                   if ksym in r["Kernel_Name"] and int(r["Grid_Size"]) == kgrid])
+        kw = real([float(r["Counter_Value"]) for r in write_rows
+                   if ksym in r["Kernel_Name"] and int(r["Grid_Size"]) == kgrid])
+        kalg = kbpc * cells
+        kavg = statistics.mean(kt)
+        krd = statistics.mean(kf) * 1024 * fs
+        kwr = statistics.mean(kw) * 1024 * ws
+        per[key] = {"kernel": ksym, "launches": len(kt), "avg_us": kavg, "bytes_per_cell": kbpc,
+                    "algorithmic_GBps": kalg / (kavg * 1e-6) / 1e9,
+                    "frac_of_8TBps": kalg / (kavg * 1e-6) / 8e12, "hbm_bytes_read": krd,
+                    "hbm_bytes_written": kwr, "traffic_over_algorithmic": (krd + kwr) / kalg,
+                    "hbm_GBps": (krd + kwr) / (kavg * 1e-6) / 1e9}
+        p = per[key]
+        table.append(f"| `{ksym}` | {kbpc:g} | {len(kt)} | {kavg:.1f} | {p['algorithmic_GBps']:.0f} | "
+                     f"{p['frac_of_8TBps']:.3f} | {krd / 1e6:.1f} / {kwr / 1e6:.1f} | "
+                     f"{p['traffic_over_algorithmic']:.3f} | {p['hbm_GBps']:.0f} |")
+    json.dump({"cells": cells, "fetch_scale": fs, "write_scale": ws, "kernels": per},
+              open(os.path.join(out_dir, f"traffic_fine_kernels_{n}.json"), "w"), indent=1)
+    fk = [f"# Fine-level kernels at {n}^3: duration and HBM traffic", "",
+          "Durations from the kernel trace, traffic from the FETCH_SIZE / WRITE_SIZE runs scaled by",
+          f"the copy probe ({fs:.4f} / {ws:.4f}); the commands are those of",
+          f"`{name}_{n}cube_trace.md` and `traffic_{name}_{n}cube.md` (magnetic chi = 2 and 8 systems,",
+          "every cell active; the launches that returned at once past convergence left out).", "",
+          "| kernel | B/cell | launches | avg us | algorithmic GB/s | of 8 TB/s | read / written MB | "
+          "traffic / algorithmic | HBM GB/s |",
+          "|---|---:|---:|---:|---:|---:|---:|---:|---:|"] + table
+    open(os.path.join(out_dir, f"fine_kernels_{n}cube.md"), "w").write("\n".join(fk) + "\n")
     ld = os.path.join(d, "prof_learned", "learned_kernel_stats.csv")
     if os.path.exists(ld):
         st = rows(ld)
