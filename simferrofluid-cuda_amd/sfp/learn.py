@@ -136,9 +136,11 @@ def harvest_system(ctx: Context, which: int, phi, rng, cg_keep=(2, 6, 16, 40), n
     res = [b * active]
     res += list(cg_residuals(diag, cpl, b, active, set(cg_keep)).values())
     for k in range(noise):
+        # white noise, then box-smoothed by 2, 1 and 4 passes in turn
         w = rng.standard_normal(b.shape) * active
-        if k % 2 == 1:
-            w = scenes._box_smooth(w, 2) * active
+        passes = (0, 2, 1, 4)[k % 4]
+        if passes:
+            w = scenes._box_smooth(w, passes) * active
         res.append(w)
     out = []
     for r in res:
@@ -176,7 +178,7 @@ def scene_seeds(count):
 
 def harvest(n=128, pressure_seeds=(100, 101, 102, 103, 104, 105, 106, 107),
             magnetic=((108, 0.5), (109, 1.0), (110, 2.0), (111, 4.0), (112, 8.0), (113, 2.0)),
-            seed=0, log=print, smoothing=None):
+            seed=0, log=print, smoothing=None, cg_keep=(2, 6, 16, 40), noise=2):
     samples = []
     rng = np.random.default_rng(seed)
     with Context(n, 1.0 / n, scenes.Geometry(n, 1.0 / n).origin, tol=1e-7, max_iters=200,
@@ -184,13 +186,13 @@ def harvest(n=128, pressure_seeds=(100, 101, 102, 103, 104, 105, 106, 107),
         for s in pressure_seeds:
             sc = drops_scene(n, s)
             ctx.pressure_project(sc, want_pressure=False)
-            got = harvest_system(ctx, 0, sc["phi"], rng)
+            got = harvest_system(ctx, 0, sc["phi"], rng, cg_keep, noise)
             samples += got
             log(f"harvest pressure seed {s}: {len(got)} samples")
         for s, chi in magnetic:
             sc = scenes.magnetic_scene(n, s, chi)
             ctx.magnetic_solve(sc, want_field=False)
-            got = harvest_system(ctx, 1, sc["phi"], rng)
+            got = harvest_system(ctx, 1, sc["phi"], rng, cg_keep, noise)
             samples += got
             log(f"harvest magnetic seed {s} chi {chi}: {len(got)} samples")
     return samples
@@ -349,6 +351,10 @@ def main(argv=None):
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--cheby", default=None, metavar="DEG:RATIO:COARSE_DEG:COARSE_RATIO",
                     help="V-cycle smoother to train under (default: the library's)")
+    ap.add_argument("--cg-keep", default="2,6,16,40", metavar="I,J,...",
+                    help="CG iterations whose residuals become samples (with the right-hand side)")
+    ap.add_argument("--noise", type=int, default=2,
+                    help="noise residuals per system (white, then box-smoothed by 2, 1, 4 passes)")
     ap.add_argument("--calibrate", type=int, default=0, metavar="N",
                     help="measure the fall-back threshold of the weights at --out on N^3 scenes "
                          "instead of training; the record goes to --calibration-out")
@@ -376,12 +382,15 @@ def main(argv=None):
         sm = dict(cheby_degree=int(v[0]), cheby_ratio=float(v[1]), coarse_cheby_degree=int(v[2]),
                   coarse_cheby_ratio=float(v[3]))
     ps, ms = scene_seeds(a.systems)
-    samples = harvest(a.n, ps, ms, seed=a.seed, log=log, smoothing=sm)
+    cg_keep = tuple(int(v) for v in a.cg_keep.split(","))
+    samples = harvest(a.n, ps, ms, seed=a.seed, log=log, smoothing=sm, cg_keep=cg_keep,
+                      noise=a.noise)
     th = time.time() - t0
     log(f"{len(samples)} samples in {th:.1f}s")
     params, losses = train(samples, a.steps, a.lr, a.seed, a.batch, log=log, crop=a.crop)
     save_weights(params, a.out, dict(n=a.n, steps=a.steps, lr=a.lr, seed=a.seed, batch=a.batch,
                                      samples=len(samples), systems=a.systems, crop=a.crop,
+                                     cg_keep=list(cg_keep), noise=a.noise,
                                      crops_seen=a.steps * a.batch if a.crop else 0,
                                      harvest_s=round(th, 1),
                                      train_s=round(time.time() - t0 - th, 1),
